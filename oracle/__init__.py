@@ -1,0 +1,163 @@
+"""CPU oracle for the exact kNN graph -- test infrastructure, never the product.
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg import this
+package. It wraps ``build/liboracle.so`` (``knn_oracle.c``), a C restatement of
+the reference's kNN semantics; each C function cites the reference lines it follows.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+EUCLIDEAN = 0
+COSINE = 1
+METRICS = {"euclidean": EUCLIDEAN, "cosine": COSINE}
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "build" / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    """Compile the oracle library (gcc); returns its path."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        build()
+    lib = ctypes.CDLL(str(_LIB_PATH))
+    f32p = ctypes.POINTER(ctypes.c_float)
+    f64p = ctypes.POINTER(ctypes.c_double)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    sz = ctypes.c_size_t
+    lib.oracle_exact_sum_f32.argtypes = [f32p, sz, sz]
+    lib.oracle_exact_sum_f32.restype = ctypes.c_double
+    lib.oracle_column_means.argtypes = [f32p, sz, sz, f64p]
+    lib.oracle_column_means.restype = None
+    lib.oracle_prepare.argtypes = [f32p, sz, sz, ctypes.c_int, f32p, f32p]
+    lib.oracle_prepare.restype = None
+    lib.oracle_knn.argtypes = [f32p, sz, sz, sz, ctypes.c_int, ctypes.c_int, u32p, sz,
+                               ctypes.c_int, u32p, f32p]
+    lib.oracle_knn.restype = ctypes.c_int
+    lib.oracle_knn_fp64.argtypes = [f32p, sz, sz, sz, ctypes.c_int, u32p, sz, ctypes.c_int,
+                                    u32p, f64p]
+    lib.oracle_knn_fp64.restype = ctypes.c_int
+    lib.baseline_expansion_knn.argtypes = [f32p, sz, sz, sz, sz, u32p, sz, ctypes.c_int, u32p,
+                                           f32p]
+    lib.baseline_expansion_knn.restype = ctypes.c_int
+    lib.oracle_default_threads.argtypes = []
+    lib.oracle_default_threads.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _f32(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if x.ndim != 2:
+        raise ValueError("embedding must be 2-D")
+    return x
+
+
+def _queries(queries, n):
+    if queries is None:
+        return None, n
+    q = np.ascontiguousarray(queries, dtype=np.uint32)
+    if q.size and int(q.max()) >= n:
+        raise ValueError("query id out of range")
+    return q, q.size
+
+
+def default_threads() -> int:
+    return int(_load().oracle_default_threads())
+
+
+def exact_sum(values) -> float:
+    v = np.ascontiguousarray(values, dtype=np.float32).ravel()
+    return float(_load().oracle_exact_sum_f32(_ptr(v, ctypes.c_float), v.size, 1))
+
+
+def column_means(x) -> np.ndarray:
+    x = _f32(x)
+    out = np.empty(x.shape[1], dtype=np.float64)
+    _load().oracle_column_means(_ptr(x, ctypes.c_float), x.shape[0], x.shape[1],
+                                _ptr(out, ctypes.c_double))
+    return out
+
+
+def prepare(x, metric: str = "euclidean") -> tuple[np.ndarray, np.ndarray]:
+    """(coordinates, f32 squared norms) every oracle distance is computed from."""
+    x = _f32(x)
+    n, d = x.shape
+    coords = np.empty((n, d), dtype=np.float32)
+    norms = np.empty(n, dtype=np.float32)
+    _load().oracle_prepare(_ptr(x, ctypes.c_float), n, d, METRICS[metric],
+                           _ptr(coords, ctypes.c_float), _ptr(norms, ctypes.c_float))
+    return coords, norms
+
+
+def knn(x, k: int, *, metric: str = "euclidean", tie_rule: bool = True, queries=None,
+        threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """Oracle kNN: (uint32 ids, float32 distances) for every row (or `queries`)."""
+    x = _f32(x)
+    n, d = x.shape
+    q, nq = _queries(queries, n)
+    idx = np.empty((nq, k), dtype=np.uint32)
+    dist = np.empty((nq, k), dtype=np.float32)
+    rc = _load().oracle_knn(_ptr(x, ctypes.c_float), n, d, k, METRICS[metric], int(tie_rule),
+                            None if q is None else _ptr(q, ctypes.c_uint32), nq, threads,
+                            _ptr(idx, ctypes.c_uint32), _ptr(dist, ctypes.c_float))
+    if rc != 0:
+        raise ValueError(f"invalid arguments: n={n} d={d} k={k}")
+    return idx, dist
+
+
+def knn_fp64(x, k: int, *, metric: str = "euclidean", queries=None,
+             threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """fp64 ground truth on the raw input: (uint32 ids, float64 distances)."""
+    x = _f32(x)
+    n, d = x.shape
+    q, nq = _queries(queries, n)
+    idx = np.empty((nq, k), dtype=np.uint32)
+    dist = np.empty((nq, k), dtype=np.float64)
+    rc = _load().oracle_knn_fp64(_ptr(x, ctypes.c_float), n, d, k, METRICS[metric],
+                                 None if q is None else _ptr(q, ctypes.c_uint32), nq, threads,
+                                 _ptr(idx, ctypes.c_uint32), _ptr(dist, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"invalid arguments: n={n} d={d} k={k}")
+    return idx, dist
+
+
+def baseline_knn(x, k: int, *, tile_rows: int = 0, queries=None,
+                 threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """The reference's CPU expansion path restated (the timed CPU baseline)."""
+    x = _f32(x)
+    n, d = x.shape
+    q, nq = _queries(queries, n)
+    idx = np.empty((nq, k), dtype=np.uint32)
+    dist = np.empty((nq, k), dtype=np.float32)
+    rc = _load().baseline_expansion_knn(_ptr(x, ctypes.c_float), n, d, k, tile_rows,
+                                        None if q is None else _ptr(q, ctypes.c_uint32), nq,
+                                        threads, _ptr(idx, ctypes.c_uint32),
+                                        _ptr(dist, ctypes.c_float))
+    if rc != 0:
+        raise ValueError(f"invalid arguments: n={n} d={d} k={k}")
+    return idx, dist
+
+
+def threads_env_default() -> int:
+    value = os.environ.get("OMP_NUM_THREADS")
+    return int(value) if value and value.isdigit() else 0

@@ -1,0 +1,617 @@
+/*
+ * CPU oracle for the exact k-nearest-neighbour graph -- TEST INFRASTRUCTURE.
+ *
+ * Nothing in the product links or calls this file: it is the checker the HIP
+ * path is compared against (tests, smoke check) and the timed CPU baseline of
+ * the benchmark. Every function cites the reference code it restates.
+ *
+ * Build: oracle/Makefile (gcc, -ffp-contract=off so that every fma below is
+ * the explicit one and every other product is rounded on its own).
+ */
+#include "knn_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+/* ------------------------------------------------------------------------ */
+/* Exact summation of f32 values.                                           */
+/*                                                                          */
+/* The reference sums each column sequentially in f64                       */
+/* (crates/scrust-core/src/neighbors.rs:146-154). A sequential f64 sum      */
+/* cannot be reproduced by a parallel device reduction, so the oracle and   */
+/* the device both take the exact sum and round it once to f64: every f32   */
+/* is an integer multiple of 2^-149, so the sum is accumulated as a         */
+/* two's-complement integer in 32-bit digits held in int64 words (carries   */
+/* are resolved at the end). The result equals the sequential sum whenever  */
+/* that sum is exact, and differs from it by at most its rounding error     */
+/* otherwise.                                                               */
+/* ------------------------------------------------------------------------ */
+
+#define ACC_WORDS 10
+
+typedef struct {
+    int64_t w[ACC_WORDS];
+} superacc;
+
+static void acc_add(superacc* a, float v) {
+    uint32_t bits;
+    memcpy(&bits, &v, sizeof bits);
+    uint32_t biased = (bits >> 23) & 0xFFu;
+    uint32_t mantissa = bits & 0x7FFFFFu;
+    uint64_t m;
+    int position; /* bit position of the mantissa's lsb, in units of 2^-149 */
+    if (biased == 0) {
+        m = mantissa;
+        position = 0;
+    } else {
+        m = mantissa | 0x800000u;
+        position = (int)biased - 1;
+    }
+    if (m == 0) return;
+    uint64_t shifted = m << (position & 31);
+    int word = position >> 5;
+    int64_t lo = (int64_t)(shifted & 0xFFFFFFFFu);
+    int64_t hi = (int64_t)(shifted >> 32);
+    if (bits >> 31) {
+        a->w[word] -= lo;
+        a->w[word + 1] -= hi;
+    } else {
+        a->w[word] += lo;
+        a->w[word + 1] += hi;
+    }
+}
+
+static double acc_to_double(const superacc* a) {
+    uint32_t digit[ACC_WORDS];
+    int64_t carry = 0;
+    for (int i = 0; i < ACC_WORDS; i++) {
+        int64_t v = a->w[i] + carry;
+        digit[i] = (uint32_t)((uint64_t)v & 0xFFFFFFFFu);
+        carry = v >> 32; /* arithmetic shift: floor division by 2^32 */
+    }
+    double sign = 1.0;
+    if (carry < 0) { /* negative: take the two's complement magnitude */
+        sign = -1.0;
+        uint64_t c = 1;
+        for (int i = 0; i < ACC_WORDS; i++) {
+            uint64_t v = (uint64_t)(~digit[i]) + c;
+            digit[i] = (uint32_t)v;
+            c = v >> 32;
+        }
+    }
+    int top = -1;
+    for (int i = ACC_WORDS - 1; i >= 0; i--) {
+        if (digit[i]) {
+            top = i * 32 + 31 - __builtin_clz(digit[i]);
+            break;
+        }
+    }
+    if (top < 0) return 0.0;
+#define BIT(b) ((b) < 0 ? 0u : (digit[(b) >> 5] >> ((b) & 31)) & 1u)
+    if (top <= 52) {
+        uint64_t v = 0;
+        for (int b = top; b >= 0; b--) v = (v << 1) | BIT(b);
+        return sign * ldexp((double)v, -149);
+    }
+    uint64_t m = 0;
+    for (int b = top; b > top - 53; b--) m = (m << 1) | BIT(b);
+    uint32_t guard = BIT(top - 53);
+    uint32_t sticky = 0;
+    for (int b = top - 54; b >= 0 && !sticky; b--) sticky |= BIT(b);
+#undef BIT
+    int exponent = top - 52 - 149;
+    if (guard && (sticky || (m & 1u))) {
+        m += 1;
+        if (m == (1ull << 53)) {
+            m >>= 1;
+            exponent += 1;
+        }
+    }
+    return sign * ldexp((double)m, exponent);
+}
+
+double oracle_exact_sum_f32(const float* values, size_t n, size_t stride) {
+    superacc a;
+    memset(&a, 0, sizeof a);
+    for (size_t i = 0; i < n; i++) acc_add(&a, values[i * stride]);
+    return acc_to_double(&a);
+}
+
+void oracle_column_means(const float* x, size_t n, size_t d, double* means) {
+    for (size_t j = 0; j < d; j++) means[j] = oracle_exact_sum_f32(x + j, n, d) / (double)n;
+}
+
+/* Centring restates neighbors.rs:144-162 (value as f64 minus the f64 mean, rounded
+ * once to f32); the squared norms restate knn.rs:148-156 (f32 fma in dimension
+ * order). Cosine has no reference counterpart: rows are divided by their f64 norm
+ * and rounded once; a zero row stays zero. */
+void oracle_prepare(const float* x, size_t n, size_t d, int metric, float* coords,
+                    float* norms) {
+    if (metric == ORACLE_COSINE) {
+        for (size_t r = 0; r < n; r++) {
+            const float* row = x + r * d;
+            double s = 0.0;
+            for (size_t j = 0; j < d; j++) s = fma((double)row[j], (double)row[j], s);
+            double length = sqrt(s);
+            float nrm = 0.0f;
+            for (size_t j = 0; j < d; j++) {
+                float c = length > 0.0 ? (float)((double)row[j] / length) : 0.0f;
+                coords[r * d + j] = c;
+                nrm = fmaf(c, c, nrm);
+            }
+            norms[r] = nrm;
+        }
+        return;
+    }
+    double* means = (double*)malloc(d * sizeof(double));
+    oracle_column_means(x, n, d, means);
+    for (size_t r = 0; r < n; r++) {
+        float nrm = 0.0f;
+        for (size_t j = 0; j < d; j++) {
+            float c = (float)((double)x[r * d + j] - means[j]);
+            coords[r * d + j] = c;
+            nrm = fmaf(c, c, nrm);
+        }
+        norms[r] = nrm;
+    }
+    free(means);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Parallel for over query rows.                                            */
+/* ------------------------------------------------------------------------ */
+
+int oracle_default_threads(void) {
+    long c = sysconf(_SC_NPROCESSORS_ONLN);
+    return c > 0 ? (int)c : 1;
+}
+
+typedef void (*row_fn)(void* ctx, size_t item, void* scratch);
+
+typedef struct {
+    row_fn fn;
+    void* ctx;
+    size_t n_items;
+    size_t scratch_bytes;
+    size_t next;
+    pthread_mutex_t lock;
+} pool_job;
+
+static void* pool_worker(void* arg) {
+    pool_job* job = (pool_job*)arg;
+    void* scratch = job->scratch_bytes ? malloc(job->scratch_bytes) : NULL;
+    for (;;) {
+        pthread_mutex_lock(&job->lock);
+        size_t item = job->next++;
+        pthread_mutex_unlock(&job->lock);
+        if (item >= job->n_items) break;
+        job->fn(job->ctx, item, scratch);
+    }
+    free(scratch);
+    return NULL;
+}
+
+static void parallel_for(row_fn fn, void* ctx, size_t n_items, size_t scratch_bytes,
+                         int threads) {
+    if (threads <= 0) threads = oracle_default_threads();
+    if ((size_t)threads > n_items) threads = (int)(n_items ? n_items : 1);
+    pool_job job = {fn, ctx, n_items, scratch_bytes, 0, PTHREAD_MUTEX_INITIALIZER};
+    if (threads == 1) {
+        pool_worker(&job);
+        return;
+    }
+    pthread_t* ids = (pthread_t*)malloc((size_t)threads * sizeof(pthread_t));
+    for (int t = 0; t < threads; t++) pthread_create(&ids[t], NULL, pool_worker, &job);
+    for (int t = 0; t < threads; t++) pthread_join(ids[t], NULL);
+    free(ids);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Keys: (squared distance, id) as one u64, ordered as (d^2, id) for d^2>=0. */
+/* ------------------------------------------------------------------------ */
+
+static inline uint32_t f32_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    return u;
+}
+
+static inline float bits_f32(uint32_t u) {
+    float f;
+    memcpy(&f, &u, sizeof f);
+    return f;
+}
+
+/* Monotone map of f32 onto u32 (total order as f32::total_cmp). */
+static inline uint32_t ordered(float f) {
+    uint32_t u = f32_bits(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+static inline float unordered(uint32_t o) {
+    return bits_f32((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+static inline uint64_t make_key(float d2, uint32_t id) {
+    return ((uint64_t)ordered(d2) << 32) | id;
+}
+
+/* Keep the `keep` smallest keys of `keys[0..n)` in ascending order in `best`;
+ * returns how many were kept. */
+static size_t smallest_keys(const uint64_t* keys, size_t n, size_t keep, uint64_t* best) {
+    size_t count = 0;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t key = keys[i];
+        if (count == keep && key >= best[keep - 1]) continue;
+        size_t slot = count < keep ? count++ : keep - 1;
+        while (slot > 0 && best[slot - 1] > key) {
+            best[slot] = best[slot - 1];
+            slot--;
+        }
+        best[slot] = key;
+    }
+    return count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The oracle.                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const float* coords;
+    const float* norms;
+    size_t n, d, k;
+    int metric, tie_rule;
+    const uint32_t* queries;
+    uint32_t* out_indices;
+    float* out_distances;
+} oracle_ctx;
+
+/* Squared distance as the Metal kernel and `cpu_knn` form it: f32 difference,
+ * f32 fma accumulation in dimension order (knn.rs:229-232, :293-297), then the
+ * snap below (d+2)*eps*(|a|^2+|b|^2) evaluated in f32 left to right
+ * (knn.rs:237-240, :298-302). */
+static inline float direct_square(const float* a, const float* b, size_t d, float na,
+                                  float nb) {
+    float s = 0.0f;
+    for (size_t j = 0; j < d; j++) {
+        float delta = a[j] - b[j];
+        s = fmaf(delta, delta, s);
+    }
+    float threshold = ((float)d + 2.0f) * FLT_EPSILON * (na + nb);
+    return s < threshold ? 0.0f : s;
+}
+
+/* The fp64 recomputation used inside a boundary tie band: f64 differences of the
+ * same f32 coordinates, f64 fma in dimension order. A pair the f32 pass snapped to
+ * zero stays at zero. */
+static inline double direct_square_f64(const float* a, const float* b, size_t d) {
+    double s = 0.0;
+    for (size_t j = 0; j < d; j++) {
+        double delta = (double)a[j] - (double)b[j];
+        s = fma(delta, delta, s);
+    }
+    return s;
+}
+
+typedef struct {
+    double d64;
+    uint32_t id;
+    float d2;
+} band_entry;
+
+static int band_cmp(const void* lhs, const void* rhs) {
+    const band_entry* a = (const band_entry*)lhs;
+    const band_entry* b = (const band_entry*)rhs;
+    if (a->d64 < b->d64) return -1;
+    if (a->d64 > b->d64) return 1;
+    return a->id < b->id ? -1 : (a->id > b->id ? 1 : 0);
+}
+
+/* The fp64 boundary near-tie rule (no reference counterpart; shared verbatim in
+ * meaning with the device rerank). With the sorted keys s_0 <= s_1 <= ... and
+ * t = d2(s_{k-1}), tau = t*(d+2)*2^-23: if d2(s_k) - t <= tau, the band is every
+ * position j with |d2(s_j) - t| <= tau; positions before the band stay, and the
+ * band's slots up to k-1 take the band members in (fp64 d^2, id) order.
+ * Returns 1 when the kept keys may not hold the whole band. */
+static int apply_tie_rule(const uint64_t* best, size_t kept, size_t k, size_t n_candidates,
+                          const float* coords, size_t d, const float* q, uint32_t* ids,
+                          float* d2s) {
+    double t = (double)unordered((uint32_t)(best[k - 1] >> 32));
+    double tau = t * (double)(d + 2) * 0x1p-23;
+    if (kept <= k) return 0;
+    double next = (double)unordered((uint32_t)(best[k] >> 32));
+    if (next - t > tau) return 0;
+    size_t lo = k - 1;
+    while (lo > 0 && t - (double)unordered((uint32_t)(best[lo - 1] >> 32)) <= tau) lo--;
+    size_t hi = k;
+    while (hi + 1 < kept && (double)unordered((uint32_t)(best[hi + 1] >> 32)) - t <= tau) hi++;
+    if (hi + 1 == kept && kept < n_candidates) return 1;
+    size_t m = hi - lo + 1;
+    band_entry* band = (band_entry*)malloc(m * sizeof(band_entry));
+    for (size_t j = 0; j < m; j++) {
+        uint32_t id = (uint32_t)best[lo + j];
+        float d2 = unordered((uint32_t)(best[lo + j] >> 32));
+        band[j].id = id;
+        band[j].d2 = d2;
+        band[j].d64 = d2 == 0.0f ? 0.0 : direct_square_f64(q, coords + (size_t)id * d, d);
+    }
+    qsort(band, m, sizeof(band_entry), band_cmp);
+    for (size_t j = lo; j < k; j++) {
+        ids[j] = band[j - lo].id;
+        d2s[j] = band[j - lo].d2;
+    }
+    free(band);
+    return 0;
+}
+
+static void oracle_row(void* vctx, size_t item, void* scratch) {
+    oracle_ctx* c = (oracle_ctx*)vctx;
+    size_t q = c->queries ? c->queries[item] : item;
+    size_t n = c->n, d = c->d, k = c->k;
+    uint64_t* keys = (uint64_t*)scratch;
+    const float* qrow = c->coords + q * d;
+    size_t m = 0;
+    for (size_t r = 0; r < n; r++) {
+        if (r == q) continue; /* a cell is not its own neighbour (knn.rs:224-226) */
+        float s = direct_square(qrow, c->coords + r * d, d, c->norms[q], c->norms[r]);
+        keys[m++] = make_key(s, (uint32_t)r);
+    }
+    uint32_t ids[4096];
+    float d2s[4096];
+    size_t keep = k + 32;
+    for (;;) {
+        if (keep > m) keep = m;
+        uint64_t* best = (uint64_t*)malloc(keep * sizeof(uint64_t));
+        size_t kept = smallest_keys(keys, m, keep, best);
+        for (size_t j = 0; j < k; j++) {
+            ids[j] = (uint32_t)best[j];
+            d2s[j] = unordered((uint32_t)(best[j] >> 32));
+        }
+        int again = c->tie_rule
+                        ? apply_tie_rule(best, kept, k, m, c->coords, d, qrow, ids, d2s)
+                        : 0;
+        free(best);
+        if (!again) break;
+        keep *= 2;
+    }
+    for (size_t j = 0; j < k; j++) {
+        c->out_indices[item * k + j] = ids[j];
+        c->out_distances[item * k + j] =
+            c->metric == ORACLE_COSINE ? 0.5f * d2s[j] : sqrtf(d2s[j]);
+    }
+}
+
+int oracle_knn(const float* x, size_t n, size_t d, size_t k, int metric, int tie_rule,
+               const uint32_t* queries, size_t n_queries, int threads, uint32_t* out_indices,
+               float* out_distances) {
+    if (n == 0 || d == 0 || k == 0 || k >= n || k > 4096) return -1;
+    if (!queries) n_queries = n;
+    float* coords = (float*)malloc(n * d * sizeof(float));
+    float* norms = (float*)malloc(n * sizeof(float));
+    oracle_prepare(x, n, d, metric, coords, norms);
+    oracle_ctx c = {coords, norms, n, d, k, metric, tie_rule, queries, out_indices,
+                    out_distances};
+    parallel_for(oracle_row, &c, n_queries, n * sizeof(uint64_t), threads);
+    free(coords);
+    free(norms);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* fp64 ground truth (tests/test_neighbors_audit.py:52-61).                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const double* x;
+    size_t n, d, k;
+    int metric;
+    const uint32_t* queries;
+    uint32_t* out_indices;
+    double* out_distances;
+} fp64_ctx;
+
+typedef struct {
+    double v;
+    uint32_t id;
+} fp64_key;
+
+static void fp64_row(void* vctx, size_t item, void* scratch) {
+    fp64_ctx* c = (fp64_ctx*)vctx;
+    size_t q = c->queries ? c->queries[item] : item;
+    size_t n = c->n, d = c->d, k = c->k;
+    fp64_key* best = (fp64_key*)scratch;
+    size_t count = 0;
+    const double* a = c->x + q * d;
+    for (size_t r = 0; r < n; r++) {
+        if (r == q) continue;
+        const double* b = c->x + r * d;
+        double v;
+        if (c->metric == ORACLE_COSINE) {
+            double dot = 0.0;
+            for (size_t j = 0; j < d; j++) dot += a[j] * b[j];
+            v = 1.0 - dot; /* rows were scaled to unit norm */
+        } else {
+            v = 0.0;
+            for (size_t j = 0; j < d; j++) {
+                double delta = a[j] - b[j];
+                v += delta * delta;
+            }
+        }
+        if (count == k && (v > best[k - 1].v || (v == best[k - 1].v && r > best[k - 1].id)))
+            continue;
+        size_t slot = count < k ? count++ : k - 1;
+        while (slot > 0 && (best[slot - 1].v > v ||
+                            (best[slot - 1].v == v && best[slot - 1].id > (uint32_t)r))) {
+            best[slot] = best[slot - 1];
+            slot--;
+        }
+        best[slot].v = v;
+        best[slot].id = (uint32_t)r;
+    }
+    for (size_t j = 0; j < k; j++) {
+        c->out_indices[item * k + j] = best[j].id;
+        c->out_distances[item * k + j] =
+            c->metric == ORACLE_COSINE ? best[j].v : sqrt(best[j].v > 0 ? best[j].v : 0.0);
+    }
+}
+
+int oracle_knn_fp64(const float* x, size_t n, size_t d, size_t k, int metric,
+                    const uint32_t* queries, size_t n_queries, int threads,
+                    uint32_t* out_indices, double* out_distances) {
+    if (n == 0 || d == 0 || k == 0 || k >= n) return -1;
+    if (!queries) n_queries = n;
+    double* x64 = (double*)malloc(n * d * sizeof(double));
+    for (size_t r = 0; r < n; r++) {
+        double s = 0.0;
+        for (size_t j = 0; j < d; j++) {
+            x64[r * d + j] = (double)x[r * d + j];
+            s += x64[r * d + j] * x64[r * d + j];
+        }
+        if (metric == ORACLE_COSINE) {
+            double length = sqrt(s);
+            for (size_t j = 0; j < d; j++) x64[r * d + j] = length > 0 ? x64[r * d + j] / length : 0.0;
+        }
+    }
+    fp64_ctx c = {x64, n, d, k, metric, queries, out_indices, out_distances};
+    parallel_for(fp64_row, &c, n_queries, k * sizeof(fp64_key), threads);
+    free(x64);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The reference CPU path restated (timed baseline, not the oracle).        */
+/* crates/scrust-core/src/neighbors.rs:29-101,127-129,144-191               */
+/* ------------------------------------------------------------------------ */
+
+#define BASE_QB 8    /* query rows that share one pass over the references */
+#define BASE_CH 256  /* references per chunk */
+
+typedef struct {
+    const float* points;   /* centred, row-major n x d */
+    const float* points_t; /* centred, d x n (the reference's points_t) */
+    const float* norms;
+    size_t n, d, k, tile_rows;
+    const uint32_t* queries;
+    size_t n_queries;
+    uint32_t* out_indices;
+    float* out_distances;
+} base_ctx;
+
+/* One tile of query rows (neighbors.rs:69-97): raw = (tile . P^T) * -2 + (|a|^2 + |b|^2);
+ * floor = (|a|^2 + |b|^2) * (d+2)*eps; raw < floor -> 0; then per row the k smallest
+ * by (max(d^2, 0), id) excluding the row itself, and sqrt. The tile is consumed in
+ * chunks instead of being materialised whole; the values and the selection are the
+ * same. */
+static void base_tile(void* vctx, size_t tile, void* scratch) {
+    base_ctx* c = (base_ctx*)vctx;
+    size_t n = c->n, d = c->d, k = c->k;
+    size_t begin = tile * c->tile_rows;
+    size_t end = begin + c->tile_rows;
+    if (end > c->n_queries) end = c->n_queries;
+    uint64_t* best = (uint64_t*)scratch; /* BASE_QB x k */
+    float resolution = ((float)d + 2.0f) * FLT_EPSILON; /* neighbors.rs:127-129 */
+    for (size_t q0 = begin; q0 < end; q0 += BASE_QB) {
+        size_t qb = end - q0 < BASE_QB ? end - q0 : BASE_QB;
+        size_t rows[BASE_QB];
+        float qv[BASE_QB][512];
+        size_t count[BASE_QB];
+        for (size_t i = 0; i < qb; i++) {
+            rows[i] = c->queries ? c->queries[q0 + i] : q0 + i;
+            for (size_t j = 0; j < d; j++) qv[i][j] = c->points[rows[i] * d + j];
+            count[i] = 0;
+        }
+        for (size_t r0 = 0; r0 < n; r0 += BASE_CH) {
+            size_t ch = n - r0 < BASE_CH ? n - r0 : BASE_CH;
+            float acc[BASE_QB][BASE_CH];
+            memset(acc, 0, sizeof acc);
+            for (size_t j = 0; j < d; j++) {
+                const float* prow = c->points_t + j * n + r0;
+                for (size_t i = 0; i < qb; i++) {
+                    float v = qv[i][j];
+                    float* a = acc[i];
+                    for (size_t r = 0; r < ch; r++) a[r] += v * prow[r];
+                }
+            }
+            for (size_t i = 0; i < qb; i++) {
+                float nq = c->norms[rows[i]];
+                uint64_t* list = best + i * k;
+                for (size_t r = 0; r < ch; r++) {
+                    size_t col = r0 + r;
+                    if (col == rows[i]) continue;
+                    float sums = nq + c->norms[col];
+                    float raw = acc[i][r] * -2.0f + 0.0f;
+                    raw = raw + sums;
+                    float floor_ = sums * resolution + 0.0f;
+                    float sq = raw < floor_ ? 0.0f : raw;
+                    if (!(sq > 0.0f)) sq = 0.0f; /* square.max(0.0) */
+                    uint64_t key = make_key(sq, (uint32_t)col);
+                    if (count[i] == k && key >= list[k - 1]) continue;
+                    size_t slot = count[i] < k ? count[i]++ : k - 1;
+                    while (slot > 0 && list[slot - 1] > key) {
+                        list[slot] = list[slot - 1];
+                        slot--;
+                    }
+                    list[slot] = key;
+                }
+            }
+        }
+        for (size_t i = 0; i < qb; i++) {
+            for (size_t j = 0; j < k; j++) {
+                uint64_t key = best[i * k + j];
+                c->out_indices[(q0 + i) * k + j] = (uint32_t)key;
+                c->out_distances[(q0 + i) * k + j] = sqrtf(unordered((uint32_t)(key >> 32)));
+            }
+        }
+    }
+}
+
+int baseline_expansion_knn(const float* x, size_t n, size_t d, size_t k, size_t tile_rows,
+                           const uint32_t* queries, size_t n_queries, int threads,
+                           uint32_t* out_indices, float* out_distances) {
+    if (n == 0 || d == 0 || k == 0 || k >= n || d > 512) return -1;
+    if (!queries) n_queries = n;
+    if (tile_rows == 0) {
+        /* neighbors.rs:17-31: tiles of MAX_TILE_ELEMENTS / n rows */
+        tile_rows = (64u * 1024u * 1024u) / n;
+        if (tile_rows == 0) tile_rows = 1;
+    }
+    /* Parallel work is a slice of a tile so every thread has rows to select in, as
+     * the reference's rayon iterator does within one tile. */
+    size_t slice = tile_rows;
+    int t = threads > 0 ? threads : oracle_default_threads();
+    if (slice > (n_queries + (size_t)t - 1) / (size_t)t)
+        slice = (n_queries + (size_t)t - 1) / (size_t)t;
+    if (slice < BASE_QB) slice = BASE_QB;
+    /* Centring: sequential f64 sums as in neighbors.rs:144-162. */
+    double* means = (double*)calloc(d, sizeof(double));
+    for (size_t r = 0; r < n; r++)
+        for (size_t j = 0; j < d; j++) means[j] += (double)x[r * d + j];
+    for (size_t j = 0; j < d; j++) means[j] /= (double)n;
+    float* points = (float*)malloc(n * d * sizeof(float));
+    float* points_t = (float*)malloc(n * d * sizeof(float));
+    float* norms = (float*)malloc(n * sizeof(float));
+    for (size_t r = 0; r < n; r++) {
+        float s = 0.0f; /* points.sqr().sum_keepdim(1) */
+        for (size_t j = 0; j < d; j++) {
+            float c = (float)((double)x[r * d + j] - means[j]);
+            points[r * d + j] = c;
+            points_t[j * n + r] = c;
+            s += c * c;
+        }
+        norms[r] = s;
+    }
+    free(means);
+    base_ctx c = {points, points_t, norms, n, d, k, slice, queries, n_queries,
+                  out_indices, out_distances};
+    size_t tiles = (n_queries + slice - 1) / slice;
+    parallel_for(base_tile, &c, tiles, BASE_QB * k * sizeof(uint64_t), t);
+    free(points);
+    free(points_t);
+    free(norms);
+    return 0;
+}

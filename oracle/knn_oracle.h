@@ -1,0 +1,69 @@
+/*
+ * CPU oracle for the exact k-nearest-neighbour graph (test infrastructure only).
+ *
+ * This library is the checker for the HIP path, never part of it: only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load it.
+ *
+ * Semantics restated from the reference:
+ *   - centring: crates/scrust-core/src/neighbors.rs:131-162 (f64 column means,
+ *     one f32 rounding per coordinate) and its GPU twin
+ *     crates/scrust-gpu/src/kernels/knn.rs:127-158 (f32 fma squared norms);
+ *   - distances, snapping, order: the `cpu_knn` oracle the Metal kernel is held
+ *     to, crates/scrust-gpu/src/kernels/knn.rs:277-313 (direct difference with
+ *     fma in dimension order, snap below (d+2)*eps*(|a|^2+|b|^2), sort by
+ *     (d^2, id)), with the selection of crates/scrust-core/src/neighbors.rs:171-191;
+ *   - fp64 truth: tests/test_neighbors_audit.py:52-61;
+ *   - CPU baseline (timed, not the oracle): crates/scrust-core/src/neighbors.rs:29-101.
+ */
+#ifndef SCRUST_KNN_ORACLE_H
+#define SCRUST_KNN_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_EUCLIDEAN = 0, ORACLE_COSINE = 1 };
+
+/* Correctly rounded f64 value of the exact sum of n f32 values read with a stride. */
+double oracle_exact_sum_f32(const float* values, size_t n, size_t stride);
+
+/* Column means: exact sum of each column, rounded once to f64, divided by n in f64. */
+void oracle_column_means(const float* x, size_t n, size_t d, double* means);
+
+/* The coordinates every distance is computed from, plus their f32 fma squared norms.
+ * Euclidean: centred columns. Cosine: rows scaled to unit f64 norm, not centred. */
+void oracle_prepare(const float* x, size_t n, size_t d, int metric, float* coords,
+                    float* norms);
+
+/* Exact kNN under the oracle's semantics for the rows listed in `queries`
+ * (all rows when queries == NULL). tie_rule != 0 applies the fp64 boundary
+ * near-tie rule. Output rows follow the order of `queries`.
+ * Returns 0, or -1 on invalid arguments. */
+int oracle_knn(const float* x, size_t n, size_t d, size_t k, int metric, int tie_rule,
+               const uint32_t* queries, size_t n_queries, int threads, uint32_t* out_indices,
+               float* out_distances);
+
+/* Ground truth in f64 on the raw input (no centring): Euclidean distance, or
+ * cosine distance 1 - x.y/(|x||y|). Ties by the smaller id. */
+int oracle_knn_fp64(const float* x, size_t n, size_t d, size_t k, int metric,
+                    const uint32_t* queries, size_t n_queries, int threads,
+                    uint32_t* out_indices, double* out_distances);
+
+/* The reference's CPU path, restated: f64-centred coordinates, the expansion
+ * |a|^2+|b|^2-2a.b over tiles of tile_rows query rows, snapping at the
+ * expansion's resolution, selection by (d^2, id). Used as the timed CPU baseline. */
+int baseline_expansion_knn(const float* x, size_t n, size_t d, size_t k, size_t tile_rows,
+                           const uint32_t* queries, size_t n_queries, int threads,
+                           uint32_t* out_indices, float* out_distances);
+
+/* Threads used when `threads` <= 0: every online host core. */
+int oracle_default_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
