@@ -1,0 +1,172 @@
+"""scrust_hip: the exact k-nearest-neighbour graph behind scrust's ``pp.neighbors`` on
+AMD Instinct MI355X (gfx950).
+
+Python mirror of the drop-in boundary. ``knn(embedding, k, device)`` keeps the
+signature, argument checks and outputs of the reference's ``_scrust.knn``
+(crates/scrust-py/src/embedding.rs:60-77): a 2-D float32 numpy array in, a pair of
+(uint32 ids, float32 distances) arrays of shape (n, k) out, nearest first, ties to the
+smaller id. Work runs in libscrust_hip.so (hand-written HIP kernels); there is no CPU
+fallback in this package.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _native
+from ._native import KnnOptions, KnnStats, ScrustHipError
+
+__all__ = [
+    "Context",
+    "KnnStats",
+    "ScrustHipError",
+    "gpu_available",
+    "knn",
+    "knn_hip",
+    "knn_device",
+    "max_supported_k",
+]
+
+DEVICES = ("auto", "gpu", "hip")
+
+
+def gpu_available() -> bool:
+    """True when a gfx950 device is usable (crates/scrust-py/src/lib.rs:37-40)."""
+    return bool(_native.load().scrust_hip_gpu_available())
+
+
+class Context:
+    """A device context: one HIP stream plus a grow-only device workspace.
+
+    Mirrors ``MetalContext`` (crates/scrust-gpu/src/context.rs:12-90): construction fails
+    with ``ScrustHipError`` (status NO_GPU) when no usable device exists. Use one context
+    from one thread at a time.
+    """
+
+    def __init__(self, device: int = 0):
+        lib = _native.load()
+        handle = ctypes.c_void_p()
+        _native.raise_for(lib.scrust_hip_context_create(int(device), ctypes.byref(handle)))
+        self._handle = handle
+        self.device = int(device)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._handle is None:
+            raise ScrustHipError("context is closed")
+        return self._handle
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None:
+            _native.load().scrust_hip_context_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def max_supported_k(self) -> int:
+        return int(_native.load().scrust_hip_max_supported_k(self.handle))
+
+
+_local = threading.local()
+
+
+def _thread_context() -> Context:
+    """One context per thread, built on first use (crates/scrust-py/src/embedding.rs:22-32)."""
+    ctx = getattr(_local, "context", None)
+    if ctx is None:
+        ctx = Context(0)
+        _local.context = ctx
+    return ctx
+
+
+def _handle_or_none(context: Context | None):
+    if context is not None:
+        return context.handle
+    try:
+        return _thread_context().handle
+    except ScrustHipError:
+        return None
+
+
+def max_supported_k(context: Context | None = None) -> int:
+    """Largest k any path serves (crates/scrust-gpu/src/kernels/knn.rs:21-29)."""
+    return int(_native.load().scrust_hip_max_supported_k(
+        None if context is None else context.handle))
+
+
+def _embedding(embedding) -> np.ndarray:
+    """Dtype and rank checks of convert.rs:70-93; a strided view is copied."""
+    if not isinstance(embedding, np.ndarray) or embedding.ndim != 2 or embedding.dtype != np.float32:
+        raise ValueError("embedding must be a 2-D numpy array of float32")
+    return np.ascontiguousarray(embedding)
+
+
+def _options(metric: str, precision: str) -> KnnOptions:
+    if metric not in _native.METRICS:
+        raise ValueError(f"metric must be one of {sorted(_native.METRICS)}, got {metric!r}")
+    if precision not in _native.PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_native.PRECISIONS)}, got {precision!r}")
+    return KnnOptions(_native.METRICS[metric], _native.PRECISIONS[precision], 0, 0)
+
+
+def knn_hip(embedding, k: int, *, metric: str = "euclidean", precision: str = "bf16",
+            context: Context | None = None, stats: bool = False):
+    """kNN on the device from host memory. Returns (ids, distances) or, with
+    ``stats=True``, (ids, distances, stats dict)."""
+    x = _embedding(embedding)
+    if isinstance(k, bool) or int(k) != k or k < 0:
+        raise ValueError(f"k must be a non-negative integer, got {k!r}")
+    k = int(k)
+    opts = _options(metric, precision)
+    handle = _handle_or_none(context)
+    n, d = x.shape
+    ids = np.empty((n, k), dtype=np.uint32)
+    dist = np.empty((n, k), dtype=np.float32)
+    st = KnnStats()
+    # Without a device the library still checks the arguments first, so argument
+    # errors are reported as such and only valid calls report NO_GPU.
+    status = _native.load().scrust_hip_knn_ex(
+        handle, x.ctypes.data, n, d, k, ctypes.byref(opts), ids.ctypes.data,
+        dist.ctypes.data, ctypes.byref(st))
+    _native.raise_for(status)
+    if stats:
+        return ids, dist, st.as_dict()
+    return ids, dist
+
+
+def knn(embedding, k: int, device: str = "auto"):
+    """Drop-in for ``_scrust.knn(embedding, k, device)`` (embedding.rs:60-77)."""
+    if device not in DEVICES:
+        if device in ("cpu", "metal"):
+            raise ValueError(
+                f"device {device!r} is served by scrust's own core; this package is its HIP path")
+        raise ValueError(f"device must be one of auto, gpu, hip, got {device}")
+    return knn_hip(embedding, k)
+
+
+def knn_device(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int, distances_ptr: int,
+               *, metric: str = "euclidean", precision: str = "bf16",
+               context: Context | None = None) -> dict:
+    """kNN on device-resident buffers (raw device pointers, e.g. ``tensor.data_ptr()``).
+    Returns the call's stats."""
+    opts = _options(metric, precision)
+    ctx = context if context is not None else _thread_context()
+    st = KnnStats()
+    status = _native.load().scrust_hip_knn_device(
+        ctx.handle, ctypes.c_void_p(embedding_ptr), n, d, k, ctypes.byref(opts),
+        ctypes.c_void_p(indices_ptr), ctypes.c_void_p(distances_ptr), ctypes.byref(st))
+    _native.raise_for(status)
+    return st.as_dict()

@@ -1,0 +1,127 @@
+"""ctypes binding of libscrust_hip.so (the C ABI in include/scrust_hip.h).
+
+The library is built in-tree (``mrustsc_amd/lib/libscrust_hip.so``); importing this
+module fails loudly when it is missing -- there is no fallback implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libscrust_hip.so"
+
+OK = 0
+SHAPE = 1
+INVALID_PARAMETER = 2
+NO_GPU = 3
+KERNEL = 4
+OUT_OF_MEMORY = 5
+
+EUCLIDEAN = 0
+COSINE = 1
+METRICS = {"euclidean": EUCLIDEAN, "cosine": COSINE}
+
+BF16 = 0
+FP32 = 1
+EXACT = 2
+PRECISIONS = {"bf16": BF16, "fp32": FP32, "exact": EXACT}
+PRECISION_NAMES = {v: k for k, v in PRECISIONS.items()}
+
+# Every entry point include/scrust_hip.h declares.
+EXPORTS = (
+    "scrust_hip_last_error",
+    "scrust_hip_gpu_available",
+    "scrust_hip_context_create",
+    "scrust_hip_context_destroy",
+    "scrust_hip_max_supported_k",
+    "scrust_hip_knn",
+    "scrust_hip_knn_ex",
+    "scrust_hip_knn_device",
+)
+
+
+class KnnOptions(ctypes.Structure):
+    _fields_ = [
+        ("metric", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class KnnStats(ctypes.Structure):
+    _fields_ = [
+        ("prepare_ms", ctypes.c_double),
+        ("pass_ms", ctypes.c_double),
+        ("rerank_ms", ctypes.c_double),
+        ("fallback_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("fallback_rows", ctypes.c_uint64),
+        ("candidates", ctypes.c_uint64),
+        ("pass_flops", ctypes.c_uint64),
+        ("path", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {name: getattr(self, name) for name, _ in self._fields_}
+        out["path"] = PRECISION_NAMES.get(out["path"], out["path"])
+        return out
+
+
+class ScrustHipError(RuntimeError):
+    """A failure reported by the library that is not an argument error."""
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `make -C mrustsc_amd/csrc`"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp = ctypes.c_void_p
+    sz = ctypes.c_size_t
+    lib.scrust_hip_last_error.argtypes = []
+    lib.scrust_hip_last_error.restype = ctypes.c_char_p
+    lib.scrust_hip_gpu_available.argtypes = []
+    lib.scrust_hip_gpu_available.restype = ctypes.c_int
+    lib.scrust_hip_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.scrust_hip_context_create.restype = ctypes.c_int
+    lib.scrust_hip_context_destroy.argtypes = [vp]
+    lib.scrust_hip_context_destroy.restype = None
+    lib.scrust_hip_max_supported_k.argtypes = [vp]
+    lib.scrust_hip_max_supported_k.restype = sz
+    lib.scrust_hip_knn.argtypes = [vp, vp, sz, sz, sz, vp, vp]
+    lib.scrust_hip_knn.restype = ctypes.c_int
+    opts = ctypes.POINTER(KnnOptions)
+    stats = ctypes.POINTER(KnnStats)
+    lib.scrust_hip_knn_ex.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
+    lib.scrust_hip_knn_ex.restype = ctypes.c_int
+    lib.scrust_hip_knn_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
+    lib.scrust_hip_knn_device.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().scrust_hip_last_error().decode("utf-8", "replace")
+
+
+def raise_for(status: int) -> None:
+    """Map a status onto the exception the reference's bindings raise
+    (crates/scrust-py/src/lib.rs:24-35): Shape / InvalidParameter -> ValueError,
+    everything else -> RuntimeError."""
+    if status == OK:
+        return
+    message = last_error()
+    if status in (SHAPE, INVALID_PARAMETER):
+        raise ValueError(message)
+    raise ScrustHipError(f"[status {status}] {message}")

@@ -1,0 +1,248 @@
+"""Parity of the HIP kNN path with the CPU oracle, through the C ABI.
+
+Mirrors the reference's GPU tests (crates/scrust-gpu/src/kernels/knn.rs:327-424), its
+device-parity file (tests/test_device_parity.py) and the kNN audits
+(tests/test_neighbors_audit.py:137-153, :340-387). The bar is stricter than the
+reference's: indices AND distances must equal the oracle's bit for bit, because both
+sides form every distance with the same f32 operations.
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mrustsc_amd
+import oracle
+from mrustsc_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_known_answers.json").read_text())
+PRECISIONS = ("bf16", "fp32", "exact")
+
+
+def run(context, x, k, precision="bf16", metric="euclidean", stats=False):
+    print(f"[case] n={x.shape[0]} d={x.shape[1]} k={k} precision={precision} metric={metric}",
+          flush=True)
+    return mrustsc_amd.knn_hip(x, k, precision=precision, metric=metric, context=context,
+                               stats=stats)
+
+
+def assert_matches_oracle(context, x, k, precision, metric="euclidean", queries=None):
+    ids, dist = run(context, x, k, precision, metric)
+    want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=queries)
+    if queries is not None:
+        ids, dist = ids[queries], dist[queries]
+    np.testing.assert_array_equal(ids, want_ids, err_msg=f"{precision}: neighbour ids differ")
+    np.testing.assert_array_equal(dist, want_dist, err_msg=f"{precision}: distances differ")
+    return ids, dist
+
+
+# --- crates/scrust-gpu/src/kernels/knn.rs tests -------------------------------------------
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize(("n", "d", "k"), [(64, 8, 1), (100, 5, 7), (257, 3, 15), (300, 32, 16)])
+def test_matches_the_oracle_on_random_embeddings(context, precision, n, d, k):
+    x = synthetic.random_embedding(n, d, n)
+    assert_matches_oracle(context, x, k, precision)
+
+
+def test_matches_the_oracle_at_the_largest_supported_k(context):
+    k = mrustsc_amd.max_supported_k(context)
+    x = synthetic.random_embedding(k + 71, 4, 7)
+    assert_matches_oracle(context, x, k, "bf16")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_finds_the_obvious_neighbours_of_points_on_a_line(context, precision):
+    ids, dist = run(context, synthetic.line(50), 2, precision)
+    case = GOLDEN["line50_k2"]
+    for row, want in case["rows"].items():
+        assert ids[int(row)].tolist() == want
+    for row, want in case["distances"].items():
+        assert dist[int(row)].tolist() == want
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_duplicated_points_break_ties_by_the_smaller_index(context, precision):
+    x = np.zeros((41, 2), dtype=np.float32)
+    x[40, 0] = 100.0
+    ids, _ = run(context, x, 3, precision)
+    for row, want in GOLDEN["origin40_k3"]["rows"].items():
+        assert ids[int(row)].tolist() == want
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_two_runs_produce_identical_output(context, precision):
+    x = synthetic.random_embedding(500, 6, 11)
+    a = run(context, x, 9, precision)
+    b = run(context, x, 9, precision)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_rejects_parameters_it_cannot_serve(context):
+    x = synthetic.random_embedding(8, 3, 3)
+    for k in (0, 8, 9):
+        with pytest.raises(ValueError):
+            run(context, x, k)
+    wide = synthetic.random_embedding(4096, 2, 4)
+    with pytest.raises(ValueError):
+        run(context, wide, mrustsc_amd.max_supported_k(context) + 1)
+
+
+def test_rejects_non_finite_values(context):
+    x = synthetic.random_embedding(50, 3, 3)
+    x[7, 1] = np.nan
+    with pytest.raises(ValueError):
+        run(context, x, 4)
+
+
+# --- crates/scrust-core/src/neighbors.rs known answers --------------------------------------
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_neighbours_on_a_line_and_a_grid(context, precision):
+    ids, dist = run(context, synthetic.line(6), 2, precision)
+    for row, want in GOLDEN["line6_k2"]["rows"].items():
+        assert ids[int(row)].tolist() == want
+    assert dist[0].tolist() == GOLDEN["line6_k2"]["distances"]["0"]
+    ids, dist = run(context, synthetic.grid(3), 4, precision)
+    for row, want in GOLDEN["grid3_k4"]["rows"].items():
+        assert ids[int(row)].tolist() == want
+    np.testing.assert_allclose(dist[0], GOLDEN["grid3_k4"]["corner_distances"], atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_matches_scanpy_neighbour_sets(context, precision):
+    case = GOLDEN["scanpy"]
+    x = synthetic.reference_embedding(case["n_cells"], case["n_dims"], case["seed"])
+    ids, _ = assert_matches_oracle(context, x, case["k"], precision)
+    for cell, want in zip(case["sample_cells"], case["neighbors"]):
+        shared = len(set(ids[cell].tolist()) & set(want))
+        assert shared * 10 >= case["k"] * 9, f"cell {cell}: only {shared} shared"
+
+
+# --- tests/test_device_parity.py ---------------------------------------------------------
+
+
+def _normal(n, d, seed):
+    return np.random.default_rng(seed).normal(size=(n, d)).astype(np.float32)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_duplicate_cells_are_at_distance_zero(context, precision):
+    x = _normal(200, 6, 9)
+    x[:20] = x[0]
+    _, dist = assert_matches_oracle(context, x, 14, precision)
+    assert np.count_nonzero(dist[:20]) == 0
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize(("n", "d", "seed"), [(400, 20, 3), (250, 50, 11)])
+def test_the_graph_matches_the_oracle_on_normal_data(context, precision, n, d, seed):
+    assert_matches_oracle(context, _normal(n, d, seed), 14, precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_a_knot_tighter_than_the_resolution_collapses_like_the_oracle(context, precision):
+    x = _normal(150, 8, 21)
+    x[:10] = x[0] + np.float32(1e-7) * np.arange(10, dtype=np.float32)[:, None]
+    assert_matches_oracle(context, x, 5, precision)
+
+
+# --- tests/test_neighbors_audit.py -------------------------------------------------------
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_ties_break_towards_the_smaller_index_on_a_lattice(context, precision):
+    lat = synthetic.lattice(8)
+    ids, dist = assert_matches_oracle(context, lat, 14, precision)
+    _, exact = oracle.knn_fp64(lat, 14)
+    np.testing.assert_allclose(dist, exact, atol=1e-6)
+    for cell in range(len(lat)):
+        for a in range(13):
+            if np.isclose(dist[cell, a], dist[cell, a + 1], atol=1e-6):
+                assert ids[cell, a] < ids[cell, a + 1]
+
+
+@pytest.mark.parametrize("precision", ("bf16", "fp32"))
+@pytest.mark.parametrize("offset", [0.0, 1e2, 1e4, 1e5])
+def test_distances_survive_an_embedding_far_from_the_origin(context, precision, offset):
+    x = (np.random.default_rng(2).normal(size=(300, 12)) + offset).astype(np.float32)
+    ids, dist = assert_matches_oracle(context, x, 14, precision)
+    want_ids, want_dist = oracle.knn_fp64(x, 14)
+    for cell in range(x.shape[0]):
+        assert set(ids[cell].tolist()) == set(want_ids[cell].tolist()), f"cell {cell}"
+    np.testing.assert_allclose(dist, want_dist, rtol=1e-4, atol=1e-4 * max(1.0, offset * 1e-4))
+
+
+# --- the benchmark's data at test sizes ---------------------------------------------------
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_clustered_mixture_matches_the_oracle(context, precision):
+    x = synthetic.clustered(6000, 50, seed=3)
+    assert_matches_oracle(context, x, 15, precision)
+
+
+@pytest.mark.parametrize(("d", "k"), [(30, 30), (100, 15), (100, 30), (7, 3)])
+def test_other_dimensions_and_k(context, d, k):
+    x = synthetic.clustered(3000, d, seed=d + k)
+    for precision in PRECISIONS:
+        assert_matches_oracle(context, x, k, precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_cosine_matches_the_oracle(context, precision):
+    x = synthetic.clustered(4000, 50, seed=5)
+    x[11] = 0.0  # a zero row stays a zero vector
+    ids, dist = assert_matches_oracle(context, x, 15, precision, metric="cosine")
+    want_ids, want = oracle.knn_fp64(x, 15, metric="cosine")
+    np.testing.assert_allclose(dist, want, atol=2e-6)
+
+
+def test_all_paths_agree_on_a_larger_mixture(context):
+    x = synthetic.clustered(40000, 50, seed=9)
+    results = {p: run(context, x, 15, p, stats=True) for p in PRECISIONS}
+    for p in ("fp32", "exact"):
+        np.testing.assert_array_equal(results["bf16"][0], results[p][0])
+        np.testing.assert_array_equal(results["bf16"][1], results[p][1])
+    queries = np.random.default_rng(0).choice(x.shape[0], 400, replace=False)
+    want_ids, want_dist = oracle.knn(x, 15, queries=queries)
+    np.testing.assert_array_equal(results["bf16"][0][queries], want_ids)
+    np.testing.assert_array_equal(results["bf16"][1][queries], want_dist)
+    for p in ("bf16", "fp32"):
+        s = results[p][2]
+        print(p, s, flush=True)
+        assert s["fallback_rows"] <= x.shape[0] // 100
+
+
+def test_device_entry_point_matches_the_host_entry_point(context):
+    torch = pytest.importorskip("torch")
+    x = synthetic.clustered(5000, 50, seed=4)
+    ids, dist = run(context, x, 15)
+    dx = torch.from_numpy(x).to("cuda")
+    di = torch.empty((5000, 15), dtype=torch.int32, device="cuda")
+    dd = torch.empty((5000, 15), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    mrustsc_amd.knn_device(dx.data_ptr(), 5000, 50, 15, di.data_ptr(), dd.data_ptr(),
+                           context=context)
+    np.testing.assert_array_equal(di.cpu().numpy().view(np.uint32), ids)
+    np.testing.assert_array_equal(dd.cpu().numpy(), dist)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_boundary_near_ties_are_resolved_in_fp64(context, precision):
+    # ids 1 and 2 sit at f32 distance exactly 1 from cell 0; in f64 id 2 is nearer.
+    a, b = np.float32(1e-4), np.float32(1e-5)
+    x = np.array([[0, 0], [1, a], [1, b], [-51, -a], [49, -b]], dtype=np.float32)
+    ids, dist = assert_matches_oracle(context, x, 1, precision)
+    assert ids[0].tolist() == [2] and dist[0].tolist() == [1.0]
+    ids, _ = assert_matches_oracle(context, x, 2, precision)
+    assert ids[0].tolist() == [1, 2]
