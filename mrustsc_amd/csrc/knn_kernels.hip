@@ -724,8 +724,11 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactArgs a) {
                 while (lo > 0 && t - static_cast<double>(key_value(keys[lo - 1])) <= tau) lo--;
                 int hi = k;
                 while (hi + 1 < m && static_cast<double>(key_value(keys[hi + 1])) - t <= tau) hi++;
-                if ((hi + 1 == m && dropped) || hi - lo + 1 > kExactKeep) {
-                    atomicOr(a.flags, kFlagBandOverflow);
+                // A band wider than kBandCap keeps the (d^2, id) order. After a
+                // compaction m >= kExactKeep and lo < k <= kMaxK, so a band that reaches
+                // the end of the kept keys is always wider than the cap.
+                if (hi - lo + 1 > kBandCap || (hi + 1 == m && dropped)) {
+                    if (hi - lo + 1 <= kBandCap) atomicOr(a.flags, kFlagBandOverflow);
                 } else {
                     // Insertion sort of the band by (fp64 d^2, id); the band is small.
                     const int len = hi - lo + 1;

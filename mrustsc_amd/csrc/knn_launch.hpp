@@ -18,6 +18,10 @@ constexpr unsigned kFlagNonFinite = 1u;
 constexpr unsigned kFlagBandOverflow = 2u;
 
 constexpr int kCap = 256;   // candidate buffer slots per query row in the pass
+// Widest boundary tie band resolved in fp64; wider bands keep the (d^2, id) order.
+// Every rerank band has at most KP - 1 <= 63 members, so the cap only binds on the
+// exact path.
+constexpr int kBandCap = 64;
 constexpr int kMaxK = 256;  // largest k any path serves
 constexpr int kMaxPassK = 47;  // largest k the MFMA candidate pass serves (KP = 64)
 constexpr int kMaxPassD = 128; // largest d with a compiled MFMA variant

@@ -316,8 +316,11 @@ static int band_cmp(const void* lhs, const void* rhs) {
  * meaning with the device rerank). With the sorted keys s_0 <= s_1 <= ... and
  * t = d2(s_{k-1}), tau = t*(d+2)*2^-23: if d2(s_k) - t <= tau, the band is every
  * position j with |d2(s_j) - t| <= tau; positions before the band stay, and the
- * band's slots up to k-1 take the band members in (fp64 d^2, id) order.
+ * band's slots up to k-1 take the band members in (fp64 d^2, id) order. A band of
+ * more than ORACLE_BAND_CAP members (a near-equidistant crowd, e.g. every cell seen
+ * from a zero row under cosine) keeps the (d^2, id) order.
  * Returns 1 when the kept keys may not hold the whole band. */
+#define ORACLE_BAND_CAP 64
 static int apply_tie_rule(const uint64_t* best, size_t kept, size_t k, size_t n_candidates,
                           const float* coords, size_t d, const float* q, uint32_t* ids,
                           float* d2s) {
@@ -330,6 +333,7 @@ static int apply_tie_rule(const uint64_t* best, size_t kept, size_t k, size_t n_
     while (lo > 0 && t - (double)unordered((uint32_t)(best[lo - 1] >> 32)) <= tau) lo--;
     size_t hi = k;
     while (hi + 1 < kept && (double)unordered((uint32_t)(best[hi + 1] >> 32)) - t <= tau) hi++;
+    if (hi - lo + 1 > ORACLE_BAND_CAP) return 0; /* too wide to resolve: (d^2, id) order */
     if (hi + 1 == kept && kept < n_candidates) return 1;
     size_t m = hi - lo + 1;
     band_entry* band = (band_entry*)malloc(m * sizeof(band_entry));

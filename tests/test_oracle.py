@@ -277,3 +277,14 @@ def test_baseline_agrees_with_the_oracle_away_from_near_ties():
     same = np.mean([set(a) == set(b) for a, b in zip(b_ids.tolist(), o_ids.tolist())])
     assert same > 0.99
     np.testing.assert_allclose(b_dist, o_dist, rtol=1e-4, atol=1e-5)
+
+
+def test_a_band_wider_than_the_cap_keeps_the_plain_order():
+    # Under cosine every cell is a near-tie seen from a zero row: the band is far wider
+    # than 64 members, so that row keeps the (d^2, id) order.
+    x = synthetic.clustered(300, 8, seed=4)
+    x[7] = 0.0
+    ids, _ = oracle.knn(x, 5, metric="cosine")
+    plain, _ = oracle.knn(x, 5, metric="cosine", tie_rule=False)
+    np.testing.assert_array_equal(ids[7], plain[7])
+    assert ids[7].tolist() == sorted(ids[7].tolist())  # all at d^2 = |y|^2 ~ 1: id order
