@@ -82,10 +82,8 @@ void launch_column_means(const float* x, size_t n, int d, unsigned long long* ac
 // 2. Coordinates, norms and MFMA operands.
 //    Euclidean: (x as f64 - mean) rounded once (neighbors.rs:155-160); norms by f32
 //    fma in dimension order (knn.rs:148-156). Cosine: x / |x| with the norm in f64.
-//    bf16 operands: coordinate c = h + l + r with h = bf16(c), l = bf16(c - h).
-//      reference side  [h | l | h | 0-pad] ; query side [h | h | l | 0-pad]
-//    so one MFMA dot product over 3d packed features gives h.h + h.l + l.h.
-//    fp32 operands: feature f goes to group f / nch, slot f % nch, groups of `group`.
+//    Operand layouts are described in knn_pass.hip. Thread n writes the sentinel
+//    reference row (norm feature +inf).
 // ===========================================================================
 
 __global__ void prepare_kernel(const float* __restrict__ x, size_t n, int d, int metric,
@@ -95,6 +93,7 @@ __global__ void prepare_kernel(const float* __restrict__ x, size_t n, int d, int
                                int nch, int group, unsigned* __restrict__ max_norm_bits,
                                unsigned* __restrict__ flags) {
     const size_t row = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    constexpr uint16_t kBf16Inf = 0x7F80u, kBf16MinusHalf = 0xBF00u;
     float nrm = 0.0f;
     bool finite = true;
     if (row < n) {
@@ -133,7 +132,17 @@ __global__ void prepare_kernel(const float* __restrict__ x, size_t n, int d, int
                 q[d + j] = h;
                 q[2 * d + j] = l;
             }
-            for (int p = 3 * d; p < kc; p++) {
+            const uint16_t nh = bf16_rn(nrm);
+            const float rest = nrm - bf16_value(nh);
+            const uint16_t nl = bf16_rn(rest);
+            const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
+            r[3 * d] = nh;
+            r[3 * d + 1] = nl;
+            r[3 * d + 2] = nl2;
+            q[3 * d] = kBf16MinusHalf;
+            q[3 * d + 1] = kBf16MinusHalf;
+            q[3 * d + 2] = kBf16MinusHalf;
+            for (int p = 3 * d + 3; p < kc; p++) {
                 r[p] = 0;
                 q[p] = 0;
             }
@@ -142,7 +151,19 @@ __global__ void prepare_kernel(const float* __restrict__ x, size_t n, int d, int
             for (int s = 0; s < 4 * group; s++) {
                 int g = s / group, c = s % group;
                 int f = g * nch + c;
-                p[s] = (c < nch && f < d) ? dst[f] : 0.0f;
+                p[s] = c >= nch ? 0.0f : (f < d ? dst[f] : (f == d ? nrm : 0.0f));
+            }
+        }
+    } else if (row == n) {
+        if (operand == kOperandBf16) {
+            const int kc = 32 * nch;
+            uint16_t* r = pack_ref + row * kc;
+            for (int p = 0; p < kc; p++) r[p] = (p == 3 * d) ? kBf16Inf : 0;
+        } else if (operand == kOperandF32) {
+            float* p = pack_f32 + row * (4 * group);
+            for (int s = 0; s < 4 * group; s++) {
+                int g = s / group, c = s % group;
+                p[s] = (c < nch && g * nch + c == d) ? kInf : 0.0f;
             }
         }
     }
@@ -157,375 +178,10 @@ __global__ void prepare_kernel(const float* __restrict__ x, size_t n, int d, int
 }
 
 void launch_prepare(const PrepareArgs& a, hipStream_t stream) {
-    unsigned blocks = static_cast<unsigned>((a.n + 255) / 256);
+    unsigned blocks = static_cast<unsigned>((a.n + 1 + 255) / 256);
     hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, a.x, a.n, a.d,
                        a.metric, a.mean, a.coords, a.norms, a.operand, a.pack_ref, a.pack_query,
                        a.pack_f32, a.nch, a.group, a.max_norm_bits, a.flags);
-}
-
-// ===========================================================================
-// 3. Candidate pass.
-//
-// A workgroup owns 256 query rows (4 waves x 64) and streams every reference through
-// LDS in tiles of kTileRefs, double buffered. A wave keeps its 64 query rows' A
-// fragments in registers for the whole scan, so the only LDS traffic is the B
-// fragments (reference tile). MFMA 16x16 output: lane l holds rows 4*(l/16)+r
-// (r = 0..3) of column l%16, so each lane filters 16 (row, reference) scores per
-// 16-column block against its rows' thresholds.
-//
-// Selection: score s = |b|^2 - 2 a.b (the row's |a|^2 is a constant shift). A pair is a
-// candidate when s < theta[row]; candidates are appended to the row's buffer of kCap
-// keys in global memory (slot from an LDS counter). When a row's buffer could overflow
-// in the next tile, the wave compacts it: sorts it and keeps the KP smallest keys;
-// theta becomes the KP-th score. Every reference that is not kept therefore has
-// s >= theta_final, the bound the rerank's certificate uses.
-// ===========================================================================
-
-constexpr int kTileRefs = 64;  // references per LDS tile (4 column blocks of 16)
-constexpr int kRowsPerBlock = 256;
-
-template <int KP>
-__device__ __forceinline__ void compact_row(uint64_t* __restrict__ rowbuf, uint32_t count,
-                                            int lane, uint64_t* head, float* theta) {
-    uint64_t key[kCap / kWave];
-#pragma unroll
-    for (int i = 0; i < kCap / kWave; i++) {
-        const uint32_t e = i * kWave + lane;
-        key[i] = e < count ? rowbuf[e] : kNoKey;
-    }
-    wave_sort<kCap / kWave>(key, lane);
-    if (lane < KP) rowbuf[lane] = key[0];
-    *head = key[0];
-    *theta = key_value(shfl_u64(key[0], KP - 1));
-}
-
-// Shared epilogue of both passes: compaction checks after a tile, and the final
-// selection into cand / theta_out.
-template <int KP>
-__device__ __forceinline__ bool compact_full_rows(uint64_t* __restrict__ buf, uint32_t qbase,
-                                                  int w, int lane, uint32_t* cnt, float* thl) {
-    const uint32_t mine = cnt[w * kWave + lane];
-    uint64_t need = __ballot(mine > static_cast<uint32_t>(kCap - kTileRefs));
-    if (!need) return false;
-    while (need) {
-        const int q = __builtin_ctzll(need);
-        need &= need - 1;
-        const int ql = w * kWave + q;
-        const uint32_t count = cnt[ql];
-        uint64_t head;
-        float theta;
-        compact_row<KP>(buf + static_cast<size_t>(qbase + ql) * kCap, count, lane, &head, &theta);
-        if (lane == 0) {
-            cnt[ql] = KP;
-            thl[ql] = theta;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    return true;
-}
-
-template <int KP>
-__device__ __forceinline__ void final_rows(uint64_t* __restrict__ buf, uint32_t qbase, int w,
-                                           int lane, const uint32_t* cnt, uint32_t n,
-                                           uint64_t* __restrict__ cand,
-                                           float* __restrict__ theta_out) {
-    for (int q = 0; q < kWave; q++) {
-        const int ql = w * kWave + q;
-        const uint32_t row = qbase + ql;
-        const uint32_t count = cnt[ql];
-        uint64_t head;
-        float theta;
-        compact_row<KP>(buf + static_cast<size_t>(row) * kCap, count, lane, &head, &theta);
-        if (row < n) {
-            if (lane < KP) cand[static_cast<size_t>(row) * KP + lane] = head;
-            if (lane == 0) theta_out[row] = count >= static_cast<uint32_t>(KP) ? theta : kInf;
-        }
-    }
-}
-
-template <int NCH, int KP>
-__global__ __launch_bounds__(256, 1) void pass_bf16_kernel(
-    const uint16_t* __restrict__ pack_query, const uint16_t* __restrict__ pack_ref,
-    const float* __restrict__ norms, uint32_t n, uint64_t* __restrict__ buf,
-    uint64_t* __restrict__ cand, float* __restrict__ theta_out) {
-    constexpr int KC = 32 * NCH;       // packed bf16 features per row
-    constexpr int ROW_U4 = KC / 8;     // 16-byte chunks per packed row
-    constexpr int STRIDE = ROW_U4 + 1; // LDS row stride in 16-byte chunks (bank spread)
-    constexpr int LOADS = kTileRefs * ROW_U4 / 256;
-    __shared__ uint4 tile[2][kTileRefs * STRIDE];
-    __shared__ float tnorm[2][kTileRefs];
-    __shared__ uint32_t cnt[kRowsPerBlock];
-    __shared__ float thl[kRowsPerBlock];
-
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, lc = lane & 15;
-    const uint32_t qbase = blockIdx.x * kRowsPerBlock;
-
-    bf16x8 a[4][NCH];
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        uint32_t row = qbase + w * kWave + rb * 16 + lc;
-        if (row >= n) row = n - 1;
-        const uint4* src = reinterpret_cast<const uint4*>(pack_query + static_cast<size_t>(row) * KC) + g * NCH;
-#pragma unroll
-        for (int c = 0; c < NCH; c++) {
-            uint4 v = src[c];
-            a[rb][c] = __builtin_bit_cast(bf16x8, v);
-        }
-    }
-    cnt[tid] = 0;
-    thl[tid] = kInf;
-    float th[4][4];
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) th[rb][r] = kInf;
-
-    const uint32_t n_tiles = (n + kTileRefs - 1) / kTileRefs;
-    uint4 pre[LOADS];
-    float pren = kInf;
-    auto load_tile = [&](uint32_t t) {
-#pragma unroll
-        for (int i = 0; i < LOADS; i++) {
-            const int u = tid + 256 * i;
-            const int rl = u / ROW_U4, part = u % ROW_U4;
-            uint32_t ref = t * kTileRefs + rl;
-            if (ref >= n) ref = n - 1;
-            pre[i] = reinterpret_cast<const uint4*>(pack_ref + static_cast<size_t>(ref) * KC)[part];
-        }
-        if (tid < kTileRefs) {
-            const uint32_t ref = t * kTileRefs + tid;
-            pren = ref < n ? norms[ref] : kInf;
-        }
-    };
-    auto store_tile = [&](int b) {
-#pragma unroll
-        for (int i = 0; i < LOADS; i++) {
-            const int u = tid + 256 * i;
-            const int rl = u / ROW_U4, part = u % ROW_U4;
-            tile[b][rl * STRIDE + part] = pre[i];
-        }
-        if (tid < kTileRefs) tnorm[b][tid] = pren;
-    };
-
-    load_tile(0);
-    store_tile(0);
-    __syncthreads();
-    for (uint32_t t = 0; t < n_tiles; t++) {
-        const int cur = t & 1;
-        if (t + 1 < n_tiles) load_tile(t + 1);
-#pragma unroll
-        for (int cb = 0; cb < kTileRefs / 16; cb++) {
-            f32x4 acc[4];
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) acc[rb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            const uint4* brow = &tile[cur][(cb * 16 + lc) * STRIDE + g * NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; c++) {
-                const bf16x8 b = __builtin_bit_cast(bf16x8, brow[c]);
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-                    acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][c], b, acc[rb], 0, 0, 0);
-            }
-            const float nr = tnorm[cur][cb * 16 + lc];
-            const uint32_t id = t * kTileRefs + cb * 16 + lc;
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float s = __builtin_fmaf(-2.0f, acc[rb][r], nr);
-                    if (s < th[rb][r]) {
-                        const int ql = w * kWave + rb * 16 + g * 4 + r;
-                        const uint32_t slot = atomicAdd(&cnt[ql], 1u);
-                        buf[static_cast<size_t>(qbase + ql) * kCap + slot] = make_key(s, id);
-                    }
-                }
-            }
-        }
-        if (t + 1 < n_tiles) store_tile(cur ^ 1);
-        __syncthreads();
-        if (compact_full_rows<KP>(buf, qbase, w, lane, cnt, thl)) {
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) th[rb][r] = thl[w * kWave + rb * 16 + g * 4 + r];
-        }
-    }
-    final_rows<KP>(buf, qbase, w, lane, cnt, n, cand, theta_out);
-}
-
-template <int NCH, int KP>
-__global__ __launch_bounds__(256, 1) void pass_fp32_kernel(
-    const float* __restrict__ pack, const float* __restrict__ norms, uint32_t n,
-    uint64_t* __restrict__ buf, uint64_t* __restrict__ cand, float* __restrict__ theta_out) {
-    constexpr int GROUP = (NCH + 3) / 4 * 4;  // floats per lane group, 16-byte aligned
-    constexpr int ROW = 4 * GROUP;            // packed floats per row
-    constexpr int ROW_U4 = ROW / 4;
-    constexpr int STRIDE = ROW_U4 + 1;
-    constexpr int LOADS = kTileRefs * ROW_U4 / 256;
-    __shared__ uint4 tile[2][kTileRefs * STRIDE];
-    __shared__ float tnorm[2][kTileRefs];
-    __shared__ uint32_t cnt[kRowsPerBlock];
-    __shared__ float thl[kRowsPerBlock];
-
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, lc = lane & 15;
-    const uint32_t qbase = blockIdx.x * kRowsPerBlock;
-
-    float a[4][NCH];
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        uint32_t row = qbase + w * kWave + rb * 16 + lc;
-        if (row >= n) row = n - 1;
-        const float* src = pack + static_cast<size_t>(row) * ROW + g * GROUP;
-#pragma unroll
-        for (int c = 0; c < NCH; c++) a[rb][c] = src[c];
-    }
-    cnt[tid] = 0;
-    thl[tid] = kInf;
-    float th[4][4];
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) th[rb][r] = kInf;
-
-    const uint32_t n_tiles = (n + kTileRefs - 1) / kTileRefs;
-    uint4 pre[LOADS];
-    float pren = kInf;
-    auto load_tile = [&](uint32_t t) {
-#pragma unroll
-        for (int i = 0; i < LOADS; i++) {
-            const int u = tid + 256 * i;
-            const int rl = u / ROW_U4, part = u % ROW_U4;
-            uint32_t ref = t * kTileRefs + rl;
-            if (ref >= n) ref = n - 1;
-            pre[i] = reinterpret_cast<const uint4*>(pack + static_cast<size_t>(ref) * ROW)[part];
-        }
-        if (tid < kTileRefs) {
-            const uint32_t ref = t * kTileRefs + tid;
-            pren = ref < n ? norms[ref] : kInf;
-        }
-    };
-    auto store_tile = [&](int b) {
-#pragma unroll
-        for (int i = 0; i < LOADS; i++) {
-            const int u = tid + 256 * i;
-            const int rl = u / ROW_U4, part = u % ROW_U4;
-            tile[b][rl * STRIDE + part] = pre[i];
-        }
-        if (tid < kTileRefs) tnorm[b][tid] = pren;
-    };
-
-    load_tile(0);
-    store_tile(0);
-    __syncthreads();
-    for (uint32_t t = 0; t < n_tiles; t++) {
-        const int cur = t & 1;
-        if (t + 1 < n_tiles) load_tile(t + 1);
-#pragma unroll
-        for (int cb = 0; cb < kTileRefs / 16; cb++) {
-            f32x4 acc[4];
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) acc[rb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            const float* brow = reinterpret_cast<const float*>(&tile[cur][(cb * 16 + lc) * STRIDE]) + g * GROUP;
-#pragma unroll
-            for (int c4 = 0; c4 < GROUP / 4; c4++) {
-                const float4 b4 = reinterpret_cast<const float4*>(brow)[c4];
-                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
-#pragma unroll
-                for (int cc = 0; cc < 4; cc++) {
-                    const int c = c4 * 4 + cc;
-                    if (c < NCH) {
-#pragma unroll
-                        for (int rb = 0; rb < 4; rb++)
-                            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][c], bv[cc], acc[rb], 0, 0, 0);
-                    }
-                }
-            }
-            const float nr = tnorm[cur][cb * 16 + lc];
-            const uint32_t id = t * kTileRefs + cb * 16 + lc;
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float s = __builtin_fmaf(-2.0f, acc[rb][r], nr);
-                    if (s < th[rb][r]) {
-                        const int ql = w * kWave + rb * 16 + g * 4 + r;
-                        const uint32_t slot = atomicAdd(&cnt[ql], 1u);
-                        buf[static_cast<size_t>(qbase + ql) * kCap + slot] = make_key(s, id);
-                    }
-                }
-            }
-        }
-        if (t + 1 < n_tiles) store_tile(cur ^ 1);
-        __syncthreads();
-        if (compact_full_rows<KP>(buf, qbase, w, lane, cnt, thl)) {
-#pragma unroll
-            for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) th[rb][r] = thl[w * kWave + rb * 16 + g * 4 + r];
-        }
-    }
-    final_rows<KP>(buf, qbase, w, lane, cnt, n, cand, theta_out);
-}
-
-// Dispatch tables over the compiled (NCH, KP) variants.
-template <int NCH, int KP>
-static void run_bf16(const PassArgs& a, hipStream_t stream) {
-    const unsigned blocks = static_cast<unsigned>((a.n + kRowsPerBlock - 1) / kRowsPerBlock);
-    hipLaunchKernelGGL((pass_bf16_kernel<NCH, KP>), dim3(blocks), dim3(256), 0, stream,
-                       a.pack_query, a.pack_ref, a.norms, a.n, a.buf, a.cand, a.theta);
-}
-
-template <int NCH, int KP>
-static void run_fp32(const PassArgs& a, hipStream_t stream) {
-    const unsigned blocks = static_cast<unsigned>((a.n + kRowsPerBlock - 1) / kRowsPerBlock);
-    hipLaunchKernelGGL((pass_fp32_kernel<NCH, KP>), dim3(blocks), dim3(256), 0, stream,
-                       a.pack_f32, a.norms, a.n, a.buf, a.cand, a.theta);
-}
-
-#define SCRUST_BF16_CASE(N)                       \
-    case N:                                       \
-        if (a.kp == 32) run_bf16<N, 32>(a, stream); \
-        else run_bf16<N, 64>(a, stream);            \
-        return true;
-#define SCRUST_FP32_CASE(N)                       \
-    case N:                                       \
-        if (a.kp == 32) run_fp32<N, 32>(a, stream); \
-        else run_fp32<N, 64>(a, stream);            \
-        return true;
-
-int bf16_chunks(int d) {
-    const int need = (3 * d + 31) / 32;
-    static const int sizes[] = {1, 2, 3, 4, 5, 6, 8, 10, 12};
-    for (int s : sizes)
-        if (s >= need) return s;
-    return 0;
-}
-
-int fp32_chunks(int d) {
-    const int need = (d + 3) / 4;
-    static const int sizes[] = {1, 2, 3, 4, 6, 8, 10, 13, 16, 20, 25, 32};
-    for (int s : sizes)
-        if (s >= need) return s;
-    return 0;
-}
-
-int fp32_group(int nch) { return (nch + 3) / 4 * 4; }
-
-bool launch_pass(const PassArgs& a, hipStream_t stream) {
-    if (a.operand == kOperandBf16) {
-        switch (a.nch) {
-            SCRUST_BF16_CASE(1) SCRUST_BF16_CASE(2) SCRUST_BF16_CASE(3)
-            SCRUST_BF16_CASE(4) SCRUST_BF16_CASE(5) SCRUST_BF16_CASE(6)
-            SCRUST_BF16_CASE(8) SCRUST_BF16_CASE(10) SCRUST_BF16_CASE(12)
-            default: return false;
-        }
-    }
-    switch (a.nch) {
-        SCRUST_FP32_CASE(1) SCRUST_FP32_CASE(2) SCRUST_FP32_CASE(3) SCRUST_FP32_CASE(4)
-        SCRUST_FP32_CASE(6) SCRUST_FP32_CASE(8) SCRUST_FP32_CASE(10) SCRUST_FP32_CASE(13)
-        SCRUST_FP32_CASE(16) SCRUST_FP32_CASE(20) SCRUST_FP32_CASE(25) SCRUST_FP32_CASE(32)
-        default: return false;
-    }
 }
 
 // ===========================================================================
@@ -600,9 +256,11 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
         const double gamma = d * u / (1.0 - d * u);
         const double qn = __builtin_sqrt(nqd * (1.0 + gamma));
         const double rn = __builtin_sqrt(nmax * (1.0 + gamma));
-        const double dot_err = a.dot_rel_err * qn * rn;
-        const double score_round = u * (nmax * (1.0 + gamma) + 2.0 * qn * rn * 1.01);
-        const double err = 1.25 * (gamma * (nqd + nmax) * 1.01 + 2.0 * dot_err + score_round) + 1e-30;
+        // |score + |a|^2 - exact d^2| <= err: f32 norms, the bf16 split of the
+        // coordinates and of |b|^2, and the MFMA accumulation over every packed term.
+        const double err = 1.25 * (gamma * (nqd + nmax) * 1.01 + 2.0 * a.split_rel * qn * rn +
+                                   2.0 * a.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
+                                   a.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
         const double cd = (d + 3) * u * 1.01;
         const double lower = (static_cast<double>(a.theta[qq]) + nqd - err) * (1.0 - cd);
         const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);

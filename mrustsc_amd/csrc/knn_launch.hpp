@@ -48,6 +48,7 @@ struct PassArgs {
     int operand;
     int nch;
     int kp;
+    int d;
     const uint16_t* pack_query;
     const uint16_t* pack_ref;
     const float* pack_f32;
@@ -69,7 +70,10 @@ struct RerankArgs {
     const uint64_t* cand;
     const float* theta;
     const unsigned* max_norm_bits;
-    double dot_rel_err;  // bound on |computed a.b - a.b| / (|a| |b|) of the pass
+    // Error model of the pass (see the certificate in rerank_kernel):
+    double split_rel;  // operand split error of a.b, relative to |a||b|
+    double acc_rel;    // MFMA accumulation error, relative to the sum of |terms|
+    double norm_rel;   // error of the |b|^2 feature, relative to |b|^2
     uint32_t* out_indices;
     float* out_distances;
     uint32_t* fail_list;

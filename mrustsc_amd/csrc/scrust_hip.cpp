@@ -144,18 +144,21 @@ scrust_hip_status run_knn(scrust_hip_context* ctx, const float* x, size_t n, siz
     const uint32_t nu = static_cast<uint32_t>(n);
 
     int operand = kOperandNone, nch = 0, group = 0, kp = 0;
-    double dot_rel_err = 0.0;
+    double split_rel = 0.0, acc_rel = 0.0, norm_rel = 0.0;
     if (precision == SCRUST_HIP_BF16) {
+        // c = h + l + r with |r| <= 2^-16 |c|: dropping l.l and the residuals costs
+        // <= 3.02 * 2^-16 |a||b|; |b|^2 = nh + nl + nl2 to 2^-24; products of bf16 are
+        // exact in f32, the accumulation is bounded as 2 * (terms + 1) * 2^-24.
         operand = kOperandBf16;
         nch = bf16_chunks(di);
-        const double kc = 32.0 * nch;
-        dot_rel_err = 3.02 * 0x1p-16 + 2.0 * kc * 0x1p-24 * 1.05;
+        split_rel = 3.02 * 0x1p-16;
+        acc_rel = 2.0 * (32.0 * nch + 1.0) * 0x1p-24 * 1.05;
+        norm_rel = 0x1p-24 * 1.01;
     } else if (precision == SCRUST_HIP_FP32) {
         operand = kOperandF32;
         nch = fp32_chunks(di);
         group = fp32_group(nch);
-        const double kc = 4.0 * nch;
-        dot_rel_err = 2.0 * (kc + 1.0) * 0x1p-24 * 1.05;
+        acc_rel = 2.0 * (4.0 * nch + 1.0) * 0x1p-24 * 1.05;
     }
     if (operand != kOperandNone) kp = (ki + 17 <= 32) ? 32 : 64;
     const size_t npad = (n + 255) / 256 * 256;
@@ -169,10 +172,10 @@ scrust_hip_status run_knn(scrust_hip_context* ctx, const float* x, size_t n, siz
     SCRUST_CHECK(ws.fail_list.reserve(n * sizeof(uint32_t)), "allocate fallback list");
     if (operand == kOperandBf16) {
         const size_t row = 32ull * nch * sizeof(uint16_t);
-        SCRUST_CHECK(ws.pack_ref.reserve(n * row), "allocate operands");
-        SCRUST_CHECK(ws.pack_query.reserve(n * row), "allocate operands");
+        SCRUST_CHECK(ws.pack_ref.reserve((n + 1) * row), "allocate operands");
+        SCRUST_CHECK(ws.pack_query.reserve((n + 1) * row), "allocate operands");
     } else if (operand == kOperandF32) {
-        SCRUST_CHECK(ws.pack_ref.reserve(n * 4ull * group * sizeof(float)), "allocate operands");
+        SCRUST_CHECK(ws.pack_ref.reserve((n + 1) * 4ull * group * sizeof(float)), "allocate operands");
     }
     if (operand != kOperandNone) {
         SCRUST_CHECK(ws.buf.reserve(npad * kCap * sizeof(uint64_t)), "allocate candidate buffers");
@@ -227,6 +230,7 @@ scrust_hip_status run_knn(scrust_hip_context* ctx, const float* x, size_t n, siz
         ps.operand = operand;
         ps.nch = nch;
         ps.kp = kp;
+        ps.d = di;
         ps.pack_query = pa.pack_query;
         ps.pack_ref = pa.pack_ref;
         ps.pack_f32 = pa.pack_f32;
@@ -251,7 +255,9 @@ scrust_hip_status run_knn(scrust_hip_context* ctx, const float* x, size_t n, siz
         ra.cand = ps.cand;
         ra.theta = ps.theta;
         ra.max_norm_bits = max_norm_bits;
-        ra.dot_rel_err = dot_rel_err;
+        ra.split_rel = split_rel;
+        ra.acc_rel = acc_rel;
+        ra.norm_rel = norm_rel;
         ra.out_indices = out_indices;
         ra.out_distances = out_distances;
         ra.fail_list = static_cast<uint32_t*>(ws.fail_list.ptr);
