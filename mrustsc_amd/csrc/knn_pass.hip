@@ -7,8 +7,9 @@
 // supplies the B fragments. The reference's squared norm is one more packed feature, so
 // the MFMA accumulator is acc = a.b - |b|^2 / 2 and the score s = |b|^2 - 2 a.b (the row's
 // |a|^2 is a constant shift) is -2 acc: a lane tests each of its 16 accumulators against
-// its rows' thresholds with one compare, and the 16 compares of a 16x16 block are
-// combined into one wave-uniform branch.
+// its rows' thresholds. References sit on the MFMA's M side and queries on its N side, so
+// the four accumulators a lane holds per 16x16 block belong to one query row and are
+// tested with one max and one compare; a block's tests form one wave-uniform branch.
 //
 // Selection: a pair with s < theta[row] is appended to the row's kCap-slot buffer in
 // global memory (slot from an LDS counter). After each tile, a row whose buffer could
@@ -50,7 +51,7 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Keep the `keep` smallest of the `count` keys at rowbuf[0..count) (count > keep): they
+// Keep the `keep` smallest of the `count` keys at rowbuf[0..count) (count >= keep): they
 // are written, unordered, to out[0..keep); returns the keep-th smallest score. Ballot
 // radix select: a binary search over the ordered score bits, then over ids among keys
 // that tie on the boundary score.
@@ -122,12 +123,10 @@ struct Operand<kOperandF32, NCH> {
     typedef float Frag;
 };
 
-#ifndef SCRUST_PASS_BLOCKS_PER_CU
-#define SCRUST_PASS_BLOCKS_PER_CU 2
-#endif
-
+// At most two waves per SIMD: the allocator then keeps the tile prefetch in registers
+// instead of spilling it to reach a higher occupancy the LDS budget would allow.
 template <int OP, int NCH, int KP>
-__global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(PassArgs p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void pass_kernel(PassArgs p) {
     typedef Operand<OP, NCH> Op;
     constexpr int ROW_U4 = Op::kRowU4;
     constexpr int STRIDE = ROW_U4 + 1;  // LDS row stride in 16-byte chunks (bank spread)
@@ -145,6 +144,8 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
                                                                ? static_cast<const void*>(p.pack_ref)
                                                                : static_cast<const void*>(p.pack_f32));
 
+    // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
+    // qb's fragment in lane l carries row qb*16 + l%16.
     typename Op::Frag a[4][NCH];
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) {
@@ -163,17 +164,16 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
     }
     cnt[tid] = 0;
     phl[tid] = -kInf;
-    float phi[4][4];
+    // phi[qb] = -theta/2 of this lane's query row qb*16 + l%16.
+    float phi[4];
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) phi[rb][r] = -kInf;
+    for (int qb = 0; qb < 4; qb++) phi[qb] = -kInf;
 
     const uint32_t n_tiles = (n + kTileRefs - 1) / kTileRefs;
     uint4 pre[LOADS];
     // Global -> registers for tile t (row n is the sentinel); registers -> LDS buffer b.
     // Indices are compile-time constants so `pre` stays in registers.
-    auto load_tile = [&](uint32_t t) {
+    auto load_tile = [&](uint32_t t) __attribute__((always_inline)) {
         static_for<0, LOADS>([&](auto I) {
             constexpr int i = decltype(I)::value;
             const int u = tid + 256 * i;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
             pre[i] = ref_rows[static_cast<size_t>(ref) * ROW_U4 + part];
         });
     };
-    auto store_tile = [&](int b) {
+    auto store_tile = [&](int b) __attribute__((always_inline)) {
         static_for<0, LOADS>([&](auto I) {
             constexpr int i = decltype(I)::value;
             const int u = tid + 256 * i;
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
             tile[b][rl * STRIDE + part] = pre[i];
         });
     };
-    auto mfma_block = [&](f32x4 (&acc)[4], int cur, int cb) {
+    auto mfma_block = [&](f32x4 (&acc)[4], int cur, int cb) __attribute__((always_inline)) {
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) acc[rb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         const uint4* brow = &tile[cur][(cb * 16 + lc) * STRIDE];
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
                 const bf16x8 b = __builtin_bit_cast(bf16x8, brow[g * NCH + c]);
 #pragma unroll
                 for (int rb = 0; rb < 4; rb++)
-                    acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][c], b, acc[rb], 0, 0, 0);
+                    acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[rb][c], acc[rb], 0, 0, 0);
             }
         } else {
             const float4* bq = reinterpret_cast<const float4*>(brow) + g * (Op::kGroup / 4);
@@ -214,28 +214,32 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
                     if (c4 * 4 + cc < NCH) {
 #pragma unroll
                         for (int rb = 0; rb < 4; rb++)
-                            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][c4 * 4 + cc], bv[cc], acc[rb], 0, 0, 0);
+                            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[cc], a[rb][c4 * 4 + cc], acc[rb], 0, 0, 0);
                     }
                 }
             }
         }
     };
-    auto filter = [&](const f32x4 (&acc)[4], uint32_t t, int cb) {
+    // acc[qb][r] holds reference 4*(l/16) + r of the block against query qb*16 + l%16, so a
+    // lane's four values share one threshold: one max and one compare per query block.
+    auto filter = [&](const f32x4 (&acc)[4], uint32_t t, int cb) __attribute__((always_inline)) {
         uint64_t any = 0;
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) any |= __ballot(acc[rb][r] > phi[rb][r]);
+        for (int qb = 0; qb < 4; qb++) {
+            const float m = __builtin_fmaxf(__builtin_fmaxf(acc[qb][0], acc[qb][1]),
+                                            __builtin_fmaxf(acc[qb][2], acc[qb][3]));
+            any |= __ballot(m > phi[qb]);
+        }
         if (any) {
-            const uint32_t id = t * kTileRefs + cb * 16 + lc;
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
+            for (int qb = 0; qb < 4; qb++) {
+                const int ql = w * kWave + qb * 16 + lc;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    if (acc[rb][r] > phi[rb][r]) {
-                        const int ql = w * kWave + rb * 16 + g * 4 + r;
+                    if (acc[qb][r] > phi[qb]) {
+                        const uint32_t id = t * kTileRefs + cb * 16 + g * 4 + r;
                         const uint32_t slot = atomicAdd(&cnt[ql], 1u);
-                        p.buf[static_cast<size_t>(qbase + ql) * kCap + slot] = make_key(-2.0f * acc[rb][r], id);
+                        p.buf[static_cast<size_t>(qbase + ql) * kCap + slot] = make_key(-2.0f * acc[qb][r], id);
                     }
                 }
             }
@@ -247,7 +251,9 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
     __syncthreads();
     for (uint32_t t = 0; t < n_tiles; t++) {
         const int cur = t & 1;
-        if (t + 1 < n_tiles) load_tile(t + 1);
+        // Unconditional prefetch (the last tile reloads itself into the idle buffer): a
+        // conditional load keeps `pre` out of registers and serialises the prefetch.
+        load_tile(t + 1 < n_tiles ? t + 1 : t);
         f32x4 acc0[4], acc1[4];
         // Software pipeline: the MFMAs of block cb are in flight while block cb-1 is filtered.
         mfma_block(acc0, cur, 0);
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
         mfma_block(acc1, cur, 3);
         filter(acc0, t, 2);
         filter(acc1, t, 3);
-        if (t + 1 < n_tiles) store_tile(cur ^ 1);
+        store_tile(cur ^ 1);
         __syncthreads();
         // Compact every row of this wave whose buffer could overflow in the next tile.
         const uint32_t mine = cnt[w * kWave + lane];
@@ -278,9 +284,7 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) phi[rb][r] = phl[w * kWave + rb * 16 + g * 4 + r];
+            for (int qb = 0; qb < 4; qb++) phi[qb] = phl[w * kWave + qb * 16 + lc];
         }
     }
     // Final selection: the KP smallest keys of each row (unordered) and the KP-th score.
@@ -291,12 +295,12 @@ __global__ __launch_bounds__(256, SCRUST_PASS_BLOCKS_PER_CU) void pass_kernel(Pa
         const uint64_t* rowbuf = p.buf + static_cast<size_t>(row) * kCap;
         uint64_t* out = p.cand + static_cast<size_t>(row) * KP;
         const uint32_t count = cnt[ql];
-        if (count > static_cast<uint32_t>(KP)) {
+        if (count >= static_cast<uint32_t>(KP)) {
             const float theta = select_smallest<E>(rowbuf, count, KP, lane, out);
             if (lane == 0) p.theta[row] = theta;
         } else {
             if (lane < KP) out[lane] = static_cast<uint32_t>(lane) < count ? rowbuf[lane] : kNoKey;
-            if (lane == 0) p.theta[row] = count == static_cast<uint32_t>(KP) ? key_value(rowbuf[KP - 1]) : kInf;
+            if (lane == 0) p.theta[row] = kInf;
         }
     }
 }
