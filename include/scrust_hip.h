@@ -53,7 +53,9 @@ typedef struct scrust_hip_context scrust_hip_context;
 typedef struct scrust_hip_knn_options {
     uint32_t metric;    /* scrust_hip_metric */
     uint32_t precision; /* scrust_hip_precision */
-    uint32_t tile;      /* 0: autotuned or default tile shape; else a tile id (see DESIGN) */
+    /* Candidate buffer of the pass, slots per query row: 0 = tuned or default (256),
+     * else 128, 256 or 512. Every setting gives the same result. */
+    uint32_t tile;
     uint32_t reserved;
 } scrust_hip_knn_options;
 
@@ -110,6 +112,94 @@ scrust_hip_status scrust_hip_knn_device(scrust_hip_context* context, const float
                                         const scrust_hip_knn_options* options,
                                         uint32_t* out_indices, float* out_distances,
                                         scrust_hip_knn_stats* stats);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched builds: many embeddings of uneven n, d and k in one persistent MFMA launch,
+ * the whole pipeline captured once in a HIP graph and replayed by scrust_hip_batch_run.
+ * No counterpart in the reference (its kNN takes one embedding per call).
+ * ------------------------------------------------------------------------------------- */
+
+typedef struct scrust_hip_dataset {
+    const float* embedding;  /* device pointer, n x d float32 row-major */
+    size_t n, d, k;
+    uint32_t* out_indices;   /* device pointer, n x k */
+    float* out_distances;    /* device pointer, n x k */
+} scrust_hip_dataset;
+
+typedef struct scrust_hip_batch_stats {
+    double device_ms;        /* graph launch to completion */
+    uint64_t datasets;
+    uint64_t pass_items;     /* (dataset, 256-row block) work items of the persistent pass */
+    uint64_t exact_rows;     /* rows computed by the exact kernel (exact-path + uncertified) */
+    uint64_t pass_flops;     /* sum over pass datasets of 2 * n * n * d */
+    uint64_t cells;          /* sum of n */
+} scrust_hip_batch_stats;
+
+typedef struct scrust_hip_batch scrust_hip_batch;
+
+/* Validates every dataset (as scrust_hip_knn does), allocates the batch's device workspace
+ * and captures the pipeline in a HIP graph on the context's stream. The embedding and
+ * output pointers are bound into the graph: their contents may change between runs, the
+ * pointers may not. */
+scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
+                                          const scrust_hip_dataset* datasets, size_t count,
+                                          const scrust_hip_knn_options* options,
+                                          scrust_hip_batch** out);
+/* Replays the graph and waits for it. */
+scrust_hip_status scrust_hip_batch_run(scrust_hip_batch* batch, scrust_hip_batch_stats* stats);
+void scrust_hip_batch_destroy(scrust_hip_batch* batch);
+
+/* ---------------------------------------------------------------------------------------
+ * Serving: a request loop that admits kNN builds as device memory and slots free up and
+ * overlaps admitted builds on the device (one HIP stream per slot). Requests are served in
+ * arrival order; a request is admitted when a slot is idle and its workspace fits in the
+ * server's device-memory budget, which the server allocates once at creation. No
+ * counterpart in the reference.
+ * ------------------------------------------------------------------------------------- */
+
+typedef struct scrust_hip_server scrust_hip_server;
+
+typedef struct scrust_hip_server_config {
+    uint32_t slots;         /* concurrent builds on the device (0: 4) */
+    uint32_t reserved;
+    uint64_t memory_bytes;  /* device-memory budget (0: half of the free memory) */
+} scrust_hip_server_config;
+
+typedef struct scrust_hip_request_info {
+    int64_t submit_ns;      /* steady-clock times of submission, admission, completion */
+    int64_t admit_ns;
+    int64_t done_ns;        /* results are in the caller's buffers */
+    uint64_t fallback_rows; /* rows recomputed exactly */
+    uint32_t slot;
+    uint32_t status;        /* scrust_hip_status of the build */
+} scrust_hip_request_info;
+
+typedef struct scrust_hip_server_stats {
+    uint64_t submitted, completed, rejected;
+    uint64_t peak_in_flight;  /* most builds on the device at once */
+    uint64_t memory_bytes;    /* the budget */
+    uint64_t peak_memory_bytes;
+} scrust_hip_server_stats;
+
+scrust_hip_status scrust_hip_server_create(int device, const scrust_hip_server_config* config,
+                                           scrust_hip_server** out);
+/* Queues a build. `embedding` (n x d host memory) and the outputs (n x k host memory) must
+ * stay valid until scrust_hip_server_wait returns for the ticket. Arguments are checked
+ * here (as scrust_hip_knn does); a build larger than the whole budget is refused with
+ * OUT_OF_MEMORY. */
+scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const float* embedding,
+                                           size_t n, size_t d, size_t k,
+                                           const scrust_hip_knn_options* options,
+                                           uint32_t* out_indices, float* out_distances,
+                                           uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: forever) for a ticket; returns the build's status (KERNEL
+ * with a "timed out" message when it is still running; the ticket stays valid). */
+scrust_hip_status scrust_hip_server_wait(scrust_hip_server* server, uint64_t ticket,
+                                         int64_t timeout_ms, scrust_hip_request_info* info);
+scrust_hip_status scrust_hip_server_stats_get(scrust_hip_server* server,
+                                              scrust_hip_server_stats* stats);
+/* Finishes every queued and running build, then releases the server. */
+void scrust_hip_server_destroy(scrust_hip_server* server);
 
 #ifdef __cplusplus
 }

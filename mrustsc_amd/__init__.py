@@ -28,6 +28,8 @@ __all__ = [
     "knn_hip",
     "knn_device",
     "max_supported_k",
+    "Batch",
+    "Server",
 ]
 
 DEVICES = ("auto", "gpu", "hip")
@@ -114,23 +116,23 @@ def _embedding(embedding) -> np.ndarray:
     return np.ascontiguousarray(embedding)
 
 
-def _options(metric: str, precision: str) -> KnnOptions:
+def _options(metric: str, precision: str, tile: int = 0) -> KnnOptions:
     if metric not in _native.METRICS:
         raise ValueError(f"metric must be one of {sorted(_native.METRICS)}, got {metric!r}")
     if precision not in _native.PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_native.PRECISIONS)}, got {precision!r}")
-    return KnnOptions(_native.METRICS[metric], _native.PRECISIONS[precision], 0, 0)
+    return KnnOptions(_native.METRICS[metric], _native.PRECISIONS[precision], int(tile), 0)
 
 
 def knn_hip(embedding, k: int, *, metric: str = "euclidean", precision: str = "bf16",
-            context: Context | None = None, stats: bool = False):
+            context: Context | None = None, stats: bool = False, tile: int = 0):
     """kNN on the device from host memory. Returns (ids, distances) or, with
     ``stats=True``, (ids, distances, stats dict)."""
     x = _embedding(embedding)
     if isinstance(k, bool) or int(k) != k or k < 0:
         raise ValueError(f"k must be a non-negative integer, got {k!r}")
     k = int(k)
-    opts = _options(metric, precision)
+    opts = _options(metric, precision, tile)
     handle = _handle_or_none(context)
     n, d = x.shape
     ids = np.empty((n, k), dtype=np.uint32)
@@ -159,10 +161,10 @@ def knn(embedding, k: int, device: str = "auto"):
 
 def knn_device(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int, distances_ptr: int,
                *, metric: str = "euclidean", precision: str = "bf16",
-               context: Context | None = None) -> dict:
+               context: Context | None = None, tile: int = 0) -> dict:
     """kNN on device-resident buffers (raw device pointers, e.g. ``tensor.data_ptr()``).
     Returns the call's stats."""
-    opts = _options(metric, precision)
+    opts = _options(metric, precision, tile)
     ctx = context if context is not None else _thread_context()
     st = KnnStats()
     status = _native.load().scrust_hip_knn_device(
@@ -170,3 +172,116 @@ def knn_device(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int, dis
         ctypes.c_void_p(indices_ptr), ctypes.c_void_p(distances_ptr), ctypes.byref(st))
     _native.raise_for(status)
     return st.as_dict()
+
+
+class Batch:
+    """Many embeddings in one persistent MFMA launch, captured once in a HIP graph.
+
+    ``datasets`` is a sequence of (embedding_ptr, n, d, k, indices_ptr, distances_ptr)
+    device pointers (e.g. from torch tensors). The pointers are bound into the graph:
+    their contents may change between ``run`` calls, the pointers may not. No
+    counterpart in the reference, whose kNN takes one embedding per call.
+    """
+
+    def __init__(self, datasets, *, metric: str = "euclidean", precision: str = "bf16",
+                 context: Context | None = None, tile: int = 0):
+        self._lib = _native.load()
+        self._context = context if context is not None else _thread_context()
+        arr = (_native.Dataset * len(datasets))()
+        for i, (x, n, d, k, ids, dist) in enumerate(datasets):
+            arr[i] = _native.Dataset(ctypes.c_void_p(x), n, d, k, ctypes.c_void_p(ids),
+                                     ctypes.c_void_p(dist))
+        opts = _options(metric, precision, tile)
+        handle = ctypes.c_void_p()
+        _native.raise_for(self._lib.scrust_hip_batch_create(
+            self._context.handle, arr, len(datasets), ctypes.byref(opts), ctypes.byref(handle)))
+        self._handle = handle
+
+    def run(self) -> dict:
+        st = _native.BatchStats()
+        _native.raise_for(self._lib.scrust_hip_batch_run(self._handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None:
+            self._lib.scrust_hip_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """A kNN request loop on the device: builds are admitted in arrival order as slots
+    and device memory free up, and admitted builds run concurrently (one HIP stream per
+    slot). ``submit`` returns a ticket at once; ``wait`` returns the result.
+
+    No counterpart in the reference (its kNN is one synchronous call).
+    """
+
+    def __init__(self, device: int = 0, *, slots: int = 4, memory_bytes: int = 0):
+        self._lib = _native.load()
+        cfg = _native.ServerConfig(int(slots), 0, int(memory_bytes))
+        handle = ctypes.c_void_p()
+        _native.raise_for(self._lib.scrust_hip_server_create(int(device), ctypes.byref(cfg),
+                                                             ctypes.byref(handle)))
+        self._handle = handle
+        self._pending = {}
+
+    def submit(self, embedding, k: int, *, metric: str = "euclidean",
+               precision: str = "bf16") -> int:
+        x = _embedding(embedding)
+        n, d = x.shape
+        ids = np.empty((n, int(k)), dtype=np.uint32)
+        dist = np.empty((n, int(k)), dtype=np.float32)
+        opts = _options(metric, precision)
+        ticket = ctypes.c_uint64()
+        _native.raise_for(self._lib.scrust_hip_server_submit(
+            self._handle, x.ctypes.data, n, d, int(k), ctypes.byref(opts), ids.ctypes.data,
+            dist.ctypes.data, ctypes.byref(ticket)))
+        # Keep the buffers alive until the build is done.
+        self._pending[ticket.value] = (x, ids, dist)
+        return ticket.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(ids, distances, info) of a ticket; info holds the request's timestamps."""
+        info = _native.RequestInfo()
+        status = self._lib.scrust_hip_server_wait(self._handle, int(ticket), int(timeout_ms),
+                                                  ctypes.byref(info))
+        if status != _native.OK and "timed out" in _native.last_error():
+            _native.raise_for(status)
+        _, ids, dist = self._pending.pop(int(ticket))
+        _native.raise_for(status)
+        return ids, dist, info.as_dict()
+
+    def stats(self) -> dict:
+        st = _native.ServerStats()
+        _native.raise_for(self._lib.scrust_hip_server_stats_get(self._handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None:
+            self._lib.scrust_hip_server_destroy(self._handle)
+            self._handle = None
+            self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

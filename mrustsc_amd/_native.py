@@ -38,6 +38,14 @@ EXPORTS = (
     "scrust_hip_knn",
     "scrust_hip_knn_ex",
     "scrust_hip_knn_device",
+    "scrust_hip_batch_create",
+    "scrust_hip_batch_run",
+    "scrust_hip_batch_destroy",
+    "scrust_hip_server_create",
+    "scrust_hip_server_submit",
+    "scrust_hip_server_wait",
+    "scrust_hip_server_stats_get",
+    "scrust_hip_server_destroy",
 )
 
 
@@ -70,6 +78,67 @@ class KnnStats(ctypes.Structure):
         return out
 
 
+class Dataset(ctypes.Structure):
+    _fields_ = [
+        ("embedding", ctypes.c_void_p),
+        ("n", ctypes.c_size_t),
+        ("d", ctypes.c_size_t),
+        ("k", ctypes.c_size_t),
+        ("out_indices", ctypes.c_void_p),
+        ("out_distances", ctypes.c_void_p),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("device_ms", ctypes.c_double),
+        ("datasets", ctypes.c_uint64),
+        ("pass_items", ctypes.c_uint64),
+        ("exact_rows", ctypes.c_uint64),
+        ("pass_flops", ctypes.c_uint64),
+        ("cells", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("memory_bytes", ctypes.c_uint64),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("submit_ns", ctypes.c_int64),
+        ("admit_ns", ctypes.c_int64),
+        ("done_ns", ctypes.c_int64),
+        ("fallback_rows", ctypes.c_uint64),
+        ("slot", ctypes.c_uint32),
+        ("status", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("peak_in_flight", ctypes.c_uint64),
+        ("memory_bytes", ctypes.c_uint64),
+        ("peak_memory_bytes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class ScrustHipError(RuntimeError):
     """A failure reported by the library that is not an argument error."""
 
@@ -77,10 +146,22 @@ class ScrustHipError(RuntimeError):
 _lib = None
 
 
+def _share_hip_runtime_with_torch() -> None:
+    """PyTorch ships its own libamdhip64. Loading it before this library makes the
+    process use one HIP runtime, so device pointers can pass between torch tensors and
+    this library; loaded afterwards, torch would find the device already claimed by a
+    second runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    _share_hip_runtime_with_torch()
     if not LIB_PATH.exists():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
@@ -107,6 +188,22 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_knn_ex.restype = ctypes.c_int
     lib.scrust_hip_knn_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
     lib.scrust_hip_knn_device.restype = ctypes.c_int
+    lib.scrust_hip_batch_create.argtypes = [vp, ctypes.POINTER(Dataset), sz, opts, ctypes.POINTER(vp)]
+    lib.scrust_hip_batch_create.restype = ctypes.c_int
+    lib.scrust_hip_batch_run.argtypes = [vp, ctypes.POINTER(BatchStats)]
+    lib.scrust_hip_batch_run.restype = ctypes.c_int
+    lib.scrust_hip_batch_destroy.argtypes = [vp]
+    lib.scrust_hip_batch_destroy.restype = None
+    lib.scrust_hip_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    lib.scrust_hip_server_create.restype = ctypes.c_int
+    lib.scrust_hip_server_submit.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+    lib.scrust_hip_server_submit.restype = ctypes.c_int
+    lib.scrust_hip_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(RequestInfo)]
+    lib.scrust_hip_server_wait.restype = ctypes.c_int
+    lib.scrust_hip_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    lib.scrust_hip_server_stats_get.restype = ctypes.c_int
+    lib.scrust_hip_server_destroy.argtypes = [vp]
+    lib.scrust_hip_server_destroy.restype = None
     _lib = lib
     return lib
 

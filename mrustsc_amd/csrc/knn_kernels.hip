@@ -1,16 +1,15 @@
-// kNN kernels for gfx950.
+// kNN kernels for gfx950, except the candidate pass (knn_pass.hip).
 //
-// Pipeline for one embedding of n cells x d dims (one HIP stream):
+// Pipeline for a set of embeddings (one HIP stream; a single call is a set of one):
 //   1. colsum / colmean   exact f32 column sums -> f64 means (Euclidean only)
 //   2. prepare            centred (or unit-scaled) f32 coordinates, f32 fma squared norms,
 //                         packed MFMA operands, max norm, non-finite flag
-//   3. pass_bf16/pass_fp32  MFMA tile contraction of |b|^2 - 2 a.b fused with per-row
-//                         candidate selection (KP = k + margin candidates kept per row)
+//   3. candidate pass     (knn_pass.hip) MFMA contraction + per-row candidate selection
 //   4. rerank             fp32 direct differences of the KP candidates, (d^2, id) order,
 //                         fp64 boundary near-tie rule, completeness certificate
-//   5. exact              fp32 direct differences over every reference for rows listed by
-//                         the rerank (or for every row on the exact path)
-// The reference's semantics these restate are cited at each kernel.
+//   5. exact              fp32 direct differences over every reference for listed rows
+// Every kernel reads per-dataset DatasetDesc records and maps flat row indices onto
+// datasets. The reference's semantics these restate are cited at each kernel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,6 +34,18 @@ void launch_fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t stre
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, count, value);
 }
 
+// Dataset owning flat index `row`, given each dataset's first index (ascending).
+__device__ __forceinline__ uint32_t find_set(const DatasetDesc* sets, uint32_t count,
+                                             uint32_t row, bool prep) {
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        const uint32_t begin = prep ? sets[mid].prep_begin : sets[mid].row_begin;
+        if (begin <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ===========================================================================
 // 1. Column means (crates/scrust-core/src/neighbors.rs:144-154: f64 means).
 //    The sum is exact and rounded once; see exact_sum_add.
@@ -42,146 +53,165 @@ void launch_fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t stre
 
 constexpr int kColsumRows = 2048;
 
-__global__ __launch_bounds__(64) void colsum_kernel(const float* __restrict__ x, size_t n,
-                                                    int d, unsigned long long* acc_out) {
+__global__ __launch_bounds__(64) void colsum_kernel(const DatasetDesc* __restrict__ sets) {
     __shared__ long long acc[kWave][kAccWords + 1];
+    const DatasetDesc& s = sets[blockIdx.z];
     const int lane = threadIdx.x;
-    const int col = blockIdx.y * kWave + lane;
-    for (int w = 0; w < kAccWords; w++) acc[lane][w] = 0;
+    const uint32_t col = blockIdx.y * kWave + lane;
     const size_t r0 = static_cast<size_t>(blockIdx.x) * kColsumRows;
+    if (r0 >= s.n || blockIdx.y * kWave >= s.d) return;
+    for (int w = 0; w < kAccWords; w++) acc[lane][w] = 0;
     size_t r1 = r0 + kColsumRows;
-    if (r1 > n) r1 = n;
-    if (col < d) {
-        for (size_t r = r0; r < r1; r++) exact_sum_add(acc[lane], x[r * d + col]);
+    if (r1 > s.n) r1 = s.n;
+    if (col < s.d) {
+        for (size_t r = r0; r < r1; r++) exact_sum_add(acc[lane], s.x[r * s.d + col]);
         for (int w = 0; w < kAccWords; w++) {
             long long v = acc[lane][w];
-            if (v) atomicAdd(&acc_out[col * kAccWords + w], static_cast<unsigned long long>(v));
+            if (v) atomicAdd(&s.acc[col * kAccWords + w], static_cast<unsigned long long>(v));
         }
     }
 }
 
-__global__ void colmean_kernel(const unsigned long long* acc, int d, size_t n, double* mean) {
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= d) return;
+__global__ void colmean_kernel(const DatasetDesc* __restrict__ sets) {
+    const DatasetDesc& s = sets[blockIdx.y];
+    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= s.d) return;
     long long words[kAccWords];
-    for (int w = 0; w < kAccWords; w++) words[w] = static_cast<long long>(acc[col * kAccWords + w]);
-    mean[col] = exact_sum_value(words) / static_cast<double>(n);
+    for (int w = 0; w < kAccWords; w++) words[w] = static_cast<long long>(s.acc[col * kAccWords + w]);
+    s.mean[col] = exact_sum_value(words) / static_cast<double>(s.n);
 }
 
-void launch_column_means(const float* x, size_t n, int d, unsigned long long* acc, double* mean,
-                         hipStream_t stream) {
-    launch_fill_u32(reinterpret_cast<uint32_t*>(acc), static_cast<size_t>(d) * kAccWords * 2, 0u,
-                    stream);
-    dim3 grid(static_cast<unsigned>((n + kColsumRows - 1) / kColsumRows),
-              static_cast<unsigned>((d + kWave - 1) / kWave));
-    hipLaunchKernelGGL(colsum_kernel, grid, dim3(kWave), 0, stream, x, n, d, acc);
-    hipLaunchKernelGGL(colmean_kernel, dim3((d + 63) / 64), dim3(64), 0, stream, acc, d, n, mean);
+// Zero each dataset's column sums and max norm.
+__global__ void clear_sums_kernel(const DatasetDesc* __restrict__ sets) {
+    const DatasetDesc& s = sets[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < s.d * kAccWords; i += blockDim.x) s.acc[i] = 0ull;
+    if (threadIdx.x == 0) *s.max_norm_bits = 0u;
 }
 
 // ===========================================================================
 // 2. Coordinates, norms and MFMA operands.
 //    Euclidean: (x as f64 - mean) rounded once (neighbors.rs:155-160); norms by f32
 //    fma in dimension order (knn.rs:148-156). Cosine: x / |x| with the norm in f64.
-//    Operand layouts are described in knn_pass.hip. Thread n writes the sentinel
-//    reference row (norm feature +inf).
+//    Operand layouts are described in knn_pass.hip. Row n of each dataset writes the
+//    sentinel reference row (norm feature +inf).
 // ===========================================================================
 
-__global__ void prepare_kernel(const float* __restrict__ x, size_t n, int d, int metric,
-                               const double* __restrict__ mean, float* __restrict__ coords,
-                               float* __restrict__ norms, int operand, uint16_t* __restrict__ pack_ref,
-                               uint16_t* __restrict__ pack_query, float* __restrict__ pack_f32,
-                               int nch, int group, unsigned* __restrict__ max_norm_bits,
+__global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t count,
+                               uint32_t total_prep, int metric, int operand,
                                unsigned* __restrict__ flags) {
-    const size_t row = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t flat = blockIdx.x * blockDim.x + threadIdx.x;
     constexpr uint16_t kBf16Inf = 0x7F80u, kBf16MinusHalf = 0xBF00u;
     float nrm = 0.0f;
     bool finite = true;
-    if (row < n) {
-        const float* src = x + row * d;
-        float* dst = coords + row * d;
-        double length = 0.0;
-        if (metric == kMetricCosine) {
-            double s = 0.0;
-            for (int j = 0; j < d; j++) s = __builtin_fma((double)src[j], (double)src[j], s);
-            length = __builtin_sqrt(s);
-        }
-        for (int j = 0; j < d; j++) {
-            float v = src[j];
-            finite = finite && __builtin_isfinite(v);
-            float c;
-            if (metric == kMetricCosine)
-                c = length > 0.0 ? static_cast<float>(static_cast<double>(v) / length) : 0.0f;
-            else
-                c = static_cast<float>(static_cast<double>(v) - mean[j]);
-            dst[j] = c;
-            nrm = __builtin_fmaf(c, c, nrm);
-        }
-        norms[row] = nrm;
-        if (operand == kOperandBf16) {
-            const int kc = 32 * nch;
-            uint16_t* r = pack_ref + row * kc;
-            uint16_t* q = pack_query + row * kc;
+    uint32_t set = 0;
+    bool has_norm = false;
+    if (flat < total_prep) {
+        set = find_set(sets, count, flat, true);
+        const DatasetDesc& s = sets[set];
+        const size_t row = flat - s.prep_begin;
+        const int d = static_cast<int>(s.d);
+        const int nch = static_cast<int>(s.nch), group = static_cast<int>(s.group);
+        const int op = s.on_pass ? operand : kOperandNone;
+        if (row < s.n) {
+            has_norm = true;
+            const float* src = s.x + row * d;
+            float* dst = s.coords + row * d;
+            double length = 0.0;
+            if (metric == kMetricCosine) {
+                double acc = 0.0;
+                for (int j = 0; j < d; j++) acc = __builtin_fma((double)src[j], (double)src[j], acc);
+                length = __builtin_sqrt(acc);
+            }
             for (int j = 0; j < d; j++) {
-                float c = dst[j];
-                uint16_t h = bf16_rn(c);
-                uint16_t l = bf16_rn(c - bf16_value(h));
-                r[j] = h;
-                r[d + j] = l;
-                r[2 * d + j] = h;
-                q[j] = h;
-                q[d + j] = h;
-                q[2 * d + j] = l;
+                const float v = src[j];
+                finite = finite && __builtin_isfinite(v);
+                float c;
+                if (metric == kMetricCosine)
+                    c = length > 0.0 ? static_cast<float>(static_cast<double>(v) / length) : 0.0f;
+                else
+                    c = static_cast<float>(static_cast<double>(v) - s.mean[j]);
+                dst[j] = c;
+                nrm = __builtin_fmaf(c, c, nrm);
             }
-            const uint16_t nh = bf16_rn(nrm);
-            const float rest = nrm - bf16_value(nh);
-            const uint16_t nl = bf16_rn(rest);
-            const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
-            r[3 * d] = nh;
-            r[3 * d + 1] = nl;
-            r[3 * d + 2] = nl2;
-            q[3 * d] = kBf16MinusHalf;
-            q[3 * d + 1] = kBf16MinusHalf;
-            q[3 * d + 2] = kBf16MinusHalf;
-            for (int p = 3 * d + 3; p < kc; p++) {
-                r[p] = 0;
-                q[p] = 0;
+            s.norms[row] = nrm;
+            if (op == kOperandBf16) {
+                const int kc = 32 * nch;
+                uint16_t* r = s.pack_ref + row * kc;
+                uint16_t* q = s.pack_query + row * kc;
+                for (int j = 0; j < d; j++) {
+                    const float c = dst[j];
+                    const uint16_t h = bf16_rn(c);
+                    const uint16_t l = bf16_rn(c - bf16_value(h));
+                    r[j] = h;
+                    r[d + j] = l;
+                    r[2 * d + j] = h;
+                    q[j] = h;
+                    q[d + j] = h;
+                    q[2 * d + j] = l;
+                }
+                const uint16_t nh = bf16_rn(nrm);
+                const float rest = nrm - bf16_value(nh);
+                const uint16_t nl = bf16_rn(rest);
+                const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
+                r[3 * d] = nh;
+                r[3 * d + 1] = nl;
+                r[3 * d + 2] = nl2;
+                q[3 * d] = kBf16MinusHalf;
+                q[3 * d + 1] = kBf16MinusHalf;
+                q[3 * d + 2] = kBf16MinusHalf;
+                for (int p = 3 * d + 3; p < kc; p++) {
+                    r[p] = 0;
+                    q[p] = 0;
+                }
+            } else if (op == kOperandF32) {
+                float* p = s.pack_f32 + row * (4 * group);
+                for (int t = 0; t < 4 * group; t++) {
+                    const int g = t / group, c = t % group;
+                    const int f = g * nch + c;
+                    p[t] = c >= nch ? 0.0f : (f < d ? dst[f] : (f == d ? nrm : 0.0f));
+                }
             }
-        } else if (operand == kOperandF32) {
-            float* p = pack_f32 + row * (4 * group);
-            for (int s = 0; s < 4 * group; s++) {
-                int g = s / group, c = s % group;
-                int f = g * nch + c;
-                p[s] = c >= nch ? 0.0f : (f < d ? dst[f] : (f == d ? nrm : 0.0f));
-            }
-        }
-    } else if (row == n) {
-        if (operand == kOperandBf16) {
+        } else if (op == kOperandBf16) {
             const int kc = 32 * nch;
-            uint16_t* r = pack_ref + row * kc;
+            uint16_t* r = s.pack_ref + row * kc;
             for (int p = 0; p < kc; p++) r[p] = (p == 3 * d) ? kBf16Inf : 0;
-        } else if (operand == kOperandF32) {
-            float* p = pack_f32 + row * (4 * group);
-            for (int s = 0; s < 4 * group; s++) {
-                int g = s / group, c = s % group;
-                p[s] = (c < nch && g * nch + c == d) ? kInf : 0.0f;
+        } else if (op == kOperandF32) {
+            float* p = s.pack_f32 + row * (4 * group);
+            for (int t = 0; t < 4 * group; t++) {
+                const int g = t / group, c = t % group;
+                p[t] = (c < nch && g * nch + c == d) ? kInf : 0.0f;
             }
         }
     }
-    // Wave-reduced max norm (norms are >= 0, so their bits order as unsigned ints).
-    unsigned bits = row < n ? __float_as_uint(nrm) : 0u;
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned other = __shfl_xor(bits, off, kWave);
-        bits = other > bits ? other : bits;
+    // Max squared norm per dataset (norms are >= 0: their bits order as unsigned ints).
+    // Wave-reduce when the whole wave belongs to one dataset, else one atomic per lane.
+    const uint32_t first_set = __shfl(set, 0, kWave);
+    const bool uniform = __ballot(set != first_set) == 0ull;
+    if (uniform) {
+        unsigned bits = has_norm ? __float_as_uint(nrm) : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned other = __shfl_xor(bits, off, kWave);
+            bits = other > bits ? other : bits;
+        }
+        if ((threadIdx.x & 63) == 0 && flat < total_prep) atomicMax(sets[set].max_norm_bits, bits);
+    } else if (has_norm) {
+        atomicMax(sets[set].max_norm_bits, __float_as_uint(nrm));
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(max_norm_bits, bits);
     if (!finite) atomicOr(flags, kFlagNonFinite);
 }
 
-void launch_prepare(const PrepareArgs& a, hipStream_t stream) {
-    unsigned blocks = static_cast<unsigned>((a.n + 1 + 255) / 256);
-    hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, a.x, a.n, a.d,
-                       a.metric, a.mean, a.coords, a.norms, a.operand, a.pack_ref, a.pack_query,
-                       a.pack_f32, a.nch, a.group, a.max_norm_bits, a.flags);
+void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsigned* flags,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(clear_sums_kernel, dim3(shape.count), dim3(256), 0, stream, sets);
+    if (shape.metric == kMetricEuclidean) {
+        dim3 grid(shape.max_row_chunks, shape.max_col_groups, shape.count);
+        hipLaunchKernelGGL(colsum_kernel, grid, dim3(kWave), 0, stream, sets);
+        hipLaunchKernelGGL(colmean_kernel, dim3(shape.max_col_groups, shape.count), dim3(kWave), 0,
+                           stream, sets);
+    }
+    const unsigned blocks = (shape.total_prep + 255) / 256;
+    hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+                       shape.total_prep, shape.metric, shape.operand, flags);
 }
 
 // ===========================================================================
@@ -191,24 +221,31 @@ void launch_prepare(const PrepareArgs& a, hipStream_t stream) {
 // ===========================================================================
 
 template <int KP>
-__global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
+__global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restrict__ sets,
+                                                     uint32_t count, uint32_t total_rows,
+                                                     int metric, uint32_t* __restrict__ fail_list,
+                                                     uint32_t* __restrict__ fail_count) {
     constexpr int QPW = kWave / KP;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int seg = lane & ~(KP - 1);
     const int j = lane & (KP - 1);
-    const uint32_t q = (blockIdx.x * 4 + w) * QPW + lane / KP;
-    const bool row_ok = q < a.n;
-    const uint32_t qq = row_ok ? q : a.n - 1;
-    const int d = a.d;
-    const float* xq = a.coords + static_cast<size_t>(qq) * d;
-    const float nq = a.norms[qq];
+    const uint32_t flat0 = (blockIdx.x * 4 + w) * QPW + lane / KP;
+    const uint32_t flat = flat0 < total_rows ? flat0 : total_rows - 1;
+    const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+    const bool row_ok = flat0 < total_rows && s.on_pass && s.kp == KP;
+    const uint32_t q = flat - s.row_begin;
+    const uint32_t n = s.n;
+    const int d = static_cast<int>(s.d);
+    const float* xq = s.coords + static_cast<size_t>(q) * d;
+    const float nq = s.norms[q];
 
-    const uint64_t raw = a.cand[static_cast<size_t>(qq) * KP + j];
+    uint64_t raw = kNoKey;
+    if (row_ok) raw = s.cand[static_cast<size_t>(q) * KP + j];
     const uint32_t id = key_id(raw);
     const bool present = raw != kNoKey;
-    const bool valid = present && id != qq && id < a.n;
+    const bool valid = present && id != q && id < n;
     float dsq = kInf;
-    if (valid) dsq = direct_square(xq, a.coords + static_cast<size_t>(id) * d, d, nq, a.norms[id]);
+    if (valid) dsq = direct_square(xq, s.coords + static_cast<size_t>(id) * d, d, nq, s.norms[id]);
     uint64_t key = valid ? make_key(dsq, id) : kNoKey;
     key = segment_sort<KP>(key, lane);
 
@@ -218,23 +255,24 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     const bool full = (__ballot(present) & seg_mask) == seg_mask;
     const float my_d = key_value(key);
     const uint32_t my_id = key_id(key);
-    const int k = a.k;
-    const float t_f = __shfl(my_d, seg + k - 1, kWave);
+    const int k = static_cast<int>(s.k);
+    const int kk = row_ok ? k : 1;
+    const float t_f = __shfl(my_d, seg + kk - 1, kWave);
     const double t = static_cast<double>(t_f);
     const double tau = t * static_cast<double>(d + 2) * 0x1p-23;
     const bool in_band = (j < n_valid) && __builtin_fabs(static_cast<double>(my_d) - t) <= tau;
     const uint64_t band_mask = __ballot(in_band) & seg_mask;
-    const bool has_next = n_valid > k;
-    const double next = static_cast<double>(__shfl(my_d, seg + (k < KP ? k : KP - 1), kWave));
-    const bool tie = has_next && (next - t <= tau);
+    const bool has_next = n_valid > kk;
+    const double next = static_cast<double>(__shfl(my_d, seg + (kk < KP ? kk : KP - 1), kWave));
+    const bool tie = row_ok && has_next && (next - t <= tau);
 
     int pos = j;
-    if (__ballot(tie && row_ok) != 0) {
+    if (__ballot(tie) != 0) {
         const int lo = band_mask ? __builtin_ctzll(band_mask) - seg : 0;
         const int hi = band_mask ? 63 - __builtin_clzll(band_mask) - seg : -1;
         double d64 = 0.0;
-        if (in_band && my_d != 0.0f)
-            d64 = direct_square_f64(xq, a.coords + static_cast<size_t>(my_id) * d, d);
+        if (tie && in_band && my_d != 0.0f)
+            d64 = direct_square_f64(xq, s.coords + static_cast<size_t>(my_id) * d, d);
         int rank = 0;
         for (int i = 0; i < KP; i++) {
             const double od = __shfl(d64, seg + i, kWave);
@@ -243,51 +281,58 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
         }
         if (tie && in_band) pos = lo + rank;
     }
+    if (!row_ok) return;
 
     // Certificate: every reference outside the kept list has approximate score
-    // >= theta, so its fp32 squared distance is >= L below; if L clears the band's
-    // upper edge and every snapping threshold, the kept list holds every reference
-    // the oracle could place in the first k (or in the tie band).
+    // >= theta, so its fp32 squared distance is >= `lower`; if that clears the band's
+    // upper edge and every snapping threshold, the kept list holds every reference the
+    // oracle could place in the first k (or in the tie band).
     bool certified = true;
     if (full) {
         const double u = 0x1p-24;
         const double nqd = static_cast<double>(nq);
-        const double nmax = static_cast<double>(__uint_as_float(*a.max_norm_bits));
+        const double nmax = static_cast<double>(__uint_as_float(*s.max_norm_bits));
         const double gamma = d * u / (1.0 - d * u);
         const double qn = __builtin_sqrt(nqd * (1.0 + gamma));
         const double rn = __builtin_sqrt(nmax * (1.0 + gamma));
         // |score + |a|^2 - exact d^2| <= err: f32 norms, the bf16 split of the
         // coordinates and of |b|^2, and the MFMA accumulation over every packed term.
-        const double err = 1.25 * (gamma * (nqd + nmax) * 1.01 + 2.0 * a.split_rel * qn * rn +
-                                   2.0 * a.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
-                                   a.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
+        const double err = 1.25 * (gamma * (nqd + nmax) * 1.01 + 2.0 * s.split_rel * qn * rn +
+                                   2.0 * s.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
+                                   s.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
         const double cd = (d + 3) * u * 1.01;
-        const double lower = (static_cast<double>(a.theta[qq]) + nqd - err) * (1.0 - cd);
+        const double lower = (static_cast<double>(s.theta[q]) + nqd - err) * (1.0 - cd);
         const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);
         const double need = (t + tau) > snap_max ? (t + tau) : snap_max;
         certified = lower > need;
     }
 
-    if (!row_ok) return;
     if (certified) {
         if (pos < k && j < n_valid) {
-            a.out_indices[static_cast<size_t>(q) * k + pos] = my_id;
-            a.out_distances[static_cast<size_t>(q) * k + pos] =
-                a.metric == kMetricCosine ? 0.5f * my_d : __builtin_sqrtf(my_d);
+            s.out_indices[static_cast<size_t>(q) * k + pos] = my_id;
+            s.out_distances[static_cast<size_t>(q) * k + pos] =
+                metric == kMetricCosine ? 0.5f * my_d : __builtin_sqrtf(my_d);
         }
     } else if (j == 0) {
-        const uint32_t slot = atomicAdd(a.fail_count, 1u);
-        a.fail_list[slot] = q;
+        const uint32_t slot = atomicAdd(fail_count, 1u);
+        fail_list[slot] = flat;
     }
 }
 
-void launch_rerank(const RerankArgs& a, hipStream_t stream) {
-    const unsigned rows_per_block = 4 * (kWave / a.kp);
-    const unsigned blocks = static_cast<unsigned>((a.n + rows_per_block - 1) / rows_per_block);
-    if (a.kp == 32)
-        hipLaunchKernelGGL(rerank_kernel<32>, dim3(blocks), dim3(256), 0, stream, a);
-    else
-        hipLaunchKernelGGL(rerank_kernel<64>, dim3(blocks), dim3(256), 0, stream, a);
+void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp32,
+                   bool has_kp64, uint32_t* fail_list, uint32_t* fail_count,
+                   hipStream_t stream) {
+    if (shape.total_rows == 0) return;
+    if (has_kp32) {
+        const unsigned blocks = (shape.total_rows + 7) / 8;
+        hipLaunchKernelGGL(rerank_kernel<32>, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+                           shape.total_rows, shape.metric, fail_list, fail_count);
+    }
+    if (has_kp64) {
+        const unsigned blocks = (shape.total_rows + 3) / 4;
+        hipLaunchKernelGGL(rerank_kernel<64>, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+                           shape.total_rows, shape.metric, fail_list, fail_count);
+    }
 }
 
 // ===========================================================================
@@ -300,7 +345,7 @@ constexpr int kExactBuf = 1024;
 constexpr int kExactKeep = 512;
 
 __device__ void lds_sort(uint64_t* keys, int count) {
-    // Bitonic sort of kExactBuf keys by 256 threads (entries past count are kNoKey).
+    // Bitonic sort of kExactBuf keys by the workgroup (entries past count are kNoKey).
     for (int i = threadIdx.x; i < kExactBuf; i += blockDim.x)
         if (i >= count) keys[i] = kNoKey;
     __syncthreads();
@@ -310,7 +355,7 @@ __device__ void lds_sort(uint64_t* keys, int count) {
                 const int lo = (p / stride) * stride * 2 + (p % stride);
                 const int hi = lo + stride;
                 const bool asc = (lo & size) == 0;
-                uint64_t x = keys[lo], y = keys[hi];
+                const uint64_t x = keys[lo], y = keys[hi];
                 if ((x > y) == asc) {
                     keys[lo] = y;
                     keys[hi] = x;
@@ -321,52 +366,59 @@ __device__ void lds_sort(uint64_t* keys, int count) {
     }
 }
 
-__global__ __launch_bounds__(256) void exact_kernel(ExactArgs a) {
+__global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restrict__ sets,
+                                                    uint32_t count, uint32_t total_rows,
+                                                    int metric, const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ list_count,
+                                                    unsigned* __restrict__ flags) {
     extern __shared__ float xq[];
     __shared__ uint64_t keys[kExactBuf];
-    __shared__ uint32_t count;
+    __shared__ uint32_t kept;
     __shared__ uint64_t theta;
     __shared__ int dropped;
     __shared__ uint32_t out_id[kMaxK];
     __shared__ float out_d[kMaxK];
-    __shared__ double bd[kExactKeep];
-    __shared__ uint32_t bid[kExactKeep];
-    __shared__ float bf[kExactKeep];
+    __shared__ double bd[kBandCap];
+    __shared__ uint32_t bid[kBandCap];
+    __shared__ float bf[kBandCap];
     const int tid = threadIdx.x;
-    const int d = a.d, k = a.k;
-    const uint32_t total = a.list ? *a.list_count : a.n;
+    const uint32_t total = list ? *list_count : total_rows;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const uint32_t q = a.list ? a.list[item] : item;
-        for (int j = tid; j < d; j += blockDim.x) xq[j] = a.coords[static_cast<size_t>(q) * d + j];
+        const uint32_t flat = list ? list[item] : item;
+        const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+        const uint32_t q = flat - s.row_begin;
+        const int d = static_cast<int>(s.d), k = static_cast<int>(s.k);
+        const uint32_t n = s.n;
+        for (int j = tid; j < d; j += blockDim.x) xq[j] = s.coords[static_cast<size_t>(q) * d + j];
         if (tid == 0) {
-            count = 0;
+            kept = 0;
             theta = kNoKey;
             dropped = 0;
         }
         __syncthreads();
-        const float nq = a.norms[q];
-        for (uint32_t r0 = 0; r0 < a.n; r0 += blockDim.x) {
+        const float nq = s.norms[q];
+        for (uint32_t r0 = 0; r0 < n; r0 += blockDim.x) {
             const uint32_t r = r0 + tid;
-            if (r < a.n && r != q) {
-                const float s = direct_square(xq, a.coords + static_cast<size_t>(r) * d, d, nq, a.norms[r]);
-                const uint64_t key = make_key(s, r);
+            if (r < n && r != q) {
+                const float v = direct_square(xq, s.coords + static_cast<size_t>(r) * d, d, nq, s.norms[r]);
+                const uint64_t key = make_key(v, r);
                 if (key < theta) {
-                    const uint32_t slot = atomicAdd(&count, 1u);
+                    const uint32_t slot = atomicAdd(&kept, 1u);
                     keys[slot] = key;
                 }
             }
             __syncthreads();
-            if (count > static_cast<uint32_t>(kExactBuf) - blockDim.x) {
-                lds_sort(keys, count);
+            if (kept > static_cast<uint32_t>(kExactBuf) - blockDim.x) {
+                lds_sort(keys, kept);
                 if (tid == 0) {
-                    count = kExactKeep;
+                    kept = kExactKeep;
                     theta = keys[kExactKeep - 1];
                     dropped = 1;
                 }
                 __syncthreads();
             }
         }
-        const int m = static_cast<int>(count);
+        const int m = static_cast<int>(kept);
         lds_sort(keys, m);
         if (tid == 0) {
             // (d^2, id) order, then the fp64 boundary rule on the sorted keys.
@@ -381,30 +433,32 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactArgs a) {
                 int lo = k - 1;
                 while (lo > 0 && t - static_cast<double>(key_value(keys[lo - 1])) <= tau) lo--;
                 int hi = k;
-                while (hi + 1 < m && static_cast<double>(key_value(keys[hi + 1])) - t <= tau) hi++;
-                // A band wider than kBandCap keeps the (d^2, id) order. After a
-                // compaction m >= kExactKeep and lo < k <= kMaxK, so a band that reaches
-                // the end of the kept keys is always wider than the cap.
-                if (hi - lo + 1 > kBandCap || (hi + 1 == m && dropped)) {
-                    if (hi - lo + 1 <= kBandCap) atomicOr(a.flags, kFlagBandOverflow);
-                } else {
+                while (hi + 1 < m && hi - lo + 1 <= kBandCap &&
+                       static_cast<double>(key_value(keys[hi + 1])) - t <= tau)
+                    hi++;
+                // A band wider than kBandCap keeps the (d^2, id) order. After a compaction
+                // m >= kExactKeep and lo < k <= kMaxK, so a band that reaches the end of
+                // the kept keys is always wider than the cap.
+                const int len = hi - lo + 1;
+                if (len <= kBandCap && hi + 1 == m && dropped) {
+                    atomicOr(flags, kFlagBandOverflow);
+                } else if (len <= kBandCap) {
                     // Insertion sort of the band by (fp64 d^2, id); the band is small.
-                    const int len = hi - lo + 1;
                     for (int i = 0; i < len; i++) {
                         const uint64_t key = keys[lo + i];
                         const float v = key_value(key);
                         const uint32_t id = key_id(key);
-                        const double v64 = v == 0.0f ? 0.0 : direct_square_f64(xq, a.coords + static_cast<size_t>(id) * d, d);
-                        int s = i;
-                        while (s > 0 && (bd[s - 1] > v64 || (bd[s - 1] == v64 && bid[s - 1] > id))) {
-                            bd[s] = bd[s - 1];
-                            bid[s] = bid[s - 1];
-                            bf[s] = bf[s - 1];
-                            s--;
+                        const double v64 = v == 0.0f ? 0.0 : direct_square_f64(xq, s.coords + static_cast<size_t>(id) * d, d);
+                        int slot = i;
+                        while (slot > 0 && (bd[slot - 1] > v64 || (bd[slot - 1] == v64 && bid[slot - 1] > id))) {
+                            bd[slot] = bd[slot - 1];
+                            bid[slot] = bid[slot - 1];
+                            bf[slot] = bf[slot - 1];
+                            slot--;
                         }
-                        bd[s] = v64;
-                        bid[s] = id;
-                        bf[s] = v;
+                        bd[slot] = v64;
+                        bid[slot] = id;
+                        bf[slot] = v;
                     }
                     for (int p = lo; p < k; p++) {
                         out_id[p] = bid[p - lo];
@@ -415,17 +469,20 @@ __global__ __launch_bounds__(256) void exact_kernel(ExactArgs a) {
         }
         __syncthreads();
         for (int j = tid; j < k; j += blockDim.x) {
-            a.out_indices[static_cast<size_t>(q) * k + j] = out_id[j];
-            a.out_distances[static_cast<size_t>(q) * k + j] =
-                a.metric == kMetricCosine ? 0.5f * out_d[j] : __builtin_sqrtf(out_d[j]);
+            s.out_indices[static_cast<size_t>(q) * k + j] = out_id[j];
+            s.out_distances[static_cast<size_t>(q) * k + j] =
+                metric == kMetricCosine ? 0.5f * out_d[j] : __builtin_sqrtf(out_d[j]);
         }
         __syncthreads();
     }
 }
 
-void launch_exact(const ExactArgs& a, unsigned blocks, hipStream_t stream) {
+void launch_exact(const DatasetDesc* sets, const BatchShape& shape, const uint32_t* list,
+                  const uint32_t* list_count, unsigned blocks, unsigned* flags,
+                  hipStream_t stream) {
     if (blocks == 0) return;
-    hipLaunchKernelGGL(exact_kernel, dim3(blocks), dim3(256), a.d * sizeof(float), stream, a);
+    hipLaunchKernelGGL(exact_kernel, dim3(blocks), dim3(256), shape.max_d * sizeof(float), stream,
+                       sets, shape.count, shape.total_rows, shape.metric, list, list_count, flags);
 }
 
 }  // namespace scrust_hip

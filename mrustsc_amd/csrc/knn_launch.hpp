@@ -1,4 +1,6 @@
-// Host-side declarations of the kNN kernel launches (knn_kernels.hip).
+// Host-side declarations of the kNN kernel launches and the per-dataset descriptor every
+// kernel reads. One descriptor describes one embedding; a single call uses one, a
+// batched launch many (the kernels map flat row indices onto datasets).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,96 +19,84 @@ constexpr int kOperandF32 = 2;
 constexpr unsigned kFlagNonFinite = 1u;
 constexpr unsigned kFlagBandOverflow = 2u;
 
-constexpr int kCap = 256;   // candidate buffer slots per query row in the pass
+constexpr int kCapMax = 512;  // largest candidate buffer (slots per query row) of the pass
 // Widest boundary tie band resolved in fp64; wider bands keep the (d^2, id) order.
 // Every rerank band has at most KP - 1 <= 63 members, so the cap only binds on the
 // exact path.
 constexpr int kBandCap = 64;
-constexpr int kMaxK = 256;  // largest k any path serves
-constexpr int kMaxPassK = 47;  // largest k the MFMA candidate pass serves (KP = 64)
-constexpr int kMaxPassD = 128; // largest d with a compiled MFMA variant
+constexpr int kMaxK = 256;      // largest k any path serves
+constexpr int kMaxPassK = 47;   // largest k the MFMA candidate pass serves (KP = 64)
+constexpr int kMaxPassD = 128;  // largest d with a compiled MFMA variant
+constexpr int kAccWordsHost = 10;
+constexpr int kPassRows = 256;  // query rows per pass work item (4 waves x 64)
 
-struct PrepareArgs {
-    const float* x;
-    size_t n;
-    int d;
-    int metric;
-    const double* mean;
-    float* coords;
-    float* norms;
-    int operand;
-    uint16_t* pack_ref;
-    uint16_t* pack_query;
-    float* pack_f32;
-    int nch;
-    int group;
-    unsigned* max_norm_bits;
-    unsigned* flags;
-};
-
-struct PassArgs {
-    int operand;
-    int nch;
-    int kp;
-    int d;
-    const uint16_t* pack_query;
-    const uint16_t* pack_ref;
-    const float* pack_f32;
-    const float* norms;
-    uint32_t n;
-    uint64_t* buf;
-    uint64_t* cand;
-    float* theta;
-};
-
-struct RerankArgs {
-    const float* coords;
-    const float* norms;
-    uint32_t n;
-    int d;
-    int k;
-    int kp;
-    int metric;
-    const uint64_t* cand;
-    const float* theta;
-    const unsigned* max_norm_bits;
-    // Error model of the pass (see the certificate in rerank_kernel):
+// Device-visible description of one embedding and its workspace.
+struct DatasetDesc {
+    const float* x;               // input, n x d row-major
+    float* coords;                // centred / unit-scaled coordinates, n x d
+    float* norms;                 // f32 fma squared norms, n
+    double* mean;                 // column means, d
+    unsigned long long* acc;      // exact column sums, d x kAccWordsHost
+    uint16_t* pack_ref;           // bf16 reference operands, (n + 1) x 32 nch
+    uint16_t* pack_query;         // bf16 query operands, n x 32 nch
+    float* pack_f32;              // fp32 operands, (n + 1) x 4 group
+    uint64_t* buf;                // pass candidate buffers, npad x cap
+    uint64_t* cand;               // kept candidates, n x kp
+    float* theta;                 // kp-th candidate score per row, n
+    unsigned* max_norm_bits;      // max squared norm (as bits)
+    uint32_t* out_indices;        // n x k
+    float* out_distances;         // n x k
+    uint32_t n, d, k, kp, nch, group, cap;
+    uint32_t row_begin;           // first flat row of this dataset
+    uint32_t prep_begin;          // first flat prepare row (n + 1 per dataset)
+    uint32_t on_pass;             // 1: candidate pass + rerank; 0: exact rows only
+    // Error model of the pass (see the certificate in rerank_kernel).
     double split_rel;  // operand split error of a.b, relative to |a||b|
     double acc_rel;    // MFMA accumulation error, relative to the sum of |terms|
     double norm_rel;   // error of the |b|^2 feature, relative to |b|^2
-    uint32_t* out_indices;
-    float* out_distances;
-    uint32_t* fail_list;
-    uint32_t* fail_count;
 };
 
-struct ExactArgs {
-    const float* coords;
-    const float* norms;
-    uint32_t n;
-    int d;
-    int k;
+struct PassItem {
+    uint32_t set;    // dataset index
+    uint32_t block;  // query block of kPassRows rows
+};
+
+struct BatchShape {
+    uint32_t count;           // datasets
+    uint32_t total_rows;      // sum of n
+    uint32_t total_prep;      // sum of n + 1
+    uint32_t max_row_chunks;  // max ceil(n / colsum rows)
+    uint32_t max_col_groups;  // max ceil(d / 64)
+    uint32_t max_d;           // largest d
     int metric;
-    const uint32_t* list;        // rows to compute; nullptr: every row
-    const uint32_t* list_count;  // device count of `list`
-    uint32_t* out_indices;
-    float* out_distances;
-    unsigned* flags;
+    int operand;
 };
 
 void launch_fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t stream);
-void launch_column_means(const float* x, size_t n, int d, unsigned long long* acc, double* mean,
-                         hipStream_t stream);
-void launch_prepare(const PrepareArgs& a, hipStream_t stream);
-bool launch_pass(const PassArgs& a, hipStream_t stream);
-void launch_rerank(const RerankArgs& a, hipStream_t stream);
-void launch_exact(const ExactArgs& a, unsigned blocks, hipStream_t stream);
+// Column means, coordinates, norms and operands of every dataset.
+void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsigned* flags,
+                        hipStream_t stream);
+// Candidate pass of one dataset (compile-time chunk count).
+bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,
+                 hipStream_t stream);
+// Candidate pass of every item of a batch in one persistent launch.
+bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t n_items,
+                       uint32_t* item_counter, int operand, int nch_max, unsigned blocks,
+                       hipStream_t stream);
+// Rerank of every pass row (one launch per candidate width present).
+void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp32,
+                   bool has_kp64, uint32_t* fail_list, uint32_t* fail_count,
+                   hipStream_t stream);
+// Exact rows listed in `list` (flat rows, count on the device).
+void launch_exact(const DatasetDesc* sets, const BatchShape& shape, const uint32_t* list,
+                  const uint32_t* list_count, unsigned blocks, unsigned* flags,
+                  hipStream_t stream);
 
 // MFMA chunk counts of the compiled pass variants for a dimension d (0: none).
 int bf16_chunks(int d);
 int fp32_chunks(int d);
 int fp32_group(int nch);
-
-constexpr int kAccWordsHost = 10;
+// Chunk count of the persistent batched kernel able to serve nch (0: none).
+int batch_chunks(int operand, int nch);
 
 }  // namespace scrust_hip

@@ -1,17 +1,17 @@
 // Candidate pass: MFMA tile contraction of the squared-distance expansion fused with
 // per-row candidate selection (gfx950, wave64).
 //
-// A workgroup owns 256 query rows (4 waves x 64) and streams every reference through LDS
-// in tiles of 64, double buffered, read from HBM with 16-byte coalesced loads. A wave
-// keeps its 64 rows' A fragments in registers for the whole scan; the reference tile
-// supplies the B fragments. The reference's squared norm is one more packed feature, so
-// the MFMA accumulator is acc = a.b - |b|^2 / 2 and the score s = |b|^2 - 2 a.b (the row's
-// |a|^2 is a constant shift) is -2 acc: a lane tests each of its 16 accumulators against
-// its rows' thresholds. References sit on the MFMA's M side and queries on its N side, so
-// the four accumulators a lane holds per 16x16 block belong to one query row and are
-// tested with one max and one compare; a block's tests form one wave-uniform branch.
+// A work item is a block of 256 query rows (4 waves x 64) of one dataset. The workgroup
+// streams every reference of that dataset through LDS in tiles of 64, double buffered,
+// read from HBM with 16-byte coalesced loads. A wave keeps its 64 rows' query fragments
+// in registers for the whole scan; the reference tile supplies the other MFMA operand.
+// The reference's squared norm is one more packed feature, so the MFMA accumulator is
+// acc = a.b - |b|^2 / 2 and the score s = |b|^2 - 2 a.b (the row's |a|^2 is a constant
+// shift) is -2 acc. References sit on the MFMA's M side and queries on its N side, so the
+// four accumulators a lane holds per 16x16 block belong to one query row and are tested
+// with one max and one compare; a block's tests form one wave-uniform branch.
 //
-// Selection: a pair with s < theta[row] is appended to the row's kCap-slot buffer in
+// Selection: a pair with s < theta[row] is appended to the row's `cap`-slot buffer in
 // global memory (slot from an LDS counter). After each tile, a row whose buffer could
 // overflow in the next tile is compacted by its wave (ballot radix select of the KP
 // smallest keys); theta becomes the KP-th smallest score. theta only decreases, so every
@@ -25,6 +25,10 @@
 //         [c | |b|^2 | 0] for references; the kernel puts -1/2 at feature d for queries.
 //   Row n of the reference operands is a sentinel whose norm feature is +inf, so the
 //   padding columns of the last tile score -inf and are never candidates.
+//
+// Two entry points share the block body: pass_kernel (one dataset, chunk count fixed at
+// compile time) and pass_batch_kernel (a persistent grid that takes work items of many
+// datasets from a queue; chunk count per dataset at run time, up to a compiled maximum).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,8 +39,11 @@
 
 namespace scrust_hip {
 
-constexpr int kTileRefs = 64;       // references per LDS tile (4 column blocks of 16)
-constexpr int kRowsPerBlock = 256;  // query rows per workgroup (4 waves x 64)
+constexpr int kTileRefs = 64;  // references per LDS tile (4 column blocks of 16)
+
+// A native 16-byte vector (HIP's uint4 is a class with unions, which keeps arrays of it
+// out of registers).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -51,8 +58,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Keep the `keep` smallest of the `count` keys at rowbuf[0..count) (count >= keep): they
-// are written, unordered, to out[0..keep); returns the keep-th smallest score. Ballot
+// Keep the `keep` smallest of the `count` keys at rowbuf[0..count) (keep <= count <= E*64):
+// they are written, unordered, to out[0..keep); returns the keep-th smallest score. Ballot
 // radix select: a binary search over the ordered score bits, then over ids among keys
 // that tie on the boundary score.
 template <int E>
@@ -107,114 +114,136 @@ __device__ float select_smallest(const uint64_t* rowbuf, uint32_t count, int kee
     return unordered(v);
 }
 
+__device__ __forceinline__ float select_by_cap(uint32_t cap, const uint64_t* rowbuf,
+                                               uint32_t count, int keep, int lane, uint64_t* out) {
+    if (cap <= 128) return select_smallest<2>(rowbuf, count, keep, lane, out);
+    if (cap <= 256) return select_smallest<4>(rowbuf, count, keep, lane, out);
+    return select_smallest<8>(rowbuf, count, keep, lane, out);
+}
+
 template <int OP, int NCH>
 struct Operand;
 
 template <int NCH>
 struct Operand<kOperandBf16, NCH> {
-    static constexpr int kRowU4 = 4 * NCH;  // 16-byte chunks per packed row (32*NCH bf16)
     typedef bf16x8 Frag;
+    // 16-byte chunks per packed row (one more per LDS row for bank spread).
+    static __device__ __forceinline__ int row_u4(int nch) { return 4 * nch; }
+    static constexpr int kRowU4Max = 4 * NCH;
+    static constexpr int kLoadsMax = NCH;  // 64 rows x 4 nch chunks / 256 threads
 };
 
 template <int NCH>
 struct Operand<kOperandF32, NCH> {
-    static constexpr int kGroup = (NCH + 3) / 4 * 4;  // floats per lane group, 16-B aligned
-    static constexpr int kRowU4 = kGroup;              // 4 groups of kGroup floats
     typedef float Frag;
+    static __device__ __forceinline__ int row_u4(int nch) { return (nch + 3) / 4 * 4; }
+    static constexpr int kRowU4Max = (NCH + 3) / 4 * 4;
+    static constexpr int kLoadsMax = kRowU4Max / 4;
 };
 
-// At most two waves per SIMD: the allocator then keeps the tile prefetch in registers
-// instead of spilling it to reach a higher occupancy the LDS budget would allow.
-template <int OP, int NCH, int KP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void pass_kernel(PassArgs p) {
-    typedef Operand<OP, NCH> Op;
-    constexpr int ROW_U4 = Op::kRowU4;
-    constexpr int STRIDE = ROW_U4 + 1;  // LDS row stride in 16-byte chunks (bank spread)
-    constexpr int LOADS = kTileRefs * ROW_U4 / 256;
-    constexpr int E = kCap / kWave;
-    static_assert(kTileRefs * ROW_U4 % 256 == 0, "tile load must split evenly");
-    __shared__ uint4 tile[2][kTileRefs * STRIDE];
-    __shared__ uint32_t cnt[kRowsPerBlock];
-    __shared__ float phl[kRowsPerBlock];
+template <int OP, int NCH>
+struct PassSmem {
+    u32x4 tile[2][kTileRefs * (Operand<OP, NCH>::kRowU4Max + 1)];
+    uint32_t cnt[kPassRows];
+    float phl[kPassRows];
+};
 
+// One work item: query rows [block * 256, block * 256 + 256) of dataset s against all of
+// its references. FIXED: the dataset's chunk count equals NCH at compile time.
+template <int OP, int NCH, bool FIXED>
+__device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
+                                           PassSmem<OP, NCH>& sm) {
+    typedef Operand<OP, NCH> Op;
+    const int nch = FIXED ? NCH : static_cast<int>(s.nch);
+    const int row_u4 = FIXED ? Op::kRowU4Max : Op::row_u4(nch);
+    const int stride = row_u4 + 1;
+    const int loads = FIXED ? Op::kLoadsMax : row_u4 / 4;
+    const int group = row_u4;  // fp32: floats per lane group (4 groups per row)
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, lc = lane & 15;
-    const uint32_t n = p.n;
-    const uint32_t qbase = blockIdx.x * kRowsPerBlock;
-    const uint4* ref_rows = reinterpret_cast<const uint4*>(OP == kOperandBf16
-                                                               ? static_cast<const void*>(p.pack_ref)
-                                                               : static_cast<const void*>(p.pack_f32));
+    const uint32_t n = s.n;
+    const uint32_t cap = s.cap;
+    const int kp = static_cast<int>(s.kp);
+    const uint32_t qbase = block * kPassRows;
+    const u32x4* ref_rows = reinterpret_cast<const u32x4*>(
+        OP == kOperandBf16 ? static_cast<const void*>(s.pack_ref) : static_cast<const void*>(s.pack_f32));
 
     // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
     // qb's fragment in lane l carries row qb*16 + l%16.
     typename Op::Frag a[4][NCH];
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        uint32_t row = qbase + w * kWave + rb * 16 + lc;
+    for (int qb = 0; qb < 4; qb++) {
+        uint32_t row = qbase + w * kWave + qb * 16 + lc;
         if (row >= n) row = n - 1;
         if constexpr (OP == kOperandBf16) {
-            const uint4* src = reinterpret_cast<const uint4*>(p.pack_query) + static_cast<size_t>(row) * ROW_U4 + g * NCH;
-#pragma unroll
-            for (int c = 0; c < NCH; c++) a[rb][c] = __builtin_bit_cast(bf16x8, src[c]);
-        } else {
-            const float* src = p.pack_f32 + static_cast<size_t>(row) * 4 * Op::kGroup + g * Op::kGroup;
+            const u32x4* src = reinterpret_cast<const u32x4*>(s.pack_query) + static_cast<size_t>(row) * row_u4 + g * nch;
 #pragma unroll
             for (int c = 0; c < NCH; c++)
-                a[rb][c] = (g * NCH + c == p.d) ? -0.5f : src[c];
+                a[qb][c] = (FIXED || c < nch) ? __builtin_bit_cast(bf16x8, src[c]) : bf16x8{};
+        } else {
+            const float* src = s.pack_f32 + static_cast<size_t>(row) * 4 * group + g * group;
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+                a[qb][c] = (FIXED || c < nch) ? ((g * nch + c == static_cast<int>(s.d)) ? -0.5f : src[c]) : 0.0f;
         }
     }
-    cnt[tid] = 0;
-    phl[tid] = -kInf;
+    sm.cnt[tid] = 0;
+    sm.phl[tid] = -kInf;
     // phi[qb] = -theta/2 of this lane's query row qb*16 + l%16.
     float phi[4];
 #pragma unroll
     for (int qb = 0; qb < 4; qb++) phi[qb] = -kInf;
 
     const uint32_t n_tiles = (n + kTileRefs - 1) / kTileRefs;
-    uint4 pre[LOADS];
+    u32x4 pre[Op::kLoadsMax];
     // Global -> registers for tile t (row n is the sentinel); registers -> LDS buffer b.
-    // Indices are compile-time constants so `pre` stays in registers.
-    auto load_tile = [&](uint32_t t) __attribute__((always_inline)) {
-        static_for<0, LOADS>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            const int u = tid + 256 * i;
-            const int rl = u / ROW_U4, part = u % ROW_U4;
-            uint32_t ref = t * kTileRefs + rl;
-            if (ref > n) ref = n;
-            pre[i] = ref_rows[static_cast<size_t>(ref) * ROW_U4 + part];
-        });
-    };
-    auto store_tile = [&](int b) __attribute__((always_inline)) {
-        static_for<0, LOADS>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            const int u = tid + 256 * i;
-            const int rl = u / ROW_U4, part = u % ROW_U4;
-            tile[b][rl * STRIDE + part] = pre[i];
-        });
-    };
+    // Plain unrolled loops with constant indices keep `pre` in registers.
+#define SCRUST_LOAD_TILE(T)                                                       \
+    _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
+        if (FIXED || i < loads) {                                                 \
+            const int u = tid + 256 * i;                                          \
+            const int rl = u / row_u4, part = u % row_u4;                         \
+            uint32_t ref = (T) * kTileRefs + rl;                                  \
+            if (ref > n) ref = n;                                                 \
+            pre[i] = ref_rows[static_cast<size_t>(ref) * row_u4 + part];         \
+        }                                                                         \
+    }
+#define SCRUST_STORE_TILE(B)                                                      \
+    _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
+        if (FIXED || i < loads) {                                                 \
+            const int u = tid + 256 * i;                                          \
+            const int rl = u / row_u4, part = u % row_u4;                         \
+            sm.tile[B][rl * stride + part] = pre[i];                              \
+        }                                                                         \
+    }
     auto mfma_block = [&](f32x4 (&acc)[4], int cur, int cb) __attribute__((always_inline)) {
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++) acc[rb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const uint4* brow = &tile[cur][(cb * 16 + lc) * STRIDE];
+        for (int qb = 0; qb < 4; qb++) acc[qb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const u32x4* brow = &sm.tile[cur][(cb * 16 + lc) * stride];
         if constexpr (OP == kOperandBf16) {
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                const bf16x8 b = __builtin_bit_cast(bf16x8, brow[g * NCH + c]);
+                if (FIXED || c < nch) {
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, brow[g * nch + c]);
 #pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-                    acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[rb][c], acc[rb], 0, 0, 0);
+                    for (int qb = 0; qb < 4; qb++)
+                        acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[qb][c], acc[qb], 0, 0, 0);
+                }
             }
         } else {
-            const float4* bq = reinterpret_cast<const float4*>(brow) + g * (Op::kGroup / 4);
+            const float4* bq = reinterpret_cast<const float4*>(brow) + g * (group / 4);
 #pragma unroll
-            for (int c4 = 0; c4 < Op::kGroup / 4; c4++) {
-                const float4 b4 = bq[c4];
-                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+            for (int c4 = 0; c4 < Op::kRowU4Max / 4; c4++) {
+                if (FIXED || c4 * 4 < nch) {
+                    const float4 b4 = bq[c4];
+                    const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
-                for (int cc = 0; cc < 4; cc++) {
-                    if (c4 * 4 + cc < NCH) {
+                    for (int cc = 0; cc < 4; cc++) {
+                        const int c = c4 * 4 + cc;
+                        if (c < NCH && (FIXED || c < nch)) {
 #pragma unroll
-                        for (int rb = 0; rb < 4; rb++)
-                            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[cc], a[rb][c4 * 4 + cc], acc[rb], 0, 0, 0);
+                            for (int qb = 0; qb < 4; qb++)
+                                acc[qb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[cc], a[qb][c], acc[qb], 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -238,22 +267,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                 for (int r = 0; r < 4; r++) {
                     if (acc[qb][r] > phi[qb]) {
                         const uint32_t id = t * kTileRefs + cb * 16 + g * 4 + r;
-                        const uint32_t slot = atomicAdd(&cnt[ql], 1u);
-                        p.buf[static_cast<size_t>(qbase + ql) * kCap + slot] = make_key(-2.0f * acc[qb][r], id);
+                        const uint32_t slot = atomicAdd(&sm.cnt[ql], 1u);
+                        s.buf[static_cast<size_t>(qbase + ql) * cap + slot] = make_key(-2.0f * acc[qb][r], id);
                     }
                 }
             }
         }
     };
 
-    load_tile(0);
-    store_tile(0);
+    SCRUST_LOAD_TILE(0u)
+    SCRUST_STORE_TILE(0)
     __syncthreads();
     for (uint32_t t = 0; t < n_tiles; t++) {
         const int cur = t & 1;
         // Unconditional prefetch (the last tile reloads itself into the idle buffer): a
         // conditional load keeps `pre` out of registers and serialises the prefetch.
-        load_tile(t + 1 < n_tiles ? t + 1 : t);
+        SCRUST_LOAD_TILE(t + 1 < n_tiles ? t + 1 : t)
         f32x4 acc0[4], acc1[4];
         // Software pipeline: the MFMAs of block cb are in flight while block cb-1 is filtered.
         mfma_block(acc0, cur, 0);
@@ -264,90 +293,125 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
         mfma_block(acc1, cur, 3);
         filter(acc0, t, 2);
         filter(acc1, t, 3);
-        store_tile(cur ^ 1);
+        SCRUST_STORE_TILE(cur ^ 1)
         __syncthreads();
         // Compact every row of this wave whose buffer could overflow in the next tile.
-        const uint32_t mine = cnt[w * kWave + lane];
-        uint64_t full = __ballot(mine > static_cast<uint32_t>(kCap - kTileRefs));
+        const uint32_t mine = sm.cnt[w * kWave + lane];
+        uint64_t full = __ballot(mine > cap - kTileRefs);
         if (full) {
             while (full) {
                 const int q = __builtin_ctzll(full);
                 full &= full - 1;
                 const int ql = w * kWave + q;
-                uint64_t* rowbuf = p.buf + static_cast<size_t>(qbase + ql) * kCap;
-                const uint32_t count = cnt[ql];
-                const float theta = select_smallest<E>(rowbuf, count, KP, lane, rowbuf);
+                uint64_t* rowbuf = s.buf + static_cast<size_t>(qbase + ql) * cap;
+                const uint32_t count = sm.cnt[ql];
+                const float theta = select_by_cap(cap, rowbuf, count, kp, lane, rowbuf);
                 if (lane == 0) {
-                    cnt[ql] = KP;
-                    phl[ql] = -0.5f * theta;
+                    sm.cnt[ql] = kp;
+                    sm.phl[ql] = -0.5f * theta;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 #pragma unroll
-            for (int qb = 0; qb < 4; qb++) phi[qb] = phl[w * kWave + qb * 16 + lc];
+            for (int qb = 0; qb < 4; qb++) phi[qb] = sm.phl[w * kWave + qb * 16 + lc];
         }
     }
+#undef SCRUST_LOAD_TILE
+#undef SCRUST_STORE_TILE
     // Final selection: the KP smallest keys of each row (unordered) and the KP-th score.
     for (int q = 0; q < kWave; q++) {
         const int ql = w * kWave + q;
         const uint32_t row = qbase + ql;
         if (row >= n) break;
-        const uint64_t* rowbuf = p.buf + static_cast<size_t>(row) * kCap;
-        uint64_t* out = p.cand + static_cast<size_t>(row) * KP;
-        const uint32_t count = cnt[ql];
-        if (count >= static_cast<uint32_t>(KP)) {
-            const float theta = select_smallest<E>(rowbuf, count, KP, lane, out);
-            if (lane == 0) p.theta[row] = theta;
+        const uint64_t* rowbuf = s.buf + static_cast<size_t>(row) * cap;
+        uint64_t* out = s.cand + static_cast<size_t>(row) * kp;
+        const uint32_t count = sm.cnt[ql];
+        if (count >= static_cast<uint32_t>(kp)) {
+            const float theta = select_by_cap(cap, rowbuf, count, kp, lane, out);
+            if (lane == 0) s.theta[row] = theta;
         } else {
-            if (lane < KP) out[lane] = static_cast<uint32_t>(lane) < count ? rowbuf[lane] : kNoKey;
-            if (lane == 0) p.theta[row] = kInf;
+            if (lane < kp) out[lane] = static_cast<uint32_t>(lane) < count ? rowbuf[lane] : kNoKey;
+            if (lane == 0) s.theta[row] = kInf;
         }
     }
 }
 
-// ---------------------------------------------------------------------------
-// Dispatch over the compiled (operand, NCH, KP) variants.
-// ---------------------------------------------------------------------------
-
-template <int OP, int NCH, int KP>
-static void run_pass(const PassArgs& a, hipStream_t stream) {
-    const unsigned blocks = static_cast<unsigned>((a.n + kRowsPerBlock - 1) / kRowsPerBlock);
-    hipLaunchKernelGGL((pass_kernel<OP, NCH, KP>), dim3(blocks), dim3(256), 0, stream, a);
-}
-
+// At most two waves per SIMD: the allocator then keeps the tile prefetch in registers
+// instead of spilling it to reach a higher occupancy the LDS budget would allow.
 template <int OP, int NCH>
-static void run_kp(const PassArgs& a, hipStream_t stream) {
-    if (a.kp == 32) run_pass<OP, NCH, 32>(a, stream);
-    else run_pass<OP, NCH, 64>(a, stream);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void pass_kernel(const DatasetDesc* __restrict__ set) {
+    __shared__ PassSmem<OP, NCH> sm;
+    pass_block<OP, NCH, true>(*set, blockIdx.x, sm);
 }
+
+// Persistent grid over a queue of (dataset, query block) items, largest first. Every
+// workgroup leaves once the queue is drained.
+template <int OP, int NCH_MAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void pass_batch_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __restrict__ items,
+                       uint32_t n_items, uint32_t* __restrict__ counter) {
+    __shared__ PassSmem<OP, NCH_MAX> sm;
+    __shared__ uint32_t next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t item = next;
+        __syncthreads();
+        if (item >= n_items) break;
+        const PassItem it = items[item];
+        pass_block<OP, NCH_MAX, false>(sets[it.set], it.block, sm);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch over the compiled variants.
+// ---------------------------------------------------------------------------
 
 static const int kBf16Chunks[] = {1, 2, 3, 4, 5, 6, 8, 10, 13};
 static const int kF32Chunks[] = {1, 2, 3, 4, 6, 8, 10, 13, 16, 20, 26, 33};
 
 int bf16_chunks(int d) {
     const int need = (3 * d + 3 + 31) / 32;
-    for (int s : kBf16Chunks)
-        if (s >= need) return s;
+    for (int c : kBf16Chunks)
+        if (c >= need) return c;
     return 0;
 }
 
 int fp32_chunks(int d) {
     const int need = (d + 1 + 3) / 4;
-    for (int s : kF32Chunks)
-        if (s >= need) return s;
+    for (int c : kF32Chunks)
+        if (c >= need) return c;
     return 0;
 }
 
 int fp32_group(int nch) { return (nch + 3) / 4 * 4; }
 
-#define SCRUST_PASS_CASE(OP, N) \
-    case N:                     \
-        run_kp<OP, N>(a, stream); \
+int batch_chunks(int operand, int nch) {
+    static const int bf16_max[] = {5, 10, 13};
+    static const int f32_max[] = {13, 26, 33};
+    const int* sizes = operand == kOperandBf16 ? bf16_max : f32_max;
+    for (int i = 0; i < 3; i++)
+        if (sizes[i] >= nch) return sizes[i];
+    return 0;
+}
+
+template <int OP, int NCH>
+static void run_pass(const DatasetDesc* set, uint32_t n, hipStream_t stream) {
+    const unsigned blocks = (n + kPassRows - 1) / kPassRows;
+    hipLaunchKernelGGL((pass_kernel<OP, NCH>), dim3(blocks), dim3(256), 0, stream, set);
+}
+
+#define SCRUST_PASS_CASE(OP, N)               \
+    case N:                                   \
+        run_pass<OP, N>(set, host.n, stream); \
         return true;
 
-bool launch_pass(const PassArgs& a, hipStream_t stream) {
-    if (a.operand == kOperandBf16) {
-        switch (a.nch) {
+bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,
+                 hipStream_t stream) {
+    if (operand == kOperandBf16) {
+        switch (host.nch) {
             SCRUST_PASS_CASE(kOperandBf16, 1) SCRUST_PASS_CASE(kOperandBf16, 2)
             SCRUST_PASS_CASE(kOperandBf16, 3) SCRUST_PASS_CASE(kOperandBf16, 4)
             SCRUST_PASS_CASE(kOperandBf16, 5) SCRUST_PASS_CASE(kOperandBf16, 6)
@@ -356,13 +420,39 @@ bool launch_pass(const PassArgs& a, hipStream_t stream) {
             default: return false;
         }
     }
-    switch (a.nch) {
+    switch (host.nch) {
         SCRUST_PASS_CASE(kOperandF32, 1) SCRUST_PASS_CASE(kOperandF32, 2)
         SCRUST_PASS_CASE(kOperandF32, 3) SCRUST_PASS_CASE(kOperandF32, 4)
         SCRUST_PASS_CASE(kOperandF32, 6) SCRUST_PASS_CASE(kOperandF32, 8)
         SCRUST_PASS_CASE(kOperandF32, 10) SCRUST_PASS_CASE(kOperandF32, 13)
         SCRUST_PASS_CASE(kOperandF32, 16) SCRUST_PASS_CASE(kOperandF32, 20)
         SCRUST_PASS_CASE(kOperandF32, 26) SCRUST_PASS_CASE(kOperandF32, 33)
+        default: return false;
+    }
+}
+
+template <int OP, int NCH_MAX>
+static void run_batch(const DatasetDesc* sets, const PassItem* items, uint32_t n_items,
+                      uint32_t* counter, unsigned blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((pass_batch_kernel<OP, NCH_MAX>), dim3(blocks), dim3(256), 0, stream, sets,
+                       items, n_items, counter);
+}
+
+bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t n_items,
+                       uint32_t* item_counter, int operand, int nch_max, unsigned blocks,
+                       hipStream_t stream) {
+    if (operand == kOperandBf16) {
+        switch (nch_max) {
+            case 5: run_batch<kOperandBf16, 5>(sets, items, n_items, item_counter, blocks, stream); return true;
+            case 10: run_batch<kOperandBf16, 10>(sets, items, n_items, item_counter, blocks, stream); return true;
+            case 13: run_batch<kOperandBf16, 13>(sets, items, n_items, item_counter, blocks, stream); return true;
+            default: return false;
+        }
+    }
+    switch (nch_max) {
+        case 13: run_batch<kOperandF32, 13>(sets, items, n_items, item_counter, blocks, stream); return true;
+        case 26: run_batch<kOperandF32, 26>(sets, items, n_items, item_counter, blocks, stream); return true;
+        case 33: run_batch<kOperandF32, 33>(sets, items, n_items, item_counter, blocks, stream); return true;
         default: return false;
     }
 }

@@ -1,0 +1,245 @@
+// Device placement and launch order of the kNN pipeline for a set of embeddings.
+//
+// A plan lays every dataset's workspace out in one arena (coordinates, norms, column
+// sums, packed MFMA operands, candidate buffers, kept candidates, thresholds) and the
+// launch metadata in a second block (descriptors, persistent-pass work items, the list of
+// rows for the exact kernel, scalars). Enqueueing issues the same launches for one
+// embedding (with the pass kernel specialised at compile time) or for a batch (one
+// persistent pass launch over every dataset's work items).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "knn_launch.hpp"
+#include "scrust_hip_internal.hpp"
+
+namespace scrust_hip {
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+constexpr int kColsumRowsHost = 2048;  // rows per colsum block (knn_kernels.hip)
+
+}  // namespace
+
+scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
+                              const scrust_hip_knn_options* options, Plan& plan) {
+    plan = Plan{};
+    const int metric = options ? static_cast<int>(options->metric) : kMetricEuclidean;
+    const int precision = options ? static_cast<int>(options->precision) : SCRUST_HIP_BF16;
+    uint32_t cap = options && options->tile ? options->tile : 256u;
+    if (cap != 128 && cap != 256 && cap != 512)
+        return invalid("tile", "0, 128, 256 or 512", std::to_string(cap));
+    const bool single = specs.size() == 1;
+    plan.precision = precision;
+    plan.operand = precision == SCRUST_HIP_BF16 ? kOperandBf16
+                   : precision == SCRUST_HIP_FP32 ? kOperandF32
+                                                  : kOperandNone;
+    plan.shape.count = static_cast<uint32_t>(specs.size());
+    plan.shape.metric = metric;
+    plan.shape.operand = plan.operand;
+
+    size_t arena = 0;
+    uint32_t row = 0, prep = 0;
+    for (const SetSpec& sp : specs) {
+        DatasetDesc s{};
+        s.x = sp.x;
+        s.out_indices = sp.out_indices;
+        s.out_distances = sp.out_distances;
+        s.n = static_cast<uint32_t>(sp.n);
+        s.d = static_cast<uint32_t>(sp.d);
+        s.k = static_cast<uint32_t>(sp.k);
+        s.row_begin = row;
+        s.prep_begin = prep;
+        s.cap = cap;
+        const bool pass = plan.operand != kOperandNone && sp.k <= static_cast<size_t>(kMaxPassK) &&
+                          sp.d <= static_cast<size_t>(kMaxPassD);
+        s.on_pass = pass ? 1u : 0u;
+        if (pass) {
+            s.kp = sp.k + 17 <= 32 ? 32u : 64u;
+            if (plan.operand == kOperandBf16) {
+                // Single calls use the compiled chunk counts; the batched kernel takes any.
+                s.nch = single ? bf16_chunks(static_cast<int>(sp.d))
+                               : static_cast<uint32_t>((3 * sp.d + 3 + 31) / 32);
+                s.group = 0;
+                // c = h + l + r with |r| <= 2^-16 |c|: dropping l.l and the residuals costs
+                // <= 3.02 * 2^-16 |a||b|; |b|^2 = nh + nl + nl2 to 2^-24; products of bf16 are
+                // exact in f32, the accumulation is bounded as 2 * (terms + 1) * 2^-24.
+                s.split_rel = 3.02 * 0x1p-16;
+                s.acc_rel = 2.0 * (32.0 * s.nch + 1.0) * 0x1p-24 * 1.05;
+                s.norm_rel = 0x1p-24 * 1.01;
+            } else {
+                s.nch = single ? fp32_chunks(static_cast<int>(sp.d))
+                               : static_cast<uint32_t>((sp.d + 1 + 3) / 4);
+                s.group = fp32_group(static_cast<int>(s.nch));
+                s.split_rel = 0.0;
+                s.acc_rel = 2.0 * (4.0 * s.nch + 1.0) * 0x1p-24 * 1.05;
+                s.norm_rel = 0.0;
+            }
+            if (s.nch == 0) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no pass variant for d = " + std::to_string(sp.d));
+            plan.any_pass = true;
+            plan.kp32 |= s.kp == 32;
+            plan.kp64 |= s.kp == 64;
+            plan.nch_max = std::max(plan.nch_max, static_cast<int>(s.nch));
+            plan.pass_flops += 2ull * sp.n * sp.n * sp.d;
+        } else {
+            for (uint32_t r = 0; r < s.n; r++) plan.exact_rows.push_back(row + r);
+        }
+        // Arena offsets are stored in the pointer fields until plan_place.
+        auto take = [&](size_t bytes) {
+            const size_t off = arena;
+            arena += align_up(bytes);
+            return reinterpret_cast<void*>(off);
+        };
+        s.coords = static_cast<float*>(take(sp.n * sp.d * sizeof(float)));
+        s.norms = static_cast<float*>(take(sp.n * sizeof(float)));
+        s.mean = static_cast<double*>(take(sp.d * sizeof(double)));
+        s.acc = static_cast<unsigned long long*>(take(sp.d * kAccWordsHost * sizeof(unsigned long long)));
+        s.max_norm_bits = static_cast<unsigned*>(take(sizeof(unsigned)));
+        if (pass) {
+            const size_t npad = (sp.n + kPassRows - 1) / kPassRows * kPassRows;
+            if (plan.operand == kOperandBf16) {
+                const size_t rowb = 64ull * s.nch;
+                s.pack_ref = static_cast<uint16_t*>(take((sp.n + 1) * rowb));
+                s.pack_query = static_cast<uint16_t*>(take(sp.n * rowb));
+            } else {
+                s.pack_f32 = static_cast<float*>(take((sp.n + 1) * 16ull * s.group));
+            }
+            s.buf = static_cast<uint64_t*>(take(npad * cap * sizeof(uint64_t)));
+            s.cand = static_cast<uint64_t*>(take(sp.n * s.kp * sizeof(uint64_t)));
+            s.theta = static_cast<float*>(take(sp.n * sizeof(float)));
+            for (uint32_t b = 0; b < npad / kPassRows; b++) plan.items.push_back(PassItem{static_cast<uint32_t>(plan.sets.size()), b});
+        }
+        plan.sets.push_back(s);
+        plan.shape.max_row_chunks = std::max<uint32_t>(plan.shape.max_row_chunks, static_cast<uint32_t>((sp.n + kColsumRowsHost - 1) / kColsumRowsHost));
+        plan.shape.max_col_groups = std::max<uint32_t>(plan.shape.max_col_groups, static_cast<uint32_t>((sp.d + 63) / 64));
+        plan.shape.max_d = std::max<uint32_t>(plan.shape.max_d, static_cast<uint32_t>(sp.d));
+        row += s.n;
+        prep += s.n + 1;
+    }
+    plan.shape.total_rows = row;
+    plan.shape.total_prep = prep;
+    if (plan.any_pass && !single) {
+        plan.nch_max = batch_chunks(plan.operand, plan.nch_max);
+        if (plan.nch_max == 0) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no batched pass variant");
+        // Largest work first: an item costs about n (the references it scans).
+        std::stable_sort(plan.items.begin(), plan.items.end(), [&](const PassItem& a, const PassItem& b) {
+            return plan.sets[a.set].n > plan.sets[b.set].n;
+        });
+    }
+    if (!plan.any_pass) plan.operand = kOperandNone;
+    plan.shape.operand = plan.operand;
+    plan.arena_bytes = arena;
+    plan.meta_bytes = align_up(plan.sets.size() * sizeof(DatasetDesc)) +
+                      align_up(plan.items.size() * sizeof(PassItem) + 1) +
+                      align_up(static_cast<size_t>(row) * sizeof(uint32_t) + 1) +
+                      align_up(kScalarCount * sizeof(uint32_t));
+    return SCRUST_HIP_OK;
+}
+
+scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t stream,
+                             void* pinned) {
+    char* base = static_cast<char*>(arena);
+    // Every field passed here holds an arena offset (possibly 0) from plan_layout.
+    auto fix = [&](auto*& p) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        p = reinterpret_cast<T*>(base + reinterpret_cast<size_t>(p));
+    };
+    for (DatasetDesc& s : plan.sets) {
+        fix(s.coords);
+        fix(s.norms);
+        fix(s.mean);
+        fix(s.acc);
+        fix(s.max_norm_bits);
+        if (s.on_pass) {
+            if (plan.operand == kOperandBf16) {
+                fix(s.pack_ref);
+                fix(s.pack_query);
+            } else {
+                fix(s.pack_f32);
+            }
+            fix(s.buf);
+            fix(s.cand);
+            fix(s.theta);
+        }
+    }
+    char* m = static_cast<char*>(meta);
+    plan.d_sets = reinterpret_cast<DatasetDesc*>(m);
+    m += align_up(plan.sets.size() * sizeof(DatasetDesc));
+    plan.d_items = reinterpret_cast<PassItem*>(m);
+    m += align_up(plan.items.size() * sizeof(PassItem) + 1);
+    plan.d_list = reinterpret_cast<uint32_t*>(m);
+    m += align_up(static_cast<size_t>(plan.shape.total_rows) * sizeof(uint32_t) + 1);
+    plan.d_scalars = reinterpret_cast<uint32_t*>(m);
+    const size_t sets_bytes = plan.sets.size() * sizeof(DatasetDesc);
+    const size_t items_bytes = plan.items.size() * sizeof(PassItem);
+    const size_t list_bytes = plan.exact_rows.size() * sizeof(uint32_t);
+    hipError_t err = hipSuccess;
+    if (pinned) {
+        // Stage the metadata in pinned memory with the device layout: one async copy.
+        char* h = static_cast<char*>(pinned);
+        const size_t items_at = reinterpret_cast<char*>(plan.d_items) - static_cast<char*>(meta);
+        const size_t list_at = reinterpret_cast<char*>(plan.d_list) - static_cast<char*>(meta);
+        std::memcpy(h, plan.sets.data(), sets_bytes);
+        if (items_bytes) std::memcpy(h + items_at, plan.items.data(), items_bytes);
+        if (list_bytes) std::memcpy(h + list_at, plan.exact_rows.data(), list_bytes);
+        err = hipMemcpyAsync(meta, h, list_at + list_bytes, hipMemcpyHostToDevice, stream);
+    } else {
+        err = hipMemcpyAsync(plan.d_sets, plan.sets.data(), sets_bytes, hipMemcpyHostToDevice, stream);
+        if (err == hipSuccess && items_bytes)
+            err = hipMemcpyAsync(plan.d_items, plan.items.data(), items_bytes, hipMemcpyHostToDevice, stream);
+        if (err == hipSuccess && list_bytes)
+            err = hipMemcpyAsync(plan.d_list, plan.exact_rows.data(), list_bytes, hipMemcpyHostToDevice, stream);
+    }
+    if (err != hipSuccess) return hip_failure(err, "upload launch metadata");
+    return SCRUST_HIP_OK;
+}
+
+scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
+                               hipStream_t stream, hipEvent_t* events) {
+    auto mark = [&](int i) -> hipError_t { return events ? hipEventRecord(events[i], stream) : hipSuccess; };
+    hipError_t err = mark(0);
+    if (err != hipSuccess) return hip_failure(err, "record event");
+    // Scalars: flags = 0, exact-list count = the exact-path rows already listed, item
+    // counter = 0. Cleared with a kernel (graph-safe).
+    launch_fill_u32(plan.d_scalars, kScalarCount, 0u, stream);
+    launch_fill_u32(plan.d_scalars + kScalarListCount, 1, static_cast<uint32_t>(plan.exact_rows.size()), stream);
+    unsigned* flags = plan.d_scalars + kScalarFlags;
+    launch_prepare_all(plan.d_sets, plan.shape, flags, stream);
+    if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch prepare");
+    if ((err = mark(1)) != hipSuccess) return hip_failure(err, "record event");
+    if (plan.any_pass) {
+        bool ok;
+        if (single) {
+            ok = launch_pass(plan.d_sets, plan.sets[0], plan.operand, stream);
+        } else {
+            const unsigned blocks = static_cast<unsigned>(std::min<size_t>(plan.items.size(), 2ull * compute_units));
+            ok = launch_pass_batch(plan.d_sets, plan.d_items, static_cast<uint32_t>(plan.items.size()),
+                                   plan.d_scalars + kScalarItemCounter, plan.operand, plan.nch_max,
+                                   blocks, stream);
+        }
+        if (!ok) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no compiled pass variant for this shape");
+        if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch candidate pass");
+    }
+    if ((err = mark(2)) != hipSuccess) return hip_failure(err, "record event");
+    if (plan.any_pass) {
+        launch_rerank(plan.d_sets, plan.shape, plan.kp32, plan.kp64, plan.d_list,
+                      plan.d_scalars + kScalarListCount, stream);
+        if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch rerank");
+    }
+    if ((err = mark(3)) != hipSuccess) return hip_failure(err, "record event");
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>(plan.shape.total_rows, 4096));
+    launch_exact(plan.d_sets, plan.shape, plan.d_list, plan.d_scalars + kScalarListCount, blocks, flags,
+                 stream);
+    if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch exact rows");
+    if ((err = mark(4)) != hipSuccess) return hip_failure(err, "record event");
+    return SCRUST_HIP_OK;
+}
+
+}  // namespace scrust_hip

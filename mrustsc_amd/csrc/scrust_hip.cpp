@@ -1,10 +1,10 @@
-// Host side of scrust_hip: context, argument validation, the per-call kernel pipeline
-// and the extern "C" boundary declared in include/scrust_hip.h.
+// Host side of scrust_hip: context, argument validation, and the extern "C" boundary
+// declared in include/scrust_hip.h.
 //
 // Mirrors the reference's Metal host code (crates/scrust-gpu/src/kernels/knn.rs:36-114
 // and crates/scrust-gpu/src/context.rs:12-90): validate, prepare coordinates, dispatch,
-// wait, check the command status, read back. Centring and packing happen on the device
-// here; the reference does them on the host (knn.rs:72, :127-158).
+// wait, check the status, read back. Centring and packing happen on the device here; the
+// reference does them on the host (knn.rs:72, :127-158).
 #include "../../include/scrust_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "knn_launch.hpp"
 #include "scrust_hip_internal.hpp"
@@ -65,13 +66,11 @@ scrust_hip_status validate(const void* embedding, size_t n, size_t d, size_t k,
             return invalid("metric", "euclidean or cosine", std::to_string(options->metric));
         if (options->precision > SCRUST_HIP_EXACT)
             return invalid("precision", "bf16, fp32 or exact", std::to_string(options->precision));
+        if (options->tile != 0 && options->tile != 128 && options->tile != 256 && options->tile != 512)
+            return invalid("tile", "0, 128, 256 or 512", std::to_string(options->tile));
     }
     return SCRUST_HIP_OK;
 }
-
-// ---------------------------------------------------------------------------
-// Context.
-// ---------------------------------------------------------------------------
 
 Buffer::~Buffer() {
     if (ptr) (void)hipFree(ptr);
@@ -102,6 +101,64 @@ bool device_is_gfx950(int device, hipDeviceProp_t* props_out) {
     return std::strncmp(props.gcnArchName, "gfx950", 6) == 0;
 }
 
+static double event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return static_cast<double>(ms);
+}
+
+#define SCRUST_CHECK(expr, step)                                 \
+    do {                                                         \
+        hipError_t err_ = (expr);                                \
+        if (err_ != hipSuccess) return hip_failure(err_, step);  \
+    } while (0)
+
+// Reads the plan's flags and exact-row count after completion; maps flags to errors.
+static scrust_hip_status read_status(const Plan& plan, hipStream_t stream, uint32_t* list_count) {
+    uint32_t host[kScalarCount] = {};
+    SCRUST_CHECK(hipMemcpyAsync(host, plan.d_scalars, sizeof host, hipMemcpyDeviceToHost, stream),
+                 "read status");
+    SCRUST_CHECK(hipStreamSynchronize(stream), "wait for completion");
+    if (host[kScalarFlags] & kFlagNonFinite) return invalid("embedding", "finite values", "NaN or infinity");
+    if (host[kScalarFlags] & kFlagBandOverflow)
+        return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: a boundary tie band ran past the kept keys");
+    if (list_count) *list_count = host[kScalarListCount];
+    return SCRUST_HIP_OK;
+}
+
+// One embedding already on the device.
+scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
+                             const scrust_hip_knn_options* options, uint32_t* out_indices,
+                             float* out_distances, scrust_hip_knn_stats* stats) {
+    std::vector<SetSpec> specs{SetSpec{x, n, d, k, out_indices, out_distances}};
+    Plan plan;
+    scrust_hip_status st = plan_layout(specs, options, plan);
+    if (st != SCRUST_HIP_OK) return st;
+    SCRUST_CHECK(ctx->arena.reserve(plan.arena_bytes), "allocate workspace");
+    SCRUST_CHECK(ctx->meta.reserve(plan.meta_bytes), "allocate metadata");
+    st = plan_place(plan, ctx->arena.ptr, ctx->meta.ptr, ctx->stream);
+    if (st != SCRUST_HIP_OK) return st;
+    st = plan_enqueue(plan, true, ctx->compute_units, ctx->stream, ctx->events);
+    if (st != SCRUST_HIP_OK) return st;
+    uint32_t listed = 0;
+    st = read_status(plan, ctx->stream, &listed);
+    if (st != SCRUST_HIP_OK) return st;
+    if (stats) {
+        hipEvent_t* ev = ctx->events;
+        stats->prepare_ms = event_ms(ev[0], ev[1]);
+        stats->pass_ms = event_ms(ev[1], ev[2]);
+        stats->rerank_ms = event_ms(ev[2], ev[3]);
+        stats->fallback_ms = event_ms(ev[3], ev[4]);
+        stats->device_ms = event_ms(ev[0], ev[4]);
+        stats->fallback_rows = listed;
+        stats->candidates = plan.any_pass ? plan.sets[0].kp : 0;
+        stats->pass_flops = plan.pass_flops;
+        stats->path = plan.any_pass ? static_cast<uint32_t>(plan.precision) : SCRUST_HIP_EXACT;
+        stats->tile = plan.sets[0].cap;
+    }
+    return SCRUST_HIP_OK;
+}
+
 }  // namespace scrust_hip
 
 using namespace scrust_hip;
@@ -116,196 +173,11 @@ scrust_hip_context::~scrust_hip_context() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-// ---------------------------------------------------------------------------
-// The pipeline on device-resident data.
-// ---------------------------------------------------------------------------
-
-namespace scrust_hip {
-
-static double event_ms(hipEvent_t a, hipEvent_t b) {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
-    return static_cast<double>(ms);
+scrust_hip_batch::~scrust_hip_batch() {
+    if (context) (void)hipSetDevice(context->device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
 }
-
-#define SCRUST_CHECK(expr, step)                       \
-    do {                                               \
-        hipError_t err_ = (expr);                      \
-        if (err_ != hipSuccess) return hip_failure(err_, step); \
-    } while (0)
-
-scrust_hip_status run_knn(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
-                          const scrust_hip_knn_options* options, uint32_t* out_indices,
-                          float* out_distances, scrust_hip_knn_stats* stats) {
-    const int metric = options ? static_cast<int>(options->metric) : kMetricEuclidean;
-    int precision = options ? static_cast<int>(options->precision) : SCRUST_HIP_BF16;
-    if (k > static_cast<size_t>(kMaxPassK) || d > static_cast<size_t>(kMaxPassD)) precision = SCRUST_HIP_EXACT;
-    const int di = static_cast<int>(d), ki = static_cast<int>(k);
-    const uint32_t nu = static_cast<uint32_t>(n);
-
-    int operand = kOperandNone, nch = 0, group = 0, kp = 0;
-    double split_rel = 0.0, acc_rel = 0.0, norm_rel = 0.0;
-    if (precision == SCRUST_HIP_BF16) {
-        // c = h + l + r with |r| <= 2^-16 |c|: dropping l.l and the residuals costs
-        // <= 3.02 * 2^-16 |a||b|; |b|^2 = nh + nl + nl2 to 2^-24; products of bf16 are
-        // exact in f32, the accumulation is bounded as 2 * (terms + 1) * 2^-24.
-        operand = kOperandBf16;
-        nch = bf16_chunks(di);
-        split_rel = 3.02 * 0x1p-16;
-        acc_rel = 2.0 * (32.0 * nch + 1.0) * 0x1p-24 * 1.05;
-        norm_rel = 0x1p-24 * 1.01;
-    } else if (precision == SCRUST_HIP_FP32) {
-        operand = kOperandF32;
-        nch = fp32_chunks(di);
-        group = fp32_group(nch);
-        acc_rel = 2.0 * (4.0 * nch + 1.0) * 0x1p-24 * 1.05;
-    }
-    if (operand != kOperandNone) kp = (ki + 17 <= 32) ? 32 : 64;
-    const size_t npad = (n + 255) / 256 * 256;
-
-    Workspace& ws = ctx->ws;
-    SCRUST_CHECK(ws.coords.reserve(n * d * sizeof(float)), "allocate coordinates");
-    SCRUST_CHECK(ws.norms.reserve(n * sizeof(float)), "allocate norms");
-    SCRUST_CHECK(ws.mean.reserve(d * sizeof(double)), "allocate means");
-    SCRUST_CHECK(ws.acc.reserve(d * kAccWordsHost * sizeof(unsigned long long)), "allocate sums");
-    SCRUST_CHECK(ws.scalars.reserve(16 * sizeof(uint32_t)), "allocate scalars");
-    SCRUST_CHECK(ws.fail_list.reserve(n * sizeof(uint32_t)), "allocate fallback list");
-    if (operand == kOperandBf16) {
-        const size_t row = 32ull * nch * sizeof(uint16_t);
-        SCRUST_CHECK(ws.pack_ref.reserve((n + 1) * row), "allocate operands");
-        SCRUST_CHECK(ws.pack_query.reserve((n + 1) * row), "allocate operands");
-    } else if (operand == kOperandF32) {
-        SCRUST_CHECK(ws.pack_ref.reserve((n + 1) * 4ull * group * sizeof(float)), "allocate operands");
-    }
-    if (operand != kOperandNone) {
-        SCRUST_CHECK(ws.buf.reserve(npad * kCap * sizeof(uint64_t)), "allocate candidate buffers");
-        SCRUST_CHECK(ws.cand.reserve(n * kp * sizeof(uint64_t)), "allocate candidates");
-        SCRUST_CHECK(ws.theta.reserve(n * sizeof(float)), "allocate thresholds");
-    }
-    uint32_t* scalars = static_cast<uint32_t*>(ws.scalars.ptr);
-    unsigned* max_norm_bits = scalars + 0;
-    unsigned* flags = scalars + 1;
-    uint32_t* fail_count = scalars + 2;
-    hipStream_t s = ctx->stream;
-    hipEvent_t* ev = ctx->events;
-
-    SCRUST_CHECK(hipEventRecord(ev[0], s), "record event");
-    launch_fill_u32(scalars, 16, 0u, s);
-    if (metric == kMetricEuclidean)
-        launch_column_means(x, n, di, static_cast<unsigned long long*>(ws.acc.ptr),
-                            static_cast<double*>(ws.mean.ptr), s);
-    PrepareArgs pa{};
-    pa.x = x;
-    pa.n = n;
-    pa.d = di;
-    pa.metric = metric;
-    pa.mean = static_cast<const double*>(ws.mean.ptr);
-    pa.coords = static_cast<float*>(ws.coords.ptr);
-    pa.norms = static_cast<float*>(ws.norms.ptr);
-    pa.operand = operand;
-    pa.pack_ref = static_cast<uint16_t*>(ws.pack_ref.ptr);
-    pa.pack_query = static_cast<uint16_t*>(ws.pack_query.ptr);
-    pa.pack_f32 = static_cast<float*>(ws.pack_ref.ptr);
-    pa.nch = nch;
-    pa.group = group;
-    pa.max_norm_bits = max_norm_bits;
-    pa.flags = flags;
-    launch_prepare(pa, s);
-    SCRUST_CHECK(hipGetLastError(), "launch prepare");
-    SCRUST_CHECK(hipEventRecord(ev[1], s), "record event");
-
-    ExactArgs ea{};
-    ea.coords = pa.coords;
-    ea.norms = pa.norms;
-    ea.n = nu;
-    ea.d = di;
-    ea.k = ki;
-    ea.metric = metric;
-    ea.out_indices = out_indices;
-    ea.out_distances = out_distances;
-    ea.flags = flags;
-
-    if (operand != kOperandNone) {
-        PassArgs ps{};
-        ps.operand = operand;
-        ps.nch = nch;
-        ps.kp = kp;
-        ps.d = di;
-        ps.pack_query = pa.pack_query;
-        ps.pack_ref = pa.pack_ref;
-        ps.pack_f32 = pa.pack_f32;
-        ps.norms = pa.norms;
-        ps.n = nu;
-        ps.buf = static_cast<uint64_t*>(ws.buf.ptr);
-        ps.cand = static_cast<uint64_t*>(ws.cand.ptr);
-        ps.theta = static_cast<float*>(ws.theta.ptr);
-        if (!launch_pass(ps, s))
-            return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no compiled pass for d = " + std::to_string(d));
-        SCRUST_CHECK(hipGetLastError(), "launch candidate pass");
-        SCRUST_CHECK(hipEventRecord(ev[2], s), "record event");
-
-        RerankArgs ra{};
-        ra.coords = pa.coords;
-        ra.norms = pa.norms;
-        ra.n = nu;
-        ra.d = di;
-        ra.k = ki;
-        ra.kp = kp;
-        ra.metric = metric;
-        ra.cand = ps.cand;
-        ra.theta = ps.theta;
-        ra.max_norm_bits = max_norm_bits;
-        ra.split_rel = split_rel;
-        ra.acc_rel = acc_rel;
-        ra.norm_rel = norm_rel;
-        ra.out_indices = out_indices;
-        ra.out_distances = out_distances;
-        ra.fail_list = static_cast<uint32_t*>(ws.fail_list.ptr);
-        ra.fail_count = fail_count;
-        launch_rerank(ra, s);
-        SCRUST_CHECK(hipGetLastError(), "launch rerank");
-        SCRUST_CHECK(hipEventRecord(ev[3], s), "record event");
-
-        ea.list = ra.fail_list;
-        ea.list_count = fail_count;
-        const unsigned blocks = static_cast<unsigned>(std::min<size_t>(n, 2048));
-        launch_exact(ea, blocks, s);
-        SCRUST_CHECK(hipGetLastError(), "launch exact rows");
-        SCRUST_CHECK(hipEventRecord(ev[4], s), "record event");
-    } else {
-        SCRUST_CHECK(hipEventRecord(ev[2], s), "record event");
-        SCRUST_CHECK(hipEventRecord(ev[3], s), "record event");
-        ea.list = nullptr;
-        ea.list_count = nullptr;
-        const unsigned blocks = static_cast<unsigned>(std::min<size_t>(n, 8192));
-        launch_exact(ea, blocks, s);
-        SCRUST_CHECK(hipGetLastError(), "launch exact rows");
-        SCRUST_CHECK(hipEventRecord(ev[4], s), "record event");
-    }
-    uint32_t host_scalars[4] = {0, 0, 0, 0};
-    SCRUST_CHECK(hipMemcpyAsync(host_scalars, scalars, sizeof host_scalars, hipMemcpyDeviceToHost, s),
-                 "read status");
-    SCRUST_CHECK(hipStreamSynchronize(s), "wait for completion");
-    if (host_scalars[1] & kFlagNonFinite)
-        return invalid("embedding", "finite values", "NaN or infinity");
-    if (host_scalars[1] & kFlagBandOverflow)
-        return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: a boundary tie band exceeded 512 entries");
-    if (stats) {
-        stats->prepare_ms = event_ms(ev[0], ev[1]);
-        stats->pass_ms = operand != kOperandNone ? event_ms(ev[1], ev[2]) : 0.0;
-        stats->rerank_ms = operand != kOperandNone ? event_ms(ev[2], ev[3]) : 0.0;
-        stats->fallback_ms = event_ms(ev[3], ev[4]);
-        stats->device_ms = event_ms(ev[0], ev[4]);
-        stats->fallback_rows = operand != kOperandNone ? host_scalars[2] : n;
-        stats->candidates = static_cast<uint64_t>(kp);
-        stats->pass_flops = operand != kOperandNone ? 2ull * n * n * d : 0ull;
-        stats->path = static_cast<uint32_t>(precision);
-        stats->tile = 0;
-    }
-    return SCRUST_HIP_OK;
-}
-
-}  // namespace scrust_hip
 
 // ---------------------------------------------------------------------------
 // extern "C" boundary.
@@ -369,7 +241,7 @@ scrust_hip_status scrust_hip_knn_device(scrust_hip_context* context, const float
     if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
     if (hipSetDevice(context->device) != hipSuccess)
         return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
-    return run_knn(context, embedding, n, d, k, options, out_indices, out_distances, stats);
+    return run_single(context, embedding, n, d, k, options, out_indices, out_distances, stats);
 }
 
 scrust_hip_status scrust_hip_knn_ex(scrust_hip_context* context, const float* embedding,
@@ -383,20 +255,20 @@ scrust_hip_status scrust_hip_knn_ex(scrust_hip_context* context, const float* em
     if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
     if (hipSetDevice(context->device) != hipSuccess)
         return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
-    Workspace& ws = context->ws;
     hipStream_t s = context->stream;
-    SCRUST_CHECK(ws.input.reserve(n * d * sizeof(float)), "allocate input");
-    SCRUST_CHECK(ws.out_indices.reserve(n * k * sizeof(uint32_t)), "allocate output");
-    SCRUST_CHECK(ws.out_distances.reserve(n * k * sizeof(float)), "allocate output");
-    SCRUST_CHECK(hipMemcpyAsync(ws.input.ptr, embedding, n * d * sizeof(float), hipMemcpyHostToDevice, s),
+    SCRUST_CHECK(context->input.reserve(n * d * sizeof(float)), "allocate input");
+    SCRUST_CHECK(context->out_indices.reserve(n * k * sizeof(uint32_t)), "allocate output");
+    SCRUST_CHECK(context->out_distances.reserve(n * k * sizeof(float)), "allocate output");
+    SCRUST_CHECK(hipMemcpyAsync(context->input.ptr, embedding, n * d * sizeof(float), hipMemcpyHostToDevice, s),
                  "copy input");
-    st = run_knn(context, static_cast<const float*>(ws.input.ptr), n, d, k, options,
-                 static_cast<uint32_t*>(ws.out_indices.ptr), static_cast<float*>(ws.out_distances.ptr), stats);
+    st = run_single(context, static_cast<const float*>(context->input.ptr), n, d, k, options,
+                    static_cast<uint32_t*>(context->out_indices.ptr),
+                    static_cast<float*>(context->out_distances.ptr), stats);
     if (st != SCRUST_HIP_OK) return st;
-    SCRUST_CHECK(hipMemcpyAsync(out_indices, ws.out_indices.ptr, n * k * sizeof(uint32_t),
+    SCRUST_CHECK(hipMemcpyAsync(out_indices, context->out_indices.ptr, n * k * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s),
                  "copy indices");
-    SCRUST_CHECK(hipMemcpyAsync(out_distances, ws.out_distances.ptr, n * k * sizeof(float),
+    SCRUST_CHECK(hipMemcpyAsync(out_distances, context->out_distances.ptr, n * k * sizeof(float),
                                 hipMemcpyDeviceToHost, s),
                  "copy distances");
     SCRUST_CHECK(hipStreamSynchronize(s), "wait for completion");
@@ -407,5 +279,89 @@ scrust_hip_status scrust_hip_knn(scrust_hip_context* context, const float* embed
                                  size_t d, size_t k, uint32_t* out_indices, float* out_distances) {
     return scrust_hip_knn_ex(context, embedding, n, d, k, nullptr, out_indices, out_distances, nullptr);
 }
+
+scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
+                                          const scrust_hip_dataset* datasets, size_t count,
+                                          const scrust_hip_knn_options* options,
+                                          scrust_hip_batch** out) {
+    clear_error();
+    if (!out) return invalid("out", "a non-null pointer", "null");
+    *out = nullptr;
+    if (count == 0 || !datasets) return invalid("datasets", "at least one dataset", std::to_string(count));
+    std::vector<SetSpec> specs;
+    size_t total = 0;
+    for (size_t i = 0; i < count; i++) {
+        const scrust_hip_dataset& ds = datasets[i];
+        scrust_hip_status st = validate(ds.embedding, ds.n, ds.d, ds.k, ds.out_indices, ds.out_distances, options);
+        if (st != SCRUST_HIP_OK) return fail(st, "dataset " + std::to_string(i) + ": " + last_error());
+        total += ds.n;
+        specs.push_back(SetSpec{ds.embedding, ds.n, ds.d, ds.k, ds.out_indices, ds.out_distances});
+    }
+    if (total >= 0xFFFFFFFFull) return invalid("datasets", "fewer than 2^32 - 1 cells in total", std::to_string(total));
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    scrust_hip_batch* batch = new (std::nothrow) scrust_hip_batch();
+    if (!batch) return fail(SCRUST_HIP_OUT_OF_MEMORY, "host allocation failed");
+    batch->context = context;
+    auto bail = [&](scrust_hip_status st) {
+        delete batch;
+        return st;
+    };
+    scrust_hip_status st = plan_layout(specs, options, batch->plan);
+    if (st != SCRUST_HIP_OK) return bail(st);
+    Plan& plan = batch->plan;
+    hipError_t err = batch->arena.reserve(plan.arena_bytes);
+    if (err == hipSuccess) err = batch->meta.reserve(plan.meta_bytes);
+    if (err != hipSuccess) return bail(hip_failure(err, "allocate batch workspace"));
+    hipStream_t s = context->stream;
+    st = plan_place(plan, batch->arena.ptr, batch->meta.ptr, s);
+    if (st != SCRUST_HIP_OK) return bail(st);
+    err = hipStreamSynchronize(s);
+    if (err != hipSuccess) return bail(hip_failure(err, "upload batch metadata"));
+    // Capture the whole pipeline once; runs replay it.
+    err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) return bail(hip_failure(err, "begin graph capture"));
+    st = plan_enqueue(plan, false, context->compute_units, s, nullptr);
+    hipGraph_t graph = nullptr;
+    err = hipStreamEndCapture(s, &graph);
+    if (st != SCRUST_HIP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return bail(st);
+    }
+    if (err != hipSuccess) return bail(hip_failure(err, "end graph capture"));
+    batch->graph = graph;
+    err = hipGraphInstantiate(&batch->exec, graph, nullptr, nullptr, 0);
+    if (err != hipSuccess) return bail(hip_failure(err, "instantiate graph"));
+    *out = batch;
+    return SCRUST_HIP_OK;
+}
+
+scrust_hip_status scrust_hip_batch_run(scrust_hip_batch* batch, scrust_hip_batch_stats* stats) {
+    clear_error();
+    if (!batch) return invalid("batch", "a non-null pointer", "null");
+    scrust_hip_context* ctx = batch->context;
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(ctx->device));
+    hipStream_t s = ctx->stream;
+    SCRUST_CHECK(hipEventRecord(ctx->events[0], s), "record event");
+    SCRUST_CHECK(hipGraphLaunch(batch->exec, s), "launch graph");
+    SCRUST_CHECK(hipEventRecord(ctx->events[1], s), "record event");
+    uint32_t listed = 0;
+    scrust_hip_status st = read_status(batch->plan, s, &listed);
+    if (st != SCRUST_HIP_OK) return st;
+    if (stats) {
+        const Plan& plan = batch->plan;
+        stats->device_ms = event_ms(ctx->events[0], ctx->events[1]);
+        stats->datasets = plan.sets.size();
+        stats->pass_items = plan.items.size();
+        stats->exact_rows = listed;
+        stats->pass_flops = plan.pass_flops;
+        stats->cells = plan.shape.total_rows;
+    }
+    return SCRUST_HIP_OK;
+}
+
+void scrust_hip_batch_destroy(scrust_hip_batch* batch) { delete batch; }
 
 }  // extern "C"

@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/scrust_hip.h"
+#include "knn_launch.hpp"
 
 namespace scrust_hip {
 
@@ -21,12 +23,40 @@ struct Buffer {
     hipError_t reserve(size_t bytes);
 };
 
-struct Workspace {
-    Buffer input, out_indices, out_distances;  // host-input entry point only
-    Buffer coords, norms, mean, acc, scalars, fail_list;
-    Buffer pack_ref, pack_query;
-    Buffer buf, cand, theta;
+// One embedding of a call or a batch, before device placement.
+struct SetSpec {
+    const float* x;
+    size_t n, d, k;
+    uint32_t* out_indices;
+    float* out_distances;
 };
+
+// Device placement and launch parameters of a set of embeddings.
+struct Plan {
+    std::vector<DatasetDesc> sets;  // device pointers filled in
+    std::vector<PassItem> items;    // persistent-pass work items, largest first
+    std::vector<uint32_t> exact_rows;  // flat rows of datasets on the exact path
+    BatchShape shape{};
+    int operand = kOperandNone;
+    int precision = SCRUST_HIP_BF16;
+    int nch_max = 0;
+    bool kp32 = false, kp64 = false;
+    bool any_pass = false;
+    uint64_t pass_flops = 0;
+    size_t arena_bytes = 0;
+    size_t meta_bytes = 0;
+    // Device views (valid after place()).
+    DatasetDesc* d_sets = nullptr;
+    PassItem* d_items = nullptr;
+    uint32_t* d_list = nullptr;     // exact rows: prefilled exact-path rows, then uncertified
+    uint32_t* d_scalars = nullptr;  // [0] flags, [1] list count, [2] item counter
+};
+
+// Scalar slots of Plan::d_scalars.
+constexpr int kScalarFlags = 0;
+constexpr int kScalarListCount = 1;
+constexpr int kScalarItemCounter = 2;
+constexpr int kScalarCount = 16;
 
 scrust_hip_status fail(scrust_hip_status status, const std::string& message);
 scrust_hip_status invalid(const char* parameter, const char* constraint, const std::string& value);
@@ -38,6 +68,21 @@ const char* last_error();
 void clear_error();
 bool device_is_gfx950(int device, hipDeviceProp_t* props_out);
 
+// Lays out a plan for `specs` (no device work).
+scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
+                              const scrust_hip_knn_options* options, Plan& plan);
+// Points the plan's descriptors into `arena` and `meta` (sized arena_bytes / meta_bytes)
+// and uploads descriptors, items and the exact-row list on `stream`. With `pinned` (host
+// memory of at least meta_bytes that stays untouched until the copy completes) the upload
+// is one asynchronous copy; without it the copies read the plan's own vectors.
+scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t stream,
+                             void* pinned = nullptr);
+// Enqueues the whole pipeline. `single` uses the compile-time-specialised pass kernel
+// (one dataset); otherwise the persistent batched pass. Events (6, optional) mark the
+// stage boundaries.
+scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
+                               hipStream_t stream, hipEvent_t* events);
+
 }  // namespace scrust_hip
 
 struct scrust_hip_context {
@@ -45,12 +90,16 @@ struct scrust_hip_context {
     int compute_units = 0;
     hipStream_t stream = nullptr;
     hipEvent_t events[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    scrust_hip::Workspace ws;
+    scrust_hip::Buffer arena, meta;                       // single-call plan workspace
+    scrust_hip::Buffer input, out_indices, out_distances;  // host-input entry point
     ~scrust_hip_context();
 };
 
-namespace scrust_hip {
-scrust_hip_status run_knn(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
-                          const scrust_hip_knn_options* options, uint32_t* out_indices,
-                          float* out_distances, scrust_hip_knn_stats* stats);
-}
+struct scrust_hip_batch {
+    scrust_hip_context* context = nullptr;
+    scrust_hip::Plan plan;
+    scrust_hip::Buffer arena, meta;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~scrust_hip_batch();
+};

@@ -1,0 +1,83 @@
+"""The serving loop: admission by device memory, concurrent builds, and results that are
+the same alone and in the mix (bit for bit: every build is deterministic)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mrustsc_amd
+from mrustsc_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(300), np.log(20000))))
+        d = int(rng.choice([8, 30, 50]))
+        k = int(rng.choice([5, 15, 30]))
+        out.append((synthetic.clustered(n, d, seed=seed * 100 + i, n_clusters=5), k))
+    return out
+
+
+def test_results_in_the_mix_equal_results_alone(context):
+    reqs = _requests(24, seed=1)
+    alone = [mrustsc_amd.knn_hip(x, k, context=context) for x, k in reqs]
+    print(f"[case] server 24 requests, 4 slots", flush=True)
+    with mrustsc_amd.Server(slots=4) as server:
+        tickets = [server.submit(x, k) for x, k in reqs]
+        for (ids_a, dist_a), t in zip(alone, tickets):
+            ids, dist, info = server.wait(t)
+            np.testing.assert_array_equal(ids, ids_a)
+            np.testing.assert_array_equal(dist, dist_a)
+            assert info["submit_ns"] <= info["admit_ns"] <= info["done_ns"]
+        stats = server.stats()
+    assert stats["completed"] == 24 and stats["submitted"] == 24
+    assert stats["peak_in_flight"] >= 2, "builds never overlapped"
+
+
+def test_admission_respects_the_memory_budget(context):
+    reqs = _requests(12, seed=2)
+    budget = 96 << 20
+    print(f"[case] server budget {budget}", flush=True)
+    with mrustsc_amd.Server(slots=8, memory_bytes=budget) as server:
+        tickets = []
+        for x, k in reqs:
+            try:
+                tickets.append((server.submit(x, k), x, k))
+            except RuntimeError as err:  # larger than the whole budget
+                assert "budget" in str(err)
+        assert tickets, "every request was larger than the budget"
+        for t, x, k in tickets:
+            ids, dist, _ = server.wait(t)
+            want_ids, want_dist = mrustsc_amd.knn_hip(x, k, context=context)
+            np.testing.assert_array_equal(ids, want_ids)
+            np.testing.assert_array_equal(dist, want_dist)
+        stats = server.stats()
+    assert stats["peak_memory_bytes"] <= budget
+
+
+def test_a_request_larger_than_the_budget_is_refused():
+    with mrustsc_amd.Server(slots=2, memory_bytes=16 << 20) as server:
+        x = synthetic.clustered(60000, 50, seed=3)
+        with pytest.raises(RuntimeError, match="budget"):
+            server.submit(x, 15)
+        assert server.stats()["rejected"] == 1
+
+
+def test_a_bad_request_fails_alone(context):
+    good = synthetic.clustered(2000, 16, seed=4)
+    bad = good.copy()
+    bad[3, 2] = np.inf
+    with mrustsc_amd.Server(slots=2) as server:
+        t_bad = server.submit(bad, 10)
+        t_good = server.submit(good, 10)
+        with pytest.raises(ValueError):
+            server.wait(t_bad)
+        ids, dist, _ = server.wait(t_good)
+    want_ids, want_dist = mrustsc_amd.knn_hip(good, 10, context=context)
+    np.testing.assert_array_equal(ids, want_ids)
+    np.testing.assert_array_equal(dist, want_dist)
