@@ -18,6 +18,16 @@ constexpr uint64_t kNoKey = ~0ull;
 constexpr float kF32Epsilon = 1.1920928955078125e-07f;  // 2^-23, f32::EPSILON
 constexpr int kAccWords = 10;                          // exact-sum digits, 32 bits each
 
+// Global-memory address space. Pointers read from a descriptor in memory are generic to
+// the compiler, which then emits FLAT accesses; FLAT loads also count against the LDS
+// wait counter, so every LDS wait would wait for outstanding HBM loads as well.
+#define SCRUST_GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ SCRUST_GLOBAL T* global_ptr(T* p) {
+    return (SCRUST_GLOBAL T*)p;
+}
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 

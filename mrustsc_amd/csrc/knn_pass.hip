@@ -63,8 +63,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 // radix select: a binary search over the ordered score bits, then over ids among keys
 // that tie on the boundary score.
 template <int E>
-__device__ float select_smallest(const uint64_t* rowbuf, uint32_t count, int keep, int lane,
-                                 uint64_t* out) {
+__device__ float select_smallest(const SCRUST_GLOBAL uint64_t* rowbuf, uint32_t count, int keep,
+                                 int lane, SCRUST_GLOBAL uint64_t* out) {
     uint64_t key[E];
 #pragma unroll
     for (int i = 0; i < E; i++) {
@@ -114,8 +114,9 @@ __device__ float select_smallest(const uint64_t* rowbuf, uint32_t count, int kee
     return unordered(v);
 }
 
-__device__ __forceinline__ float select_by_cap(uint32_t cap, const uint64_t* rowbuf,
-                                               uint32_t count, int keep, int lane, uint64_t* out) {
+__device__ __forceinline__ float select_by_cap(uint32_t cap, const SCRUST_GLOBAL uint64_t* rowbuf,
+                                               uint32_t count, int keep, int lane,
+                                               SCRUST_GLOBAL uint64_t* out) {
     if (cap <= 128) return select_smallest<2>(rowbuf, count, keep, lane, out);
     if (cap <= 256) return select_smallest<4>(rowbuf, count, keep, lane, out);
     return select_smallest<8>(rowbuf, count, keep, lane, out);
@@ -164,8 +165,9 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
     const uint32_t cap = s.cap;
     const int kp = static_cast<int>(s.kp);
     const uint32_t qbase = block * kPassRows;
-    const u32x4* ref_rows = reinterpret_cast<const u32x4*>(
-        OP == kOperandBf16 ? static_cast<const void*>(s.pack_ref) : static_cast<const void*>(s.pack_f32));
+    const SCRUST_GLOBAL u32x4* ref_rows = global_ptr(reinterpret_cast<const u32x4*>(
+        OP == kOperandBf16 ? static_cast<const void*>(s.pack_ref) : static_cast<const void*>(s.pack_f32)));
+    SCRUST_GLOBAL uint64_t* const buf = global_ptr(s.buf);
 
     // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
     // qb's fragment in lane l carries row qb*16 + l%16.
@@ -175,12 +177,12 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
         uint32_t row = qbase + w * kWave + qb * 16 + lc;
         if (row >= n) row = n - 1;
         if constexpr (OP == kOperandBf16) {
-            const u32x4* src = reinterpret_cast<const u32x4*>(s.pack_query) + static_cast<size_t>(row) * row_u4 + g * nch;
+            const SCRUST_GLOBAL u32x4* src = global_ptr(reinterpret_cast<const u32x4*>(s.pack_query)) + static_cast<size_t>(row) * row_u4 + g * nch;
 #pragma unroll
             for (int c = 0; c < NCH; c++)
                 a[qb][c] = (FIXED || c < nch) ? __builtin_bit_cast(bf16x8, src[c]) : bf16x8{};
         } else {
-            const float* src = s.pack_f32 + static_cast<size_t>(row) * 4 * group + g * group;
+            const SCRUST_GLOBAL float* src = global_ptr(s.pack_f32) + static_cast<size_t>(row) * 4 * group + g * group;
 #pragma unroll
             for (int c = 0; c < NCH; c++)
                 a[qb][c] = (FIXED || c < nch) ? ((g * nch + c == static_cast<int>(s.d)) ? -0.5f : src[c]) : 0.0f;
@@ -268,7 +270,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
                     if (acc[qb][r] > phi[qb]) {
                         const uint32_t id = t * kTileRefs + cb * 16 + g * 4 + r;
                         const uint32_t slot = atomicAdd(&sm.cnt[ql], 1u);
-                        s.buf[static_cast<size_t>(qbase + ql) * cap + slot] = make_key(-2.0f * acc[qb][r], id);
+                        buf[static_cast<size_t>(qbase + ql) * cap + slot] = make_key(-2.0f * acc[qb][r], id);
                     }
                 }
             }
@@ -303,7 +305,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
                 const int q = __builtin_ctzll(full);
                 full &= full - 1;
                 const int ql = w * kWave + q;
-                uint64_t* rowbuf = s.buf + static_cast<size_t>(qbase + ql) * cap;
+                SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(qbase + ql) * cap;
                 const uint32_t count = sm.cnt[ql];
                 const float theta = select_by_cap(cap, rowbuf, count, kp, lane, rowbuf);
                 if (lane == 0) {
@@ -323,15 +325,15 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
         const int ql = w * kWave + q;
         const uint32_t row = qbase + ql;
         if (row >= n) break;
-        const uint64_t* rowbuf = s.buf + static_cast<size_t>(row) * cap;
-        uint64_t* out = s.cand + static_cast<size_t>(row) * kp;
+        const SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(row) * cap;
+        SCRUST_GLOBAL uint64_t* out = global_ptr(s.cand) + static_cast<size_t>(row) * kp;
         const uint32_t count = sm.cnt[ql];
         if (count >= static_cast<uint32_t>(kp)) {
             const float theta = select_by_cap(cap, rowbuf, count, kp, lane, out);
-            if (lane == 0) s.theta[row] = theta;
+            if (lane == 0) global_ptr(s.theta)[row] = theta;
         } else {
             if (lane < kp) out[lane] = static_cast<uint32_t>(lane) < count ? rowbuf[lane] : kNoKey;
-            if (lane == 0) s.theta[row] = kInf;
+            if (lane == 0) global_ptr(s.theta)[row] = kInf;
         }
     }
 }
