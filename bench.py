@@ -38,9 +38,40 @@ def parse_args():
     p.add_argument("--fp32-steps", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp32", action="store_true")
-    p.add_argument("--cpu-rows", type=int, default=4096)
+    p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only copy a MIB-sized device buffer three times and exit: the "
+                        "known-size copy that calibrates the HBM byte counters")
     return p.parse_args()
+
+
+# HBM bytes per launch of the bf16 pass at the default workload, from the FETCH_SIZE and
+# WRITE_SIZE counters (each scaled by the reading of a known-size copy); the summary file
+# holds the commands and the raw readings.
+TRAFFIC_SUMMARY = ROOT / "profiles" / "pass_kernel_bf16_1Mx50_k15_traffic.md"
+
+
+def measured_traffic(n: int, d: int, k: int):
+    if (n, d, k) != (1_000_000, 50, 15) or not TRAFFIC_SUMMARY.exists():
+        return None
+    for line in TRAFFIC_SUMMARY.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(mib: int) -> int:
+    import torch
+
+    src = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel(), "launches": 3}), flush=True)
+    return 0
 
 
 def log(msg: str) -> None:
@@ -54,6 +85,8 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(the path shards by query rows; see the design notes)"}), flush=True)
         return 2
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
 
     import numpy as np
     import torch
@@ -122,8 +155,10 @@ def main() -> int:
         "fallback_rows": int(st_bf16[-1]["fallback_rows"]),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
-                     "traffic": None,
-                     "kernel": "pass_bf16_kernel", "algorithmic_flops": flops},
+                     "traffic": measured_traffic(n, d, k),
+                     "kernel": "pass_kernel<1, 5> (bf16 operands, 5 chunks of 32)",
+                     "algorithmic_flops": flops,
+                     "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs"},
     }
 
     if not args.no_fp32:

@@ -12,9 +12,11 @@
 // with one max and one compare; a block's tests form one wave-uniform branch.
 //
 // Selection: a pair with s < theta[row] is appended to the row's `cap`-slot buffer in
-// global memory (slot from an LDS counter). After each tile, a row whose buffer could
-// overflow in the next tile is compacted by its wave (ballot radix select of the KP
-// smallest keys); theta becomes the KP-th smallest score. theta only decreases, so every
+// global memory. The buffer is four segments, one per lane group of the query row's four
+// lanes, each with its own counter in a register: appends need no atomics and no LDS.
+// After each tile, a row with a segment that could overflow in the next tile is compacted
+// by its wave (ballot radix select of the KP smallest keys, KP/4 written back to each
+// segment); theta becomes the KP-th smallest score. theta only decreases, so every
 // reference that is not kept has s >= theta_final: the bound the rerank certifies with.
 //
 // Operands (written by prepare_kernel):
@@ -58,19 +60,37 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Keep the `keep` smallest of the `count` keys at rowbuf[0..count) (keep <= count <= E*64):
-// they are written, unordered, to out[0..keep); returns the keep-th smallest score. Ballot
-// radix select: a binary search over the ordered score bits, then over ids among keys
-// that tie on the boundary score.
+// Row buffers: a row's `cap` slots are four segments of cap/4, segment g appended to by
+// lane group g (lanes 16g..16g+15) with a per-lane counter; c[g] keys of segment g are
+// valid.
+
+// Loads a row's valid keys, E per lane (E * 64 == cap, key index e = i*64 + lane), and
+// kNoKey in the other places.
 template <int E>
-__device__ float select_smallest(const SCRUST_GLOBAL uint64_t* rowbuf, uint32_t count, int keep,
-                                 int lane, SCRUST_GLOBAL uint64_t* out) {
-    uint64_t key[E];
+__device__ __forceinline__ void gather_row(const SCRUST_GLOBAL uint64_t* rowbuf,
+                                           const uint32_t (&c)[4], int lane, uint64_t (&key)[E]) {
+    constexpr int kSeg = E * 16;
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        const uint32_t e = i * kWave + lane;
-        key[i] = e < count ? rowbuf[e] : kNoKey;
+        const int e = i * kWave + lane;
+        const int sg = e / kSeg, o = e % kSeg;
+        const uint32_t cs = sg == 0 ? c[0] : sg == 1 ? c[1] : sg == 2 ? c[2] : c[3];
+        key[i] = static_cast<uint32_t>(o) < cs ? rowbuf[e] : kNoKey;
     }
+}
+
+// Place of the j-th kept key: out[j], or with quarter > 0 back in the four segments
+// (quarter keys at the start of each segment of `seg` slots).
+__device__ __forceinline__ uint32_t kept_slot(uint32_t j, uint32_t quarter, uint32_t seg) {
+    return quarter ? (j / quarter) * seg + j % quarter : j;
+}
+
+// Keeps the `keep` smallest keys (at least `keep` are valid), written unordered through
+// kept_slot; returns the keep-th smallest score. Ballot radix select: a binary search over
+// the ordered score bits, then over ids among keys that tie on the boundary score.
+template <int E>
+__device__ float select_keys(const uint64_t (&key)[E], int keep, int lane,
+                             SCRUST_GLOBAL uint64_t* out, uint32_t quarter, uint32_t seg) {
     uint32_t lo = 0, hi = 0xFFFFFFFFu;
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
@@ -83,9 +103,9 @@ __device__ float select_smallest(const SCRUST_GLOBAL uint64_t* rowbuf, uint32_t 
     int below = 0, equal = 0;
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        const uint32_t s = static_cast<uint32_t>(key[i] >> 32);
-        below += __popcll(__ballot(s < v));
-        equal += __popcll(__ballot(s == v));
+        const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
+        below += __popcll(__ballot(sc < v));
+        equal += __popcll(__ballot(sc == v));
     }
     const int need = keep - below;
     uint32_t idcut = 0xFFFFFFFFu;
@@ -105,21 +125,62 @@ __device__ float select_smallest(const SCRUST_GLOBAL uint64_t* rowbuf, uint32_t 
     uint32_t base = 0;
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        const uint32_t s = static_cast<uint32_t>(key[i] >> 32);
-        const bool kept = s < v || (s == v && static_cast<uint32_t>(key[i]) <= idcut);
+        const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
+        const bool kept = sc < v || (sc == v && static_cast<uint32_t>(key[i]) <= idcut);
         const uint64_t m = __ballot(kept);
-        if (kept) out[base + lanes_below(m)] = key[i];
+        if (kept) out[kept_slot(base + lanes_below(m), quarter, seg)] = key[i];
         base += static_cast<uint32_t>(__popcll(m));
     }
     return unordered(v);
 }
 
-__device__ __forceinline__ float select_by_cap(uint32_t cap, const SCRUST_GLOBAL uint64_t* rowbuf,
-                                               uint32_t count, int keep, int lane,
-                                               SCRUST_GLOBAL uint64_t* out) {
-    if (cap <= 128) return select_smallest<2>(rowbuf, count, keep, lane, out);
-    if (cap <= 256) return select_smallest<4>(rowbuf, count, keep, lane, out);
-    return select_smallest<8>(rowbuf, count, keep, lane, out);
+// Compaction (quarter > 0, out == rowbuf) or final selection (quarter == 0) of one row.
+template <int E>
+__device__ __forceinline__ float select_row(const SCRUST_GLOBAL uint64_t* rowbuf,
+                                            const uint32_t (&c)[4], int keep, int lane,
+                                            SCRUST_GLOBAL uint64_t* out, uint32_t quarter) {
+    uint64_t key[E];
+    gather_row<E>(rowbuf, c, lane, key);
+    return select_keys<E>(key, keep, lane, out, quarter, E * 16);
+}
+
+__device__ __forceinline__ float select_row_by_cap(uint32_t cap, const SCRUST_GLOBAL uint64_t* rowbuf,
+                                                   const uint32_t (&c)[4], int keep, int lane,
+                                                   SCRUST_GLOBAL uint64_t* out, uint32_t quarter) {
+    if (cap <= 128) return select_row<2>(rowbuf, c, keep, lane, out, quarter);
+    if (cap <= 256) return select_row<4>(rowbuf, c, keep, lane, out, quarter);
+    return select_row<8>(rowbuf, c, keep, lane, out, quarter);
+}
+
+// A row with fewer than `keep` valid keys: with quarter == 0 (final output, n - 1 < keep)
+// its keys then kNoKey to out[0..keep); with quarter > 0 its keys spread back over the
+// four segments through kept_slot, none discarded.
+template <int E>
+__device__ __forceinline__ void copy_row(const SCRUST_GLOBAL uint64_t* rowbuf, const uint32_t (&c)[4],
+                                         int keep, int lane, SCRUST_GLOBAL uint64_t* out,
+                                         uint32_t quarter) {
+    uint64_t key[E];
+    gather_row<E>(rowbuf, c, lane, key);
+    // Every ballot before any store: all keys are in registers before out (possibly the
+    // same buffer) is written.
+    uint64_t m[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) m[i] = __ballot(key[i] != kNoKey);
+    uint32_t base = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        if (key[i] != kNoKey) out[kept_slot(base + lanes_below(m[i]), quarter, E * 16)] = key[i];
+        base += static_cast<uint32_t>(__popcll(m[i]));
+    }
+    if (quarter == 0 && static_cast<uint32_t>(lane) >= base && lane < keep) out[lane] = kNoKey;
+}
+
+__device__ __forceinline__ void copy_row_by_cap(uint32_t cap, const SCRUST_GLOBAL uint64_t* rowbuf,
+                                                const uint32_t (&c)[4], int keep, int lane,
+                                                SCRUST_GLOBAL uint64_t* out, uint32_t quarter) {
+    if (cap <= 128) copy_row<2>(rowbuf, c, keep, lane, out, quarter);
+    else if (cap <= 256) copy_row<4>(rowbuf, c, keep, lane, out, quarter);
+    else copy_row<8>(rowbuf, c, keep, lane, out, quarter);
 }
 
 template <int OP, int NCH>
@@ -145,8 +206,6 @@ struct Operand<kOperandF32, NCH> {
 template <int OP, int NCH>
 struct PassSmem {
     u32x4 tile[2][kTileRefs * (Operand<OP, NCH>::kRowU4Max + 1)];
-    uint32_t cnt[kPassRows];
-    float phl[kPassRows];
 };
 
 // One work item: query rows [block * 256, block * 256 + 256) of dataset s against all of
@@ -188,33 +247,45 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
                 a[qb][c] = (FIXED || c < nch) ? ((g * nch + c == static_cast<int>(s.d)) ? -0.5f : src[c]) : 0.0f;
         }
     }
-    sm.cnt[tid] = 0;
-    sm.phl[tid] = -kInf;
-    // phi[qb] = -theta/2 of this lane's query row qb*16 + l%16.
+    // phi[qb] = -theta/2 of this lane's query row qb*16 + l%16; cnt[qb] = keys in this
+    // lane's segment of that row's buffer.
     float phi[4];
+    uint32_t cnt[4];
 #pragma unroll
-    for (int qb = 0; qb < 4; qb++) phi[qb] = -kInf;
+    for (int qb = 0; qb < 4; qb++) {
+        phi[qb] = -kInf;
+        cnt[qb] = 0;
+    }
+    const uint32_t seg = cap / 4;
+    const uint32_t quarter = static_cast<uint32_t>(kp) / 4;
+    // A tile appends at most 16 keys to a segment (4 column blocks x 4 references).
+    const uint32_t trigger = seg - 16;
+    SCRUST_GLOBAL uint64_t* const lane_seg =
+        buf + static_cast<size_t>(qbase + w * kWave + lc) * cap + g * seg;
 
     const uint32_t n_tiles = (n + kTileRefs - 1) / kTileRefs;
-    u32x4 pre[Op::kLoadsMax];
-    // Global -> registers for tile t (row n is the sentinel); registers -> LDS buffer b.
-    // Plain unrolled loops with constant indices keep `pre` in registers.
-#define SCRUST_LOAD_TILE(T)                                                       \
+    // Two register sets stage tiles t+1 and t+2 while tile t is consumed from LDS, so a
+    // tile's loads are issued two tiles before they are stored (HBM/L3 latency exceeds
+    // one tile's MFMA time).
+    u32x4 pa[Op::kLoadsMax], pb[Op::kLoadsMax];
+    // Global -> registers P for tile T (row n is the sentinel); registers P -> LDS buffer B.
+    // Plain unrolled loops with constant indices keep the sets in registers.
+#define SCRUST_LOAD_TILE(P, T)                                                    \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
         if (FIXED || i < loads) {                                                 \
             const int u = tid + 256 * i;                                          \
             const int rl = u / row_u4, part = u % row_u4;                         \
             uint32_t ref = (T) * kTileRefs + rl;                                  \
             if (ref > n) ref = n;                                                 \
-            pre[i] = ref_rows[static_cast<size_t>(ref) * row_u4 + part];         \
+            P[i] = ref_rows[static_cast<size_t>(ref) * row_u4 + part];           \
         }                                                                         \
     }
-#define SCRUST_STORE_TILE(B)                                                      \
+#define SCRUST_STORE_TILE(P, B)                                                   \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
         if (FIXED || i < loads) {                                                 \
             const int u = tid + 256 * i;                                          \
             const int rl = u / row_u4, part = u % row_u4;                         \
-            sm.tile[B][rl * stride + part] = pre[i];                              \
+            sm.tile[B][rl * stride + part] = P[i];                                \
         }                                                                         \
     }
     auto mfma_block = [&](f32x4 (&acc)[4], int cur, int cb) __attribute__((always_inline)) {
@@ -254,37 +325,47 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
     // acc[qb][r] holds reference 4*(l/16) + r of the block against query qb*16 + l%16, so a
     // lane's four values share one threshold: one max and one compare per query block.
     auto filter = [&](const f32x4 (&acc)[4], uint32_t t, int cb) __attribute__((always_inline)) {
+        uint64_t hit[4];
         uint64_t any = 0;
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
             const float m = __builtin_fmaxf(__builtin_fmaxf(acc[qb][0], acc[qb][1]),
                                             __builtin_fmaxf(acc[qb][2], acc[qb][3]));
-            any |= __ballot(m > phi[qb]);
+            hit[qb] = __ballot(m > phi[qb]);
+            any |= hit[qb];
         }
         if (any) {
+            const uint32_t id0 = t * kTileRefs + cb * 16 + g * 4;
 #pragma unroll
             for (int qb = 0; qb < 4; qb++) {
-                const int ql = w * kWave + qb * 16 + lc;
+                if (hit[qb]) {
+                    SCRUST_GLOBAL uint64_t* const dst = lane_seg + static_cast<size_t>(qb * 16) * cap;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    if (acc[qb][r] > phi[qb]) {
-                        const uint32_t id = t * kTileRefs + cb * 16 + g * 4 + r;
-                        const uint32_t slot = atomicAdd(&sm.cnt[ql], 1u);
-                        buf[static_cast<size_t>(qbase + ql) * cap + slot] = make_key(-2.0f * acc[qb][r], id);
+                    for (int r = 0; r < 4; r++) {
+                        if (acc[qb][r] > phi[qb]) {
+                            dst[cnt[qb]] = make_key(-2.0f * acc[qb][r], id0 + r);
+                            cnt[qb]++;
+                        }
                     }
                 }
             }
         }
     };
+    // Segment counts of wave-local row q = qb*16 + lr, read from its four lanes.
+    auto row_counts = [&](int q, uint32_t (&c)[4]) __attribute__((always_inline)) {
+        const int qb = q >> 4, lr = q & 15;
+        static_for<0, 4>([&](auto QB) {
+            if (qb == QB) {
+#pragma unroll
+                for (int gg = 0; gg < 4; gg++)
+                    c[gg] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt[QB]), gg * 16 + lr));
+            }
+        });
+    };
 
-    SCRUST_LOAD_TILE(0u)
-    SCRUST_STORE_TILE(0)
-    __syncthreads();
-    for (uint32_t t = 0; t < n_tiles; t++) {
-        const int cur = t & 1;
-        // Unconditional prefetch (the last tile reloads itself into the idle buffer): a
-        // conditional load keeps `pre` out of registers and serialises the prefetch.
-        SCRUST_LOAD_TILE(t + 1 < n_tiles ? t + 1 : t)
+    // Tile t: its MFMAs and filter on LDS buffer cur, then the next tile's registers to the
+    // other buffer, a barrier, and the compaction of rows whose buffers could overflow.
+    auto consume = [&](uint32_t t, int cur) __attribute__((always_inline)) {
         f32x4 acc0[4], acc1[4];
         // Software pipeline: the MFMAs of block cb are in flight while block cb-1 is filtered.
         mfma_block(acc0, cur, 0);
@@ -295,44 +376,83 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
         mfma_block(acc1, cur, 3);
         filter(acc0, t, 2);
         filter(acc1, t, 3);
-        SCRUST_STORE_TILE(cur ^ 1)
-        __syncthreads();
-        // Compact every row of this wave whose buffer could overflow in the next tile.
-        const uint32_t mine = sm.cnt[w * kWave + lane];
-        uint64_t full = __ballot(mine > cap - kTileRefs);
-        if (full) {
-            while (full) {
-                const int q = __builtin_ctzll(full);
-                full &= full - 1;
-                const int ql = w * kWave + q;
-                SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(qbase + ql) * cap;
-                const uint32_t count = sm.cnt[ql];
-                const float theta = select_by_cap(cap, rowbuf, count, kp, lane, rowbuf);
-                if (lane == 0) {
-                    sm.cnt[ql] = kp;
-                    sm.phl[ql] = -0.5f * theta;
-                }
+    };
+    // Compacts every row of this wave with a segment that could overflow in the next tile:
+    // to its KP smallest keys (quarter per segment), theta becoming the KP-th score; a row
+    // that holds fewer than KP keys (a skewed segment, small caps) is only spread evenly
+    // over its segments, its theta unchanged.
+    auto compact = [&]() __attribute__((always_inline)) {
+        uint64_t full = 0;
+        static_for<0, 4>([&](auto QB) {
+            const uint64_t f = __ballot(cnt[QB] > trigger);
+            full |= ((f | f >> 16 | f >> 32 | f >> 48) & 0xFFFFull) << (16 * QB);
+        });
+        while (full) {
+            const int q = __builtin_ctzll(full);
+            full &= full - 1;
+            uint32_t c[4];
+            row_counts(q, c);
+            SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(qbase + w * kWave + q) * cap;
+            const uint32_t total = c[0] + c[1] + c[2] + c[3];
+            const int qb = q >> 4, lr = q & 15;
+            if (total >= static_cast<uint32_t>(kp)) {
+                const float next_phi = -0.5f * select_row_by_cap(cap, rowbuf, c, kp, lane, rowbuf, quarter);
+                static_for<0, 4>([&](auto QB) {
+                    if (qb == QB && lc == lr) {
+                        cnt[QB] = quarter;
+                        phi[QB] = next_phi;
+                    }
+                });
+            } else {
+                copy_row_by_cap(cap, rowbuf, c, kp, lane, rowbuf, quarter);
+                // Segment g now holds min(quarter, total - g * quarter) keys.
+                const uint32_t start = static_cast<uint32_t>(g) * quarter;
+                const uint32_t mine = total > start ? (total - start < quarter ? total - start : quarter) : 0u;
+                static_for<0, 4>([&](auto QB) {
+                    if (qb == QB && lc == lr) cnt[QB] = mine;
+                });
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-#pragma unroll
-            for (int qb = 0; qb < 4; qb++) phi[qb] = sm.phl[w * kWave + qb * 16 + lc];
         }
+    };
+
+    // The scan runs over an even number of tiles so the loop has one exit, at its bottom:
+    // an odd count gets one more tile of sentinel rows, which score -inf and are never
+    // candidates. (A mid-loop exit would merge into the loop head, where the compiler must
+    // then assume the prefetch just issued is outstanding and waits for it.) Loads past
+    // the end reload the last tile; unconditional loads keep both sets in registers.
+    const uint32_t n_even = (n_tiles + 1) & ~1u;
+    const uint32_t last_load = n_even - 1;
+    SCRUST_LOAD_TILE(pa, 0u)
+    SCRUST_LOAD_TILE(pb, 1u)
+    SCRUST_STORE_TILE(pa, 0)
+    __syncthreads();
+    for (uint32_t t = 0; t < n_even; t += 2) {
+        SCRUST_LOAD_TILE(pa, t + 2 < last_load ? t + 2 : last_load)
+        consume(t, 0);
+        SCRUST_STORE_TILE(pb, 1)
+        __syncthreads();
+        compact();
+        SCRUST_LOAD_TILE(pb, t + 3 < last_load ? t + 3 : last_load)
+        consume(t + 1, 1);
+        SCRUST_STORE_TILE(pa, 0)
+        __syncthreads();
+        compact();
     }
 #undef SCRUST_LOAD_TILE
 #undef SCRUST_STORE_TILE
     // Final selection: the KP smallest keys of each row (unordered) and the KP-th score.
     for (int q = 0; q < kWave; q++) {
-        const int ql = w * kWave + q;
-        const uint32_t row = qbase + ql;
+        const uint32_t row = qbase + w * kWave + q;
         if (row >= n) break;
+        uint32_t c[4];
+        row_counts(q, c);
         const SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(row) * cap;
         SCRUST_GLOBAL uint64_t* out = global_ptr(s.cand) + static_cast<size_t>(row) * kp;
-        const uint32_t count = sm.cnt[ql];
-        if (count >= static_cast<uint32_t>(kp)) {
-            const float theta = select_by_cap(cap, rowbuf, count, kp, lane, out);
+        if (c[0] + c[1] + c[2] + c[3] >= static_cast<uint32_t>(kp)) {
+            const float theta = select_row_by_cap(cap, rowbuf, c, kp, lane, out, 0u);
             if (lane == 0) global_ptr(s.theta)[row] = theta;
         } else {
-            if (lane < kp) out[lane] = static_cast<uint32_t>(lane) < count ? rowbuf[lane] : kNoKey;
+            copy_row_by_cap(cap, rowbuf, c, kp, lane, out, 0u);
             if (lane == 0) global_ptr(s.theta)[row] = kInf;
         }
     }

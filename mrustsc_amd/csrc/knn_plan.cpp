@@ -63,6 +63,9 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         s.on_pass = pass ? 1u : 0u;
         if (pass) {
             s.kp = sp.k + 17 <= 32 ? 32u : 64u;
+            // A row's buffer is four lane segments of cap/4 slots; a compaction leaves kp/4
+            // keys in each and a tile appends at most 16 per segment.
+            if (s.cap < 4 * s.kp) s.cap = 4 * s.kp;
             if (plan.operand == kOperandBf16) {
                 // Single calls use the compiled chunk counts; the batched kernel takes any.
                 s.nch = single ? bf16_chunks(static_cast<int>(sp.d))
@@ -111,7 +114,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             } else {
                 s.pack_f32 = static_cast<float*>(take((sp.n + 1) * 16ull * s.group));
             }
-            s.buf = static_cast<uint64_t*>(take(npad * cap * sizeof(uint64_t)));
+            s.buf = static_cast<uint64_t*>(take(npad * s.cap * sizeof(uint64_t)));
             s.cand = static_cast<uint64_t*>(take(sp.n * s.kp * sizeof(uint64_t)));
             s.theta = static_cast<float*>(take(sp.n * sizeof(float)));
             for (uint32_t b = 0; b < npad / kPassRows; b++) plan.items.push_back(PassItem{static_cast<uint32_t>(plan.sets.size()), b});
