@@ -92,8 +92,8 @@ __global__ void clear_sums_kernel(const DatasetDesc* __restrict__ sets) {
 // 2. Coordinates, norms and MFMA operands.
 //    Euclidean: (x as f64 - mean) rounded once (neighbors.rs:155-160); norms by f32
 //    fma in dimension order (knn.rs:148-156). Cosine: x / |x| with the norm in f64.
-//    Operand layouts are described in knn_pass.hip. Row n of each dataset writes the
-//    sentinel reference row (norm feature +inf).
+//    Operand layouts are described in knn_pass.hip. Rows n.. of a dataset on the pass
+//    (up to a whole pair of 64-row tiles) write sentinel reference rows (norm feature +inf).
 // ===========================================================================
 
 __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t count,

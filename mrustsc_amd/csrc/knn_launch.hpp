@@ -29,6 +29,7 @@ constexpr int kMaxPassK = 47;   // largest k the MFMA candidate pass serves (KP 
 constexpr int kMaxPassD = 128;  // largest d with a compiled MFMA variant
 constexpr int kAccWordsHost = 10;
 constexpr int kPassRows = 256;  // query rows per pass work item (4 waves x 64)
+constexpr int kTileRefsHost = 64;  // references per pass tile
 
 // Device-visible description of one embedding and its workspace.
 struct DatasetDesc {

@@ -25,8 +25,8 @@
 //         (|b|^2 = nh + nl + nl2 + O(2^-24)); 32 * nch packed features per row.
 //   fp32: feature f of 4 * group slots sits at (f / nch) * group + f % nch; features are
 //         [c | |b|^2 | 0] for references; the kernel puts -1/2 at feature d for queries.
-//   Row n of the reference operands is a sentinel whose norm feature is +inf, so the
-//   padding columns of the last tile score -inf and are never candidates.
+//   The reference operands are padded to a whole pair of 64-row tiles with sentinel rows
+//   whose norm feature is +inf: they score -inf and are never candidates.
 //
 // Two entry points share the block body: pass_kernel (one dataset, chunk count fixed at
 // compile time) and pass_batch_kernel (a persistent grid that takes work items of many
@@ -263,30 +263,32 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
     SCRUST_GLOBAL uint64_t* const lane_seg =
         buf + static_cast<size_t>(qbase + w * kWave + lc) * cap + g * seg;
 
-    const uint32_t n_tiles = (n + kTileRefs - 1) / kTileRefs;
+    // The reference operand holds whole pairs of tiles (rows n.. are sentinels), so a
+    // tile is one contiguous run of tile_u4 16-byte chunks and the scan has an even count.
+    const uint32_t n_even = (n + 2 * kTileRefs - 1) / (2 * kTileRefs) * 2;
+    const uint32_t tile_u4 = kTileRefs * row_u4;
     // Two register sets stage tiles t+1 and t+2 while tile t is consumed from LDS, so a
     // tile's loads are issued two tiles before they are stored (HBM/L3 latency exceeds
     // one tile's MFMA time).
     u32x4 pa[Op::kLoadsMax], pb[Op::kLoadsMax];
-    // Global -> registers P for tile T (row n is the sentinel); registers P -> LDS buffer B.
-    // Plain unrolled loops with constant indices keep the sets in registers.
+    // Thread chunk i of a tile is u = tid + 256 i: reference row u / row_u4, part
+    // u % row_u4, at LDS slot row * stride + part = u + u / row_u4.
+    int lds_at[Op::kLoadsMax];
+#pragma unroll
+    for (int i = 0; i < Op::kLoadsMax; i++) {
+        const int u = tid + 256 * i;
+        lds_at[i] = u + u / row_u4;
+    }
+    // Global -> registers P for tile T; registers P -> LDS buffer B. Plain unrolled loops
+    // with constant indices keep the sets in registers.
 #define SCRUST_LOAD_TILE(P, T)                                                    \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
-        if (FIXED || i < loads) {                                                 \
-            const int u = tid + 256 * i;                                          \
-            const int rl = u / row_u4, part = u % row_u4;                         \
-            uint32_t ref = (T) * kTileRefs + rl;                                  \
-            if (ref > n) ref = n;                                                 \
-            P[i] = ref_rows[static_cast<size_t>(ref) * row_u4 + part];           \
-        }                                                                         \
+        if (FIXED || i < loads)                                                   \
+            P[i] = ref_rows[static_cast<size_t>(T) * tile_u4 + tid + 256 * i];   \
     }
 #define SCRUST_STORE_TILE(P, B)                                                   \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
-        if (FIXED || i < loads) {                                                 \
-            const int u = tid + 256 * i;                                          \
-            const int rl = u / row_u4, part = u % row_u4;                         \
-            sm.tile[B][rl * stride + part] = P[i];                                \
-        }                                                                         \
+        if (FIXED || i < loads) sm.tile[B][lds_at[i]] = P[i];                     \
     }
     auto mfma_block = [&](f32x4 (&acc)[4], int cur, int cb) __attribute__((always_inline)) {
 #pragma unroll
@@ -415,12 +417,9 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
         }
     };
 
-    // The scan runs over an even number of tiles so the loop has one exit, at its bottom:
-    // an odd count gets one more tile of sentinel rows, which score -inf and are never
-    // candidates. (A mid-loop exit would merge into the loop head, where the compiler must
-    // then assume the prefetch just issued is outstanding and waits for it.) Loads past
-    // the end reload the last tile; unconditional loads keep both sets in registers.
-    const uint32_t n_even = (n_tiles + 1) & ~1u;
+    // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
+    // merge into the loop head, where the compiler must then assume the prefetch just
+    // issued is outstanding and waits for it. Loads past the end reload the last tile.
     const uint32_t last_load = n_even - 1;
     SCRUST_LOAD_TILE(pa, 0u)
     SCRUST_LOAD_TILE(pb, 1u)

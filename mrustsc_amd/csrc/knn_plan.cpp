@@ -105,14 +105,17 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         s.mean = static_cast<double*>(take(sp.d * sizeof(double)));
         s.acc = static_cast<unsigned long long*>(take(sp.d * kAccWordsHost * sizeof(unsigned long long)));
         s.max_norm_bits = static_cast<unsigned*>(take(sizeof(unsigned)));
+        // Reference operand rows: n rounded up to whole pairs of 64-row tiles, the padding
+        // rows being sentinels (norm feature +inf).
+        const size_t ref_rows = (sp.n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
         if (pass) {
             const size_t npad = (sp.n + kPassRows - 1) / kPassRows * kPassRows;
             if (plan.operand == kOperandBf16) {
                 const size_t rowb = 64ull * s.nch;
-                s.pack_ref = static_cast<uint16_t*>(take((sp.n + 1) * rowb));
+                s.pack_ref = static_cast<uint16_t*>(take(ref_rows * rowb));
                 s.pack_query = static_cast<uint16_t*>(take(sp.n * rowb));
             } else {
-                s.pack_f32 = static_cast<float*>(take((sp.n + 1) * 16ull * s.group));
+                s.pack_f32 = static_cast<float*>(take(ref_rows * 16ull * s.group));
             }
             s.buf = static_cast<uint64_t*>(take(npad * s.cap * sizeof(uint64_t)));
             s.cand = static_cast<uint64_t*>(take(sp.n * s.kp * sizeof(uint64_t)));
@@ -124,7 +127,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         plan.shape.max_col_groups = std::max<uint32_t>(plan.shape.max_col_groups, static_cast<uint32_t>((sp.d + 63) / 64));
         plan.shape.max_d = std::max<uint32_t>(plan.shape.max_d, static_cast<uint32_t>(sp.d));
         row += s.n;
-        prep += s.n + 1;
+        prep += pass ? static_cast<uint32_t>(ref_rows) : s.n;
     }
     plan.shape.total_rows = row;
     plan.shape.total_prep = prep;

@@ -250,3 +250,17 @@ def test_boundary_near_ties_are_resolved_in_fp64(context, precision):
     assert ids[0].tolist() == [2] and dist[0].tolist() == [1.0]
     ids, _ = assert_matches_oracle(context, x, 2, precision)
     assert ids[0].tolist() == [1, 2]
+
+
+@pytest.mark.parametrize("precision", ("bf16", "fp32"))
+@pytest.mark.parametrize("tile", [128, 256, 512])
+@pytest.mark.parametrize("k", [5, 15, 30, 47])
+def test_every_candidate_buffer_size_matches_the_oracle(context, precision, tile, k):
+    # The buffer size changes when rows are compacted, never the result: each size, with
+    # KP = 32 (k <= 15) and KP = 64 (k <= 47, where 128 slots are raised to 256).
+    x = synthetic.clustered(2500, 24, seed=k + tile)
+    print(f"[case] tile={tile} k={k} precision={precision}", flush=True)
+    ids, dist = mrustsc_amd.knn_hip(x, k, precision=precision, tile=tile, context=context)
+    want_ids, want_dist = oracle.knn(x, k)
+    np.testing.assert_array_equal(ids, want_ids)
+    np.testing.assert_array_equal(dist, want_dist)
