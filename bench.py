@@ -1,13 +1,20 @@
-"""Benchmark: exact kNN graph cells/s at 1,000,000 x 50, k = 15 on one MI355X.
+"""Benchmark: exact kNN graph cells/s on one MI355X.
 
 One step = one complete kNN build of a device-resident embedding (centring, MFMA
 candidate pass, fp32 rerank, exact recomputation of uncertified rows) with outputs left
-on the device. The headline `value` is the BF16+rerank path; the plain FP32 kernel is
-timed on the same data and reported beside it, with the ratio.
+on the device. The default workload is 1,000,000 x 50, k = 15 (a 20-cluster anisotropic
+mixture with 0.5% duplicate rows); its headline `value` is the BF16+rerank path, and the
+plain FP32 kernel, the cosine metric and the host-to-host rate are measured beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
+                    [--n N] [--no-cpu] [--no-fp32]
 
-Prints ONE JSON line.
+Every configuration prints ONE JSON line:
+    c1  2,638 x 50 (8 clusters, 20 duplicate rows), full CPU baseline (best of 3)
+    c2  100,000 x 50 mixture, full CPU baseline
+    c3  1,000,000 x 50 mixture (default): BF16 vs FP32, cosine, distance errors
+    c4  64 datasets (n 500-50,000, d 30/50/100, k 15/30) in one graph launch
+    c5  serving: Poisson arrivals, n 1,000-200,000, d 50, k 15, three offered loads
 """
 
 from __future__ import annotations
@@ -25,31 +32,38 @@ if str(ROOT) not in sys.path:
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 FP32_MATRIX_PEAK_TFLOPS = 157.3  # MI355X fp32 MFMA
+METRIC = "kNN-graph cells/s at 1M x 50, k=15 (BF16+rerank vs FP32); p99 request latency"
+
+# HBM bytes per launch of the bf16 pass at the default workload, from the FETCH_SIZE and
+# WRITE_SIZE counters (each scaled by the reading of a known-size copy); the summary file
+# holds the commands and the raw readings.
+TRAFFIC_SUMMARY = ROOT / "profiles" / "pass_kernel_bf16_1Mx50_k15_traffic.md"
 
 
 def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--n", type=int, default=1_000_000)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--config", default="c3", choices=("c1", "c2", "c3", "c4", "c5"))
+    p.add_argument("--n", type=int, default=0, help="cells (c1-c3; default per config)")
     p.add_argument("--d", type=int, default=50)
     p.add_argument("--k", type=int, default=15)
     p.add_argument("--fp32-steps", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp32", action="store_true")
+    p.add_argument("--no-cosine", action="store_true")
     p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
+    p.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer three times and exit: the "
                         "known-size copy that calibrates the HBM byte counters")
     return p.parse_args()
 
 
-# HBM bytes per launch of the bf16 pass at the default workload, from the FETCH_SIZE and
-# WRITE_SIZE counters (each scaled by the reading of a known-size copy); the summary file
-# holds the commands and the raw readings.
-TRAFFIC_SUMMARY = ROOT / "profiles" / "pass_kernel_bf16_1Mx50_k15_traffic.md"
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
 
 
 def measured_traffic(n: int, d: int, k: int):
@@ -74,8 +88,310 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
-def log(msg: str) -> None:
-    print(msg, file=sys.stderr, flush=True)
+def base_line(args, value, ms, dtype="bf16"):
+    return {"metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic"}
+
+
+class Single:
+    """Device-resident single-embedding builds through knn_device."""
+
+    def __init__(self, x, k, ctx):
+        import torch
+
+        self.n, self.d = x.shape
+        self.k = k
+        self.ctx = ctx
+        self.x = torch.from_numpy(x).to("cuda")
+        self.ids = torch.empty((self.n, k), dtype=torch.int32, device="cuda")
+        self.dist = torch.empty((self.n, k), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+
+    def step(self, precision="bf16", metric="euclidean"):
+        import mrustsc_amd
+
+        return mrustsc_amd.knn_device(self.x.data_ptr(), self.n, self.d, self.k,
+                                      self.ids.data_ptr(), self.dist.data_ptr(),
+                                      precision=precision, metric=metric, context=self.ctx)
+
+    def timed(self, steps, warmup, precision="bf16", metric="euclidean"):
+        """Wall time per step (host clock around synchronised steps) and device stats."""
+        import torch
+
+        for i in range(warmup):
+            s = self.step(precision, metric)
+            log(f"[bench] {precision}/{metric} warmup {i}: device {s['device_ms']:.2f} ms "
+                f"(pass {s['pass_ms']:.2f}, fallback rows {s['fallback_rows']})")
+        torch.cuda.synchronize()
+        stats, walls = [], []
+        for i in range(steps):
+            t0 = time.perf_counter()
+            stats.append(self.step(precision, metric))
+            walls.append(time.perf_counter() - t0)
+            log(f"[bench] {precision}/{metric} step {i}: device {stats[-1]['device_ms']:.2f} ms")
+        torch.cuda.synchronize()
+        return walls, stats
+
+    def result(self):
+        import numpy as np
+
+        return self.ids.cpu().numpy().view(np.uint32), self.dist.cpu().numpy()
+
+
+def stage_means(stats):
+    return {key: sum(s[key] for s in stats) / len(stats)
+            for key in ("prepare_ms", "pass_ms", "rerank_ms", "fallback_ms", "device_ms")}
+
+
+def parity_block(x, k, got_ids, got_dist, rows, metric="euclidean"):
+    """Indices and distances against the oracle on `rows`; distance errors against fp64."""
+    import numpy as np
+
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    w_ids, w_dist = oracle.knn(x, k, metric=metric, queries=rows, threads=threads)
+    f_ids, f_dist = oracle.knn_fp64(x, k, metric=metric, queries=rows, threads=threads)
+    err = np.abs(got_dist[rows].astype(np.float64) - f_dist)
+    return {"rows_checked": int(rows.size),
+            "ids_equal": bool(np.array_equal(got_ids[rows], w_ids)),
+            "distances_equal": bool(np.array_equal(got_dist[rows], w_dist)),
+            "ids_equal_fp64_truth": float(np.mean(got_ids[rows] == f_ids)),
+            "distance_error_vs_fp64": {"max": float(err.max()), "mean": float(err.mean())},
+            "oracle_seconds": time.time() - t0}
+
+
+def cpu_baseline(x, k, rows=None, repeats=1):
+    import numpy as np
+
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    n = x.shape[0]
+    q = np.arange(n if rows is None else min(n, rows), dtype=np.uint32)
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        oracle.baseline_knn(x, k, queries=None if rows is None else q, threads=threads)
+        sec = time.perf_counter() - t0
+        best = sec if best is None else min(best, sec)
+    sample = (f"all {n} rows" if rows is None or q.size == n else
+              f"{q.size} query rows x all {n} references, scaled: cells/s = rows / seconds")
+    return {"value": q.size / best, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": sample + (f", best of {repeats}" if repeats > 1 else "") +
+            " (expansion path of neighbors.rs restated in C)"}
+
+
+def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
+    import numpy as np
+    import torch
+
+    import mrustsc_amd
+
+    n, d = x.shape
+    k = args.k
+    ctx = mrustsc_amd.Context(0)
+    single = Single(x, k, ctx)
+    walls, st = single.timed(args.steps, args.warmup, "bf16")
+    sec = sum(walls) / len(walls)
+    dev = stage_means(st)
+    flops = 2.0 * n * n * d
+    achieved = flops / (dev["pass_ms"] * 1e-3) / 1e12
+    out = base_line(args, n / sec, sec * 1e3)
+    out["config"] = {"workload": workload, "n_cells": n, "n_dims": d, "k": k}
+    out["best_ms"] = min(walls) * 1e3
+    out["device_cells_per_s_best"] = n / (min(s["device_ms"] for s in st) * 1e-3)
+    out["stages_ms"] = dev
+    out["fallback_rows"] = int(st[-1]["fallback_rows"])
+    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
+                       "traffic": measured_traffic(n, d, k),
+                       "kernel": f"pass_kernel<1, {(3 * d + 3 + 31) // 32}> "
+                                 "(bf16 operands, chunks of 32)",
+                       "algorithmic_flops": flops,
+                       "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs"}
+    got_ids, got_dist = single.result()
+
+    if not args.no_fp32:
+        walls32, st32 = single.timed(args.fp32_steps, 1, "fp32")
+        sec32 = sum(walls32) / len(walls32)
+        pass32 = stage_means(st32)["pass_ms"]
+        ach32 = flops / (pass32 * 1e-3) / 1e12
+        ids32, dist32 = single.result()
+        out["fp32"] = {"cells_per_s": n / sec32, "ms_per_step": sec32 * 1e3, "pass_ms": pass32,
+                       "achieved_tflops": ach32, "roofline_frac": ach32 / FP32_MATRIX_PEAK_TFLOPS,
+                       "fallback_rows": int(st32[-1]["fallback_rows"]),
+                       "identical_to_bf16": bool(np.array_equal(ids32, got_ids) and
+                                                 np.array_equal(dist32, got_dist))}
+        out["bf16_over_fp32"] = (n / sec) / (n / sec32)
+
+    rng = np.random.default_rng(0)
+    rows = np.sort(rng.choice(n, min(n, args.parity_rows), replace=False)).astype(np.uint32)
+    if args.parity_rows > 0:
+        out["parity"] = parity_block(x, k, got_ids, got_dist, rows)
+
+    if not args.no_cosine:
+        wc, stc = single.timed(max(1, args.steps // 2), 1, "bf16", "cosine")
+        c_ids, c_dist = single.result()
+        out["cosine"] = {"cells_per_s": n / (sum(wc) / len(wc)),
+                         "ms_per_step": sum(wc) / len(wc) * 1e3,
+                         "fallback_rows": int(stc[-1]["fallback_rows"])}
+        if args.parity_rows > 0:
+            out["cosine"]["parity"] = parity_block(x, k, c_ids, c_dist, rows, "cosine")
+
+    # Host to host: host embedding in, host outputs back (PCIe transfers included).
+    h2h = []
+    for i in range(3):
+        t0 = time.perf_counter()
+        mrustsc_amd.knn_hip(x, k, context=ctx)
+        h2h.append(time.perf_counter() - t0)
+    out["host_to_host"] = {"cells_per_s": n / min(h2h[1:]), "best_ms": min(h2h[1:]) * 1e3,
+                           "note": "knn_hip from host memory, best of 2 after 1 warm-up"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(x, k, cpu_rows, cpu_repeats)
+    del single
+    torch.cuda.synchronize()
+    ctx.close()
+    return out
+
+
+def run_batch_config(args):
+    """64 datasets of uneven n, d and k in one graph launch; each equals its own run."""
+    import numpy as np
+    import torch
+
+    import mrustsc_amd
+    from mrustsc_amd import synthetic
+
+    specs = synthetic.batch_specs(64, seed=7)
+    ctx = mrustsc_amd.Context(0)
+    data, outs, entries = [], [], []
+    t0 = time.time()
+    for sp in specs:
+        x = synthetic.batch_dataset(sp)
+        dx = torch.from_numpy(x).to("cuda")
+        di = torch.empty((sp["n"], sp["k"]), dtype=torch.int32, device="cuda")
+        dd = torch.empty((sp["n"], sp["k"]), dtype=torch.float32, device="cuda")
+        data.append((x, dx))
+        outs.append((di, dd))
+        entries.append((dx.data_ptr(), sp["n"], sp["d"], sp["k"], di.data_ptr(), dd.data_ptr()))
+    torch.cuda.synchronize()
+    log(f"[bench] 64 datasets ready in {time.time() - t0:.1f}s, "
+        f"{sum(s['n'] for s in specs)} cells")
+    cells = sum(s["n"] for s in specs)
+    flops = sum(2.0 * s["n"] * s["n"] * s["d"] for s in specs)
+    out = {}
+    with mrustsc_amd.Batch(entries, context=ctx) as batch:
+        for i in range(args.warmup):
+            st = batch.run()
+            log(f"[bench] batch warmup {i}: {st}")
+        walls, devs = [], []
+        for i in range(args.steps):
+            t0 = time.perf_counter()
+            st = batch.run()
+            walls.append(time.perf_counter() - t0)
+            devs.append(st["device_ms"])
+            log(f"[bench] batch step {i}: device {st['device_ms']:.2f} ms")
+        sec = sum(walls) / len(walls)
+        # Each dataset against its own standalone run.
+        equal = 0
+        t_single = 0.0
+        for sp, (x, dx), (di, dd) in zip(specs, data, outs):
+            ids = di.cpu().numpy().view(np.uint32)
+            dist = dd.cpu().numpy()
+            s0 = time.perf_counter()
+            w_ids, w_dist = mrustsc_amd.knn_hip(x, sp["k"], context=ctx)
+            t_single += time.perf_counter() - s0
+            equal += int(np.array_equal(ids, w_ids) and np.array_equal(dist, w_dist))
+        # The same 64 builds as separate device-resident calls, for comparison.
+        seq = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for sp, (x, dx), (di, dd) in zip(specs, data, outs):
+                mrustsc_amd.knn_device(dx.data_ptr(), sp["n"], sp["d"], sp["k"], di.data_ptr(),
+                                       dd.data_ptr(), context=ctx)
+            torch.cuda.synchronize()
+            seq.append(time.perf_counter() - t0)
+        achieved = flops / (min(devs) * 1e-3) / 1e12
+        out = base_line(args, cells / sec, sec * 1e3)
+        out["config"] = {"workload": "64 datasets: n log-uniform 500-50,000, d in {30, 50, 100}, "
+                                     "k in {15, 30}, 5-30 clusters, one graph launch",
+                         "datasets": 64, "cells": cells}
+        out["device_ms_best"] = min(devs)
+        out["datasets_equal_to_standalone"] = equal
+        out["sequential_calls_ms"] = min(seq) * 1e3
+        out["batch_over_sequential"] = min(seq) / sec
+        out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
+                           "traffic": None, "kernel": "whole batch (device time)",
+                           "algorithmic_flops": flops}
+    ctx.close()
+    return out
+
+
+def run_serving_config(args):
+    """Poisson arrivals at three offered loads; latency from arrival to host result."""
+    import numpy as np
+
+    import mrustsc_amd
+    from mrustsc_amd import synthetic
+
+    rng = np.random.default_rng(11)
+    pool = []
+    for i in range(24):
+        n = int(round(np.exp(rng.uniform(np.log(1000), np.log(200000)))))
+        pool.append(synthetic.clustered(n, 50, seed=100 + i, n_clusters=int(rng.integers(5, 31))))
+    log(f"[bench] serving pool: {len(pool)} embeddings, n {min(p.shape[0] for p in pool)}"
+        f"-{max(p.shape[0] for p in pool)}")
+    out = base_line(args, 0.0, 0.0)
+    with mrustsc_amd.Server(slots=4) as server:
+        # Saturation: a closed burst of requests, all submitted at once.
+        order = rng.integers(0, len(pool), size=64)
+        t0 = time.perf_counter()
+        tickets = [server.submit(pool[i], 15) for i in order]
+        for t in tickets:
+            server.wait(t)
+        sat = len(order) / (time.perf_counter() - t0)
+        log(f"[bench] saturation {sat:.1f} requests/s")
+        loads = []
+        for frac in (0.5, 0.8, 1.1):
+            rate = frac * sat
+            count = args.requests
+            gaps = rng.exponential(1.0 / rate, size=count)
+            picks = rng.integers(0, len(pool), size=count)
+            # Arrivals and the server's timestamps share CLOCK_MONOTONIC (steady_clock).
+            start = time.monotonic_ns()
+            arrival = start + (np.cumsum(gaps) * 1e9).astype(np.int64)
+            tickets = []
+            for when, i in zip(arrival, picks):
+                delay = (int(when) - time.monotonic_ns()) * 1e-9
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append((int(when), server.submit(pool[i], 15)))
+            lat = []
+            for when, t in tickets:
+                _, _, info = server.wait(t)
+                lat.append((info["done_ns"] - when) * 1e-9)
+            span = (time.monotonic_ns() - start) * 1e-9
+            lat = np.array(lat) * 1e3
+            loads.append({"offered_rps": rate, "fraction_of_saturation": frac,
+                          "requests": count, "throughput_rps": count / span,
+                          "p50_ms": float(np.percentile(lat, 50)),
+                          "p99_ms": float(np.percentile(lat, 99))})
+            log(f"[bench] load {frac}: {loads[-1]}")
+        stats = server.stats()
+    mid = loads[1]
+    out.update({"metric": METRIC + " (serving)", "unit": "requests/s",
+                "value": mid["throughput_rps"], "ms_per_step": mid["p99_ms"],
+                "p99_ms": mid["p99_ms"], "p50_ms": mid["p50_ms"],
+                "saturation_rps": sat, "loads": loads, "server": stats,
+                "config": {"workload": "Poisson arrivals, n log-uniform 1,000-200,000, d=50, "
+                                       "k=15, host embeddings, 4 slots", "pool": len(pool)}})
+    return out
 
 
 def main() -> int:
@@ -88,119 +404,32 @@ def main() -> int:
     if args.copy_probe:
         return copy_probe(args.copy_probe)
 
-    import numpy as np
-    import torch
+    import torch  # noqa: F401  (loads the HIP runtime the library then shares)
 
-    import mrustsc_amd
     from mrustsc_amd import synthetic
 
-    n, d, k = args.n, args.d, args.k
-    log(f"[bench] generating {n} x {d} clustered mixture")
-    t0 = time.time()
-    x = synthetic.clustered(n, d, seed=1)
-    log(f"[bench] data ready in {time.time() - t0:.1f}s")
-
-    ctx = mrustsc_amd.Context(0)
-    dx = torch.from_numpy(x).to("cuda")
-    ids = {p: torch.empty((n, k), dtype=torch.int32, device="cuda") for p in ("bf16", "fp32")}
-    dist = {p: torch.empty((n, k), dtype=torch.float32, device="cuda") for p in ("bf16", "fp32")}
-    torch.cuda.synchronize()
-
-    def step(precision: str) -> dict:
-        return mrustsc_amd.knn_device(dx.data_ptr(), n, d, k, ids[precision].data_ptr(),
-                                      dist[precision].data_ptr(), precision=precision,
-                                      context=ctx)
-
-    def timed(precision: str, steps: int, warmup: int):
-        for i in range(warmup):
-            s = step(precision)
-            log(f"[bench] {precision} warmup {i}: device {s['device_ms']:.1f} ms "
-                f"(pass {s['pass_ms']:.1f}, fallback rows {s['fallback_rows']})")
-        torch.cuda.synchronize()
-        stats = []
-        start = time.perf_counter()
-        for i in range(steps):
-            stats.append(step(precision))
-            log(f"[bench] {precision} step {i}: device {stats[-1]['device_ms']:.1f} ms")
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - start
-        return wall / steps, stats
-
-    sec_bf16, st_bf16 = timed("bf16", args.steps, args.warmup)
-    cells_bf16 = n / sec_bf16
-    pass_ms = sum(s["pass_ms"] for s in st_bf16) / len(st_bf16)
-    flops = 2.0 * n * n * d
-    achieved = flops / (pass_ms * 1e-3) / 1e12
-
-    result = {
-        "metric": "kNN-graph cells/s at 1M x 50, k=15 (BF16+rerank vs FP32); p99 request latency",
-        "value": cells_bf16,
-        "unit": "cells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": sec_bf16 * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "bf16",
-        "data": "synthetic",
-        "config": {"workload": f"clustered mixture {n} x {d}, k={k}, euclidean, "
-                               "bf16 hi/lo MFMA candidates (32) + fp32 rerank + fp64 ties",
-                   "n_cells": n, "n_dims": d, "k": k, "clusters": 20,
-                   "duplicate_fraction": 0.005},
-        "stages_ms": {key: sum(s[key] for s in st_bf16) / len(st_bf16)
-                      for key in ("prepare_ms", "pass_ms", "rerank_ms", "fallback_ms",
-                                  "device_ms")},
-        "fallback_rows": int(st_bf16[-1]["fallback_rows"]),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
-                     "traffic": measured_traffic(n, d, k),
-                     "kernel": "pass_kernel<1, 5> (bf16 operands, 5 chunks of 32)",
-                     "algorithmic_flops": flops,
-                     "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs"},
-    }
-
-    if not args.no_fp32:
-        sec_fp32, st_fp32 = timed("fp32", args.fp32_steps, 1)
-        pass32 = sum(s["pass_ms"] for s in st_fp32) / len(st_fp32)
-        ach32 = flops / (pass32 * 1e-3) / 1e12
-        same = bool(torch.equal(ids["bf16"], ids["fp32"]) and torch.equal(dist["bf16"], dist["fp32"]))
-        result["fp32"] = {"cells_per_s": n / sec_fp32, "ms_per_step": sec_fp32 * 1e3,
-                          "pass_ms": pass32, "achieved_tflops": ach32,
-                          "roofline_frac": ach32 / FP32_MATRIX_PEAK_TFLOPS,
-                          "fallback_rows": int(st_fp32[-1]["fallback_rows"]),
-                          "identical_to_bf16": same}
-        result["bf16_over_fp32"] = (n / sec_bf16) / (n / sec_fp32)
-
-    got_ids = ids["bf16"].cpu().numpy().view(np.uint32)
-    got_dist = dist["bf16"].cpu().numpy()
-    if args.parity_rows > 0 or not args.no_cpu:
-        import oracle
-
-        threads = min(16, os.cpu_count() or 1)
-        rng = np.random.default_rng(0)
-        if args.parity_rows > 0:
-            rows = np.sort(rng.choice(n, min(n, args.parity_rows), replace=False)).astype(np.uint32)
-            t0 = time.time()
-            w_ids, w_dist = oracle.knn(x, k, queries=rows, threads=threads)
-            result["parity"] = {
-                "rows_checked": int(rows.size),
-                "ids_equal": bool(np.array_equal(got_ids[rows], w_ids)),
-                "distances_equal": bool(np.array_equal(got_dist[rows], w_dist)),
-                "oracle_seconds": time.time() - t0,
-            }
-        if not args.no_cpu:
-            rows = np.arange(min(n, args.cpu_rows), dtype=np.uint32)
-            t0 = time.perf_counter()
-            oracle.baseline_knn(x, k, queries=rows, threads=threads)
-            sec = time.perf_counter() - t0
-            result["cpu_baseline"] = {
-                "value": rows.size / sec, "unit": "cells/s", "cores": threads, "kind": "port",
-                "sample": f"{rows.size} query rows x all {n} references (expansion path of "
-                          "neighbors.rs restated in C), scaled: cells/s = rows / seconds"}
-    print(json.dumps(result), flush=True)
-    ctx.close()
+    if args.config == "c4":
+        out = run_batch_config(args)
+    elif args.config == "c5":
+        out = run_serving_config(args)
+    else:
+        t0 = time.time()
+        if args.config == "c1":
+            x = synthetic.pbmc_like()
+            workload = "2638 x 50, 8 clusters (sd 5) + N(0, I), 20 duplicate rows, k=15"
+            cpu_rows, repeats = None, 3
+        else:
+            n = args.n or (100_000 if args.config == "c2" else 1_000_000)
+            x = synthetic.clustered(n, args.d, seed=1)
+            workload = (f"clustered mixture {n} x {args.d}, k={args.k}, euclidean, bf16 hi/lo "
+                        "MFMA candidates + fp32 rerank + fp64 ties")
+            cpu_rows = None if args.config == "c2" else args.cpu_rows
+            repeats = 1
+        log(f"[bench] {args.config}: {x.shape[0]} x {x.shape[1]} ready in {time.time() - t0:.1f}s")
+        out = run_single_config(args, args.config, x, workload, cpu_rows, repeats)
+        out["config"].update({"clusters": 8 if args.config == "c1" else 20,
+                              "duplicate_fraction": 0.0076 if args.config == "c1" else 0.005})
+    print(json.dumps(out), flush=True)
     return 0
 
 
