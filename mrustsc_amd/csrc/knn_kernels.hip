@@ -34,18 +34,6 @@ void launch_fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t stre
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, count, value);
 }
 
-// Dataset owning flat index `row`, given each dataset's first index (ascending).
-__device__ __forceinline__ uint32_t find_set(const DatasetDesc* sets, uint32_t count,
-                                             uint32_t row, bool prep) {
-    uint32_t lo = 0, hi = count - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        const uint32_t begin = prep ? sets[mid].prep_begin : sets[mid].row_begin;
-        if (begin <= row) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
 // ===========================================================================
 // 1. Column means (crates/scrust-core/src/neighbors.rs:144-154: f64 means).
 //    The sum is exact and rounded once; see exact_sum_add.

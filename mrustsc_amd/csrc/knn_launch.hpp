@@ -38,10 +38,11 @@ struct DatasetDesc {
     float* norms;                 // f32 fma squared norms, n
     double* mean;                 // column means, d
     unsigned long long* acc;      // exact column sums, d x kAccWordsHost
-    uint16_t* pack_ref;           // bf16 reference operands, (n + 1) x 32 nch
+    uint16_t* pack_ref;           // bf16 reference operands, ref_rows x 32 nch
     uint16_t* pack_query;         // bf16 query operands, n x 32 nch
-    float* pack_f32;              // fp32 operands, (n + 1) x 4 group
-    uint64_t* buf;                // pass candidate buffers, npad x cap
+    float* pack_f32;              // fp32 operands, ref_rows x 4 group
+    uint64_t* buf;                // pass candidate buffers, splits x npad x cap
+    uint32_t* seg_counts;         // keys per buffer segment (4 x u8), splits x npad
     uint64_t* cand;               // kept candidates, n x kp
     float* theta;                 // kp-th candidate score per row, n
     unsigned* max_norm_bits;      // max squared norm (as bits)
@@ -49,17 +50,26 @@ struct DatasetDesc {
     float* out_distances;         // n x k
     uint32_t n, d, k, kp, nch, group, cap;
     uint32_t row_begin;           // first flat row of this dataset
-    uint32_t prep_begin;          // first flat prepare row (n + 1 per dataset)
+    uint32_t prep_begin;          // first flat prepare row (ref_rows per pass dataset)
     uint32_t on_pass;             // 1: candidate pass + rerank; 0: exact rows only
+    uint32_t npad;                // n rounded up to whole query blocks
+    uint32_t splits;              // buffer slots per row (reference ranges of one block)
+    uint32_t chunk_pairs;         // tile pairs per reference range (the last may be short)
+    uint32_t select_e;            // keys per lane of the final selection (splits x cap / 64)
     // Error model of the pass (see the certificate in rerank_kernel).
     double split_rel;  // operand split error of a.b, relative to |a||b|
     double acc_rel;    // MFMA accumulation error, relative to the sum of |terms|
     double norm_rel;   // error of the |b|^2 feature, relative to |b|^2
 };
 
+// One pass work item: a query block against one reference range of its dataset.
 struct PassItem {
-    uint32_t set;    // dataset index
-    uint32_t block;  // query block of kPassRows rows
+    uint32_t set;         // dataset index
+    uint32_t block;       // query block of kPassRows rows
+    uint32_t split;       // buffer slot of the range (0 .. splits - 1)
+    uint32_t tile_begin;  // first 64-row reference tile (even)
+    uint32_t tile_end;    // one past the last tile (even)
+    uint32_t pad;
 };
 
 struct BatchShape {
@@ -73,17 +83,34 @@ struct BatchShape {
     int operand;
 };
 
+// Dataset owning flat index `row`, given each dataset's first index (ascending).
+__device__ __forceinline__ uint32_t find_set(const DatasetDesc* sets, uint32_t count,
+                                             uint32_t row, bool prep) {
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        const uint32_t begin = prep ? sets[mid].prep_begin : sets[mid].row_begin;
+        if (begin <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 void launch_fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t stream);
 // Column means, coordinates, norms and operands of every dataset.
 void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsigned* flags,
                         hipStream_t stream);
-// Candidate pass of one dataset (compile-time chunk count).
+// Candidate pass of one dataset (compile-time chunk count): one workgroup per query block
+// and reference range (splits ranges of chunk_pairs tile pairs).
 bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,
                  hipStream_t stream);
 // Candidate pass of every item of a batch in one persistent launch.
 bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t n_items,
                        uint32_t* item_counter, int operand, int nch_max, unsigned blocks,
                        hipStream_t stream);
+// Final candidate selection of every pass row over its reference ranges (one launch per
+// selection width present; `widths` is a bit set of the select_e values 2, 4, 8, 16).
+void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned widths,
+                   hipStream_t stream);
 // Rerank of every pass row (one launch per candidate width present).
 void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp32,
                    bool has_kp64, uint32_t* fail_list, uint32_t* fail_count,

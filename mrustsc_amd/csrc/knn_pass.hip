@@ -134,53 +134,102 @@ __device__ float select_keys(const uint64_t (&key)[E], int keep, int lane,
     return unordered(v);
 }
 
-// Compaction (quarter > 0, out == rowbuf) or final selection (quarter == 0) of one row.
-template <int E>
-__device__ __forceinline__ float select_row(const SCRUST_GLOBAL uint64_t* rowbuf,
-                                            const uint32_t (&c)[4], int keep, int lane,
-                                            SCRUST_GLOBAL uint64_t* out, uint32_t quarter) {
-    uint64_t key[E];
-    gather_row<E>(rowbuf, c, lane, key);
-    return select_keys<E>(key, keep, lane, out, quarter, E * 16);
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off, kWave));
+        v = o < v ? o : v;
+    }
+    return v;
 }
 
-__device__ __forceinline__ float select_row_by_cap(uint32_t cap, const SCRUST_GLOBAL uint64_t* rowbuf,
-                                                   const uint32_t (&c)[4], int keep, int lane,
-                                                   SCRUST_GLOBAL uint64_t* out, uint32_t quarter) {
-    if (cap <= 128) return select_row<2>(rowbuf, c, keep, lane, out, quarter);
-    if (cap <= 256) return select_row<4>(rowbuf, c, keep, lane, out, quarter);
-    return select_row<8>(rowbuf, c, keep, lane, out, quarter);
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off, kWave));
+        v = o > v ? o : v;
+    }
+    return v;
 }
 
-// A row with fewer than `keep` valid keys: with quarter == 0 (final output, n - 1 < keep)
-// its keys then kNoKey to out[0..keep); with quarter > 0 its keys spread back over the
-// four segments through kept_slot, none discarded.
+// In-scan compaction of one row (E * 64 == cap). With at least `keep` keys it keeps every
+// key whose score is at most a cut-off v with keep <= count(score <= v) <= keep + keep/2,
+// found by a binary search over the score range the keys span (a handful of steps; the
+// exact KP-th score is only needed by the final selection); ties that defeat the window
+// fall back to the exact selection. Keys are written back spread evenly over the four
+// segments. Returns the number kept and sets theta (the new append bound, or +inf when
+// the row holds fewer than `keep` keys and is only spread out).
 template <int E>
-__device__ __forceinline__ void copy_row(const SCRUST_GLOBAL uint64_t* rowbuf, const uint32_t (&c)[4],
-                                         int keep, int lane, SCRUST_GLOBAL uint64_t* out,
-                                         uint32_t quarter) {
+__device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint32_t (&c)[4], int keep,
+                                int lane, float& theta) {
+    constexpr uint32_t kSeg = E * 16;
     uint64_t key[E];
     gather_row<E>(rowbuf, c, lane, key);
-    // Every ballot before any store: all keys are in registers before out (possibly the
-    // same buffer) is written.
+    const uint32_t total = c[0] + c[1] + c[2] + c[3];
+    uint32_t v = 0xFFFFFFFFu;
+    uint32_t kept = total;
+    theta = kInf;
+    if (total >= static_cast<uint32_t>(keep)) {
+        uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            if (key[i] != kNoKey) {
+                const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
+                lo = sc < lo ? sc : lo;
+                hi = sc > hi ? sc : hi;
+            }
+        }
+        lo = wave_min_u32(lo);
+        hi = wave_max_u32(hi);
+        const uint32_t most = static_cast<uint32_t>(keep + keep / 2);
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            uint32_t n_le = 0;
+#pragma unroll
+            for (int i = 0; i < E; i++)
+                n_le += static_cast<uint32_t>(__popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) <= mid)));
+            if (n_le < static_cast<uint32_t>(keep)) {
+                lo = mid + 1;
+            } else if (n_le > most) {
+                hi = mid;
+            } else {
+                lo = hi = mid;
+            }
+        }
+        v = lo;
+        kept = 0;
+#pragma unroll
+        for (int i = 0; i < E; i++)
+            kept += static_cast<uint32_t>(__popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) <= v)));
+        if (kept > most) {
+            // Ties on the cut-off score: exactly `keep` keys by (score, id).
+            theta = select_keys<E>(key, keep, lane, rowbuf, static_cast<uint32_t>(keep) / 4, kSeg);
+            return static_cast<uint32_t>(keep);
+        }
+        theta = unordered(v);
+    }
+    // Keys with score <= v (all valid keys when spreading), quarter per segment.
+    const uint32_t quarter = (kept + 3) / 4;
     uint64_t m[E];
 #pragma unroll
-    for (int i = 0; i < E; i++) m[i] = __ballot(key[i] != kNoKey);
+    for (int i = 0; i < E; i++)
+        m[i] = __ballot(key[i] != kNoKey && static_cast<uint32_t>(key[i] >> 32) <= v);
     uint32_t base = 0;
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        if (key[i] != kNoKey) out[kept_slot(base + lanes_below(m[i]), quarter, E * 16)] = key[i];
+        if (key[i] != kNoKey && static_cast<uint32_t>(key[i] >> 32) <= v)
+            rowbuf[kept_slot(base + lanes_below(m[i]), quarter, kSeg)] = key[i];
         base += static_cast<uint32_t>(__popcll(m[i]));
     }
-    if (quarter == 0 && static_cast<uint32_t>(lane) >= base && lane < keep) out[lane] = kNoKey;
+    return kept;
 }
 
-__device__ __forceinline__ void copy_row_by_cap(uint32_t cap, const SCRUST_GLOBAL uint64_t* rowbuf,
-                                                const uint32_t (&c)[4], int keep, int lane,
-                                                SCRUST_GLOBAL uint64_t* out, uint32_t quarter) {
-    if (cap <= 128) copy_row<2>(rowbuf, c, keep, lane, out, quarter);
-    else if (cap <= 256) copy_row<4>(rowbuf, c, keep, lane, out, quarter);
-    else copy_row<8>(rowbuf, c, keep, lane, out, quarter);
+__device__ __forceinline__ uint32_t compact_row_by_cap(uint32_t cap, SCRUST_GLOBAL uint64_t* rowbuf,
+                                                       const uint32_t (&c)[4], int keep, int lane,
+                                                       float& theta) {
+    if (cap <= 128) return compact_row<2>(rowbuf, c, keep, lane, theta);
+    if (cap <= 256) return compact_row<4>(rowbuf, c, keep, lane, theta);
+    return compact_row<8>(rowbuf, c, keep, lane, theta);
 }
 
 template <int OP, int NCH>
@@ -208,10 +257,11 @@ struct PassSmem {
     u32x4 tile[2][kTileRefs * (Operand<OP, NCH>::kRowU4Max + 1)];
 };
 
-// One work item: query rows [block * 256, block * 256 + 256) of dataset s against all of
-// its references. FIXED: the dataset's chunk count equals NCH at compile time.
+// One work item: query rows [block * 256, block * 256 + 256) of dataset s against the
+// references of tiles [tile_begin, tile_end), into buffer slot `split`. FIXED: the
+// dataset's chunk count equals NCH at compile time.
 template <int OP, int NCH, bool FIXED>
-__device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
+__device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem& item,
                                            PassSmem<OP, NCH>& sm) {
     typedef Operand<OP, NCH> Op;
     const int nch = FIXED ? NCH : static_cast<int>(s.nch);
@@ -223,10 +273,12 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
     const uint32_t n = s.n;
     const uint32_t cap = s.cap;
     const int kp = static_cast<int>(s.kp);
-    const uint32_t qbase = block * kPassRows;
+    const uint32_t qbase = item.block * kPassRows;
     const SCRUST_GLOBAL u32x4* ref_rows = global_ptr(reinterpret_cast<const u32x4*>(
         OP == kOperandBf16 ? static_cast<const void*>(s.pack_ref) : static_cast<const void*>(s.pack_f32)));
-    SCRUST_GLOBAL uint64_t* const buf = global_ptr(s.buf);
+    // This range's slot of the row buffers.
+    SCRUST_GLOBAL uint64_t* const buf =
+        global_ptr(s.buf) + static_cast<size_t>(item.split) * s.npad * s.cap;
 
     // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
     // qb's fragment in lane l carries row qb*16 + l%16.
@@ -257,15 +309,14 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
         cnt[qb] = 0;
     }
     const uint32_t seg = cap / 4;
-    const uint32_t quarter = static_cast<uint32_t>(kp) / 4;
     // A tile appends at most 16 keys to a segment (4 column blocks x 4 references).
     const uint32_t trigger = seg - 16;
     SCRUST_GLOBAL uint64_t* const lane_seg =
         buf + static_cast<size_t>(qbase + w * kWave + lc) * cap + g * seg;
 
     // The reference operand holds whole pairs of tiles (rows n.. are sentinels), so a
-    // tile is one contiguous run of tile_u4 16-byte chunks and the scan has an even count.
-    const uint32_t n_even = (n + 2 * kTileRefs - 1) / (2 * kTileRefs) * 2;
+    // tile is one contiguous run of tile_u4 16-byte chunks; ranges hold whole pairs.
+    const uint32_t t_begin = item.tile_begin, t_end = item.tile_end;
     const uint32_t tile_u4 = kTileRefs * row_u4;
     // Two register sets stage tiles t+1 and t+2 while tile t is consumed from LDS, so a
     // tile's loads are issued two tiles before they are stored (HBM/L3 latency exceeds
@@ -379,10 +430,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
         filter(acc0, t, 2);
         filter(acc1, t, 3);
     };
-    // Compacts every row of this wave with a segment that could overflow in the next tile:
-    // to its KP smallest keys (quarter per segment), theta becoming the KP-th score; a row
-    // that holds fewer than KP keys (a skewed segment, small caps) is only spread evenly
-    // over its segments, its theta unchanged.
+    // Compacts every row of this wave with a segment that could overflow in the next tile
+    // (compact_row): the append bound tightens to the kept keys' cut-off score.
     auto compact = [&]() __attribute__((always_inline)) {
         uint64_t full = 0;
         static_for<0, 4>([&](auto QB) {
@@ -395,37 +444,31 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
             uint32_t c[4];
             row_counts(q, c);
             SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(qbase + w * kWave + q) * cap;
-            const uint32_t total = c[0] + c[1] + c[2] + c[3];
+            float theta;
+            const uint32_t kept = compact_row_by_cap(cap, rowbuf, c, kp, lane, theta);
+            // Segment g now holds min(quarter, kept - g * quarter) keys.
+            const uint32_t share = (kept + 3) / 4;
+            const uint32_t start = static_cast<uint32_t>(g) * share;
+            const uint32_t mine = kept > start ? (kept - start < share ? kept - start : share) : 0u;
             const int qb = q >> 4, lr = q & 15;
-            if (total >= static_cast<uint32_t>(kp)) {
-                const float next_phi = -0.5f * select_row_by_cap(cap, rowbuf, c, kp, lane, rowbuf, quarter);
-                static_for<0, 4>([&](auto QB) {
-                    if (qb == QB && lc == lr) {
-                        cnt[QB] = quarter;
-                        phi[QB] = next_phi;
-                    }
-                });
-            } else {
-                copy_row_by_cap(cap, rowbuf, c, kp, lane, rowbuf, quarter);
-                // Segment g now holds min(quarter, total - g * quarter) keys.
-                const uint32_t start = static_cast<uint32_t>(g) * quarter;
-                const uint32_t mine = total > start ? (total - start < quarter ? total - start : quarter) : 0u;
-                static_for<0, 4>([&](auto QB) {
-                    if (qb == QB && lc == lr) cnt[QB] = mine;
-                });
-            }
+            static_for<0, 4>([&](auto QB) {
+                if (qb == QB && lc == lr) {
+                    cnt[QB] = mine;
+                    if (theta < kInf) phi[QB] = -0.5f * theta;
+                }
+            });
         }
     };
 
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
     // merge into the loop head, where the compiler must then assume the prefetch just
     // issued is outstanding and waits for it. Loads past the end reload the last tile.
-    const uint32_t last_load = n_even - 1;
-    SCRUST_LOAD_TILE(pa, 0u)
-    SCRUST_LOAD_TILE(pb, 1u)
+    const uint32_t last_load = t_end - 1;
+    SCRUST_LOAD_TILE(pa, t_begin)
+    SCRUST_LOAD_TILE(pb, t_begin + 1)
     SCRUST_STORE_TILE(pa, 0)
     __syncthreads();
-    for (uint32_t t = 0; t < n_even; t += 2) {
+    for (uint32_t t = t_begin; t < t_end; t += 2) {
         SCRUST_LOAD_TILE(pa, t + 2 < last_load ? t + 2 : last_load)
         consume(t, 0);
         SCRUST_STORE_TILE(pb, 1)
@@ -439,22 +482,13 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, uint32_t block,
     }
 #undef SCRUST_LOAD_TILE
 #undef SCRUST_STORE_TILE
-    // Final selection: the KP smallest keys of each row (unordered) and the KP-th score.
-    for (int q = 0; q < kWave; q++) {
-        const uint32_t row = qbase + w * kWave + q;
-        if (row >= n) break;
-        uint32_t c[4];
-        row_counts(q, c);
-        const SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(row) * cap;
-        SCRUST_GLOBAL uint64_t* out = global_ptr(s.cand) + static_cast<size_t>(row) * kp;
-        if (c[0] + c[1] + c[2] + c[3] >= static_cast<uint32_t>(kp)) {
-            const float theta = select_row_by_cap(cap, rowbuf, c, kp, lane, out, 0u);
-            if (lane == 0) global_ptr(s.theta)[row] = theta;
-        } else {
-            copy_row_by_cap(cap, rowbuf, c, kp, lane, out, 0u);
-            if (lane == 0) global_ptr(s.theta)[row] = kInf;
-        }
-    }
+    // Segment counts for the final selection (select_kernel): byte g of the row's word.
+    SCRUST_GLOBAL uint8_t* const counts = reinterpret_cast<SCRUST_GLOBAL uint8_t*>(
+        global_ptr(s.seg_counts) + static_cast<size_t>(item.split) * s.npad);
+#pragma unroll
+    for (int qb = 0; qb < 4; qb++)
+        counts[static_cast<size_t>(qbase + w * kWave + qb * 16 + lc) * 4 + g] =
+            static_cast<uint8_t>(cnt[qb]);
 }
 
 // At most two waves per SIMD: the allocator then keeps the tile prefetch in registers
@@ -463,11 +497,24 @@ template <int OP, int NCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
 void pass_kernel(const DatasetDesc* __restrict__ set) {
     __shared__ PassSmem<OP, NCH> sm;
-    pass_block<OP, NCH, true>(*set, blockIdx.x, sm);
+    const DatasetDesc& s = *set;
+    // Workgroup g scans range j = g / blocks of query block g % blocks: the workgroups
+    // resident together scan the same references (shared through L2).
+    const uint32_t pairs = (s.n + 2 * kTileRefs - 1) / (2 * kTileRefs);
+    const uint32_t blocks = s.npad / kPassRows;
+    PassItem item;
+    item.set = 0;
+    item.split = blockIdx.x / blocks;
+    item.block = blockIdx.x - item.split * blocks;
+    item.tile_begin = 2 * item.split * s.chunk_pairs;
+    const uint32_t stop = (item.split + 1) * s.chunk_pairs;
+    item.tile_end = 2 * (stop < pairs ? stop : pairs);
+    item.pad = 0;
+    pass_block<OP, NCH, true>(s, item, sm);
 }
 
-// Persistent grid over a queue of (dataset, query block) items, largest first. Every
-// workgroup leaves once the queue is drained.
+// Persistent grid over a queue of (dataset, query block, reference range) items, largest
+// first. Every workgroup leaves once the queue is drained.
 template <int OP, int NCH_MAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
 void pass_batch_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __restrict__ items,
@@ -481,9 +528,78 @@ void pass_batch_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __r
         __syncthreads();
         if (item >= n_items) break;
         const PassItem it = items[item];
-        pass_block<OP, NCH_MAX, false>(sets[it.set], it.block, sm);
+        pass_block<OP, NCH_MAX, false>(sets[it.set], it, sm);
         __syncthreads();
     }
+}
+
+// Final candidate selection, one wave per query row: the KP smallest keys (unordered) of
+// the row's buffers over every reference range, and the KP-th score. Non-candidates of a
+// range scored at least that range's final theta, which is at least the KP-th smallest
+// key of the union, so the union's KP-th score bounds every reference left out.
+template <int E>
+__global__ __launch_bounds__(256) void select_kernel(const DatasetDesc* __restrict__ sets,
+                                                     uint32_t count, uint32_t total_rows) {
+    const uint32_t flat = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (flat >= total_rows) return;
+    const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+    if (!s.on_pass || s.select_e != E) return;
+    const uint32_t row = flat - s.row_begin;
+    const uint32_t cap = s.cap, seg = cap / 4, npad = s.npad, splits = s.splits;
+    const int kp = static_cast<int>(s.kp);
+    const SCRUST_GLOBAL uint64_t* buf = global_ptr(s.buf);
+    const SCRUST_GLOBAL uint32_t* counts = global_ptr(s.seg_counts);
+    // Key e = i*64 + lane is slot e % cap of range e / cap (uniform in i: cap >= 64).
+    uint64_t key[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const uint32_t e = static_cast<uint32_t>(i * kWave + lane);
+        const uint32_t j = e / cap, p = e % cap;
+        uint64_t k = kNoKey;
+        if (j < splits) {
+            const size_t at = static_cast<size_t>(j) * npad + row;
+            const uint32_t c = (counts[at] >> (8 * (p / seg))) & 0xFFu;
+            if (p % seg < c) k = buf[at * cap + p];
+        }
+        key[i] = k;
+    }
+    uint64_t m[E];
+    uint32_t total = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        m[i] = __ballot(key[i] != kNoKey);
+        total += static_cast<uint32_t>(__popcll(m[i]));
+    }
+    SCRUST_GLOBAL uint64_t* out = global_ptr(s.cand) + static_cast<size_t>(row) * kp;
+    if (total >= static_cast<uint32_t>(kp)) {
+        const float theta = select_keys<E>(key, kp, lane, out, 0u, 0u);
+        if (lane == 0) global_ptr(s.theta)[row] = theta;
+    } else {
+        // Fewer keys than KP (n - 1 < KP): all of them, then kNoKey.
+        uint32_t base = 0;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            if (key[i] != kNoKey) out[base + lanes_below(m[i])] = key[i];
+            base += static_cast<uint32_t>(__popcll(m[i]));
+        }
+        if (static_cast<uint32_t>(lane) >= base && lane < kp) out[lane] = kNoKey;
+        if (lane == 0) global_ptr(s.theta)[row] = kInf;
+    }
+}
+
+void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned widths,
+                   hipStream_t stream) {
+    const unsigned blocks = (shape.total_rows + 3) / 4;
+    if (blocks == 0) return;
+    if (widths & 2u)
+        hipLaunchKernelGGL(select_kernel<2>, dim3(blocks), dim3(256), 0, stream, sets, shape.count, shape.total_rows);
+    if (widths & 4u)
+        hipLaunchKernelGGL(select_kernel<4>, dim3(blocks), dim3(256), 0, stream, sets, shape.count, shape.total_rows);
+    if (widths & 8u)
+        hipLaunchKernelGGL(select_kernel<8>, dim3(blocks), dim3(256), 0, stream, sets, shape.count, shape.total_rows);
+    if (widths & 16u)
+        hipLaunchKernelGGL(select_kernel<16>, dim3(blocks), dim3(256), 0, stream, sets, shape.count, shape.total_rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,14 +635,14 @@ int batch_chunks(int operand, int nch) {
 }
 
 template <int OP, int NCH>
-static void run_pass(const DatasetDesc* set, uint32_t n, hipStream_t stream) {
-    const unsigned blocks = (n + kPassRows - 1) / kPassRows;
-    hipLaunchKernelGGL((pass_kernel<OP, NCH>), dim3(blocks), dim3(256), 0, stream, set);
+static void run_pass(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream) {
+    const unsigned groups = host.npad / kPassRows * host.splits;
+    hipLaunchKernelGGL((pass_kernel<OP, NCH>), dim3(groups), dim3(256), 0, stream, set);
 }
 
-#define SCRUST_PASS_CASE(OP, N)               \
-    case N:                                   \
-        run_pass<OP, N>(set, host.n, stream); \
+#define SCRUST_PASS_CASE(OP, N)          \
+    case N:                              \
+        run_pass<OP, N>(set, host, stream); \
         return true;
 
 bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,

@@ -25,11 +25,58 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 constexpr int kColsumRowsHost = 2048;  // rows per colsum block (knn_kernels.hip)
+constexpr uint32_t kMinPiecePairs = 4;   // shortest reference range of a work item (tile pairs)
+constexpr uint32_t kSelectKeysMax = 1024;  // keys per row the final selection takes (16 per lane)
+
+uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Reference ranges per query block for one launch of `blocks` blocks on `slots`
+// workgroup slots: the count minimising rounds x (range length + fixed cost), where an
+// item's fixed cost (query fragments, the first compactions) is worth `overhead` tile
+// pairs, taking more ranges only for a predicted gain of 15% (the model leaves out the
+// cache traffic of shorter ranges). Ranges are whole pairs, at least kMinPiecePairs long.
+uint32_t choose_splits(uint32_t blocks, uint32_t pairs, uint64_t slots, uint32_t max_splits,
+                       double overhead) {
+    uint32_t best = 1;
+    double one = 0.0, best_time = 0.0;
+    for (uint32_t sp = 1; sp <= max_splits; sp++) {
+        const uint32_t len = (pairs + sp - 1) / sp;
+        if (sp > 1 && len < kMinPiecePairs) break;
+        const uint32_t used = (pairs + len - 1) / len;
+        if (used != sp) continue;
+        const uint64_t rounds = (static_cast<uint64_t>(blocks) * sp + slots - 1) / slots;
+        const double time = static_cast<double>(rounds) * (len + overhead);
+        if (sp == 1) {
+            one = best_time = time;
+        } else if (time < one * 0.85 && time < best_time) {
+            best = sp;
+            best_time = time;
+        }
+    }
+    return best;
+}
+
+// Every block's scan cut into `pieces` near-equal ranges (batched launches).
+uint32_t partition_even(uint32_t set, uint32_t blocks, uint32_t pairs, uint32_t pieces,
+                        std::vector<PassItem>& items) {
+    pieces = std::max(1u, std::min(pieces, pairs));
+    const uint32_t len = (pairs + pieces - 1) / pieces;
+    pieces = (pairs + len - 1) / len;
+    for (uint32_t b = 0; b < blocks; b++)
+        for (uint32_t j = 0; j < pieces; j++)
+            items.push_back(PassItem{set, b, j, 2 * j * len, 2 * std::min(pairs, (j + 1) * len), 0});
+    return pieces;
+}
 
 }  // namespace
 
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
-                              const scrust_hip_knn_options* options, Plan& plan) {
+                              const scrust_hip_knn_options* options, Plan& plan,
+                              int compute_units) {
     plan = Plan{};
     const int metric = options ? static_cast<int>(options->metric) : kMetricEuclidean;
     const int precision = options ? static_cast<int>(options->precision) : SCRUST_HIP_BF16;
@@ -45,7 +92,16 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
     plan.shape.metric = metric;
     plan.shape.operand = plan.operand;
 
+    // Workgroup slots of the pass (two per CU) and, for batches, the longest reference
+    // range of one item: about a third of a slot's share of the whole batch.
+    const uint64_t slots = 2ull * static_cast<uint64_t>(compute_units > 0 ? compute_units : 256);
+    uint64_t batch_pairs = 0;
+    for (const SetSpec& sp : specs)
+        batch_pairs += ((sp.n + kPassRows - 1) / kPassRows) * ((sp.n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost));
+    const uint64_t batch_len = std::max<uint64_t>(kMinPiecePairs, (batch_pairs + 3 * slots - 1) / (3 * slots));
+
     size_t arena = 0;
+    size_t count_words = 0;  // segment counts of every dataset, one region (one clear)
     uint32_t row = 0, prep = 0;
     for (const SetSpec& sp : specs) {
         DatasetDesc s{};
@@ -110,6 +166,29 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         const size_t ref_rows = (sp.n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
         if (pass) {
             const size_t npad = (sp.n + kPassRows - 1) / kPassRows * kPassRows;
+            const uint32_t blocks = static_cast<uint32_t>(npad / kPassRows);
+            const uint32_t pairs = static_cast<uint32_t>(ref_rows / (2 * kTileRefsHost));
+            const uint32_t set = static_cast<uint32_t>(plan.sets.size());
+            // Reference ranges per block: a row's ranges feed one final selection of at
+            // most kSelectKeysMax keys, so splits x cap <= 1024.
+            const uint32_t max_splits = kSelectKeysMax / s.cap;
+            uint32_t splits;
+            // An item's fixed cost is about 64k cycles; a tile pair costs 2 x 4 x 4 x nch
+            // MFMAs of 16 (bf16) or 32 (fp32) cycles.
+            const double pair_cycles = 32.0 * s.nch * (plan.operand == kOperandBf16 ? 16.0 : 32.0);
+            const double overhead = 65536.0 / pair_cycles;
+            if (single) {
+                splits = choose_splits(blocks, pairs, slots, max_splits, overhead);
+                s.chunk_pairs = (pairs + splits - 1) / splits;
+            } else {
+                const uint32_t pieces = static_cast<uint32_t>(std::min<uint64_t>(max_splits, (pairs + batch_len - 1) / batch_len));
+                splits = partition_even(set, blocks, pairs, pieces, plan.items);
+                s.chunk_pairs = (pairs + splits - 1) / splits;
+            }
+            s.npad = static_cast<uint32_t>(npad);
+            s.splits = splits;
+            s.select_e = next_pow2(std::max<uint32_t>(2, (splits * s.cap + 63) / 64));
+            plan.select_widths |= s.select_e;
             if (plan.operand == kOperandBf16) {
                 const size_t rowb = 64ull * s.nch;
                 s.pack_ref = static_cast<uint16_t*>(take(ref_rows * rowb));
@@ -117,10 +196,11 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             } else {
                 s.pack_f32 = static_cast<float*>(take(ref_rows * 16ull * s.group));
             }
-            s.buf = static_cast<uint64_t*>(take(npad * s.cap * sizeof(uint64_t)));
+            s.buf = static_cast<uint64_t*>(take(static_cast<size_t>(splits) * npad * s.cap * sizeof(uint64_t)));
+            s.seg_counts = reinterpret_cast<uint32_t*>(count_words * sizeof(uint32_t));  // region offset
+            count_words += static_cast<size_t>(splits) * npad;
             s.cand = static_cast<uint64_t*>(take(sp.n * s.kp * sizeof(uint64_t)));
             s.theta = static_cast<float*>(take(sp.n * sizeof(float)));
-            for (uint32_t b = 0; b < npad / kPassRows; b++) plan.items.push_back(PassItem{static_cast<uint32_t>(plan.sets.size()), b});
         }
         plan.sets.push_back(s);
         plan.shape.max_row_chunks = std::max<uint32_t>(plan.shape.max_row_chunks, static_cast<uint32_t>((sp.n + kColsumRowsHost - 1) / kColsumRowsHost));
@@ -131,12 +211,19 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
     }
     plan.shape.total_rows = row;
     plan.shape.total_prep = prep;
+    // The segment counts of all datasets: one region, cleared before every pass.
+    plan.count_words = count_words;
+    plan.counts_offset = arena;
+    arena += align_up(count_words * sizeof(uint32_t) + 1);
+    for (DatasetDesc& s : plan.sets)
+        if (s.on_pass)
+            s.seg_counts = reinterpret_cast<uint32_t*>(plan.counts_offset + reinterpret_cast<size_t>(s.seg_counts));
     if (plan.any_pass && !single) {
         plan.nch_max = batch_chunks(plan.operand, plan.nch_max);
         if (plan.nch_max == 0) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no batched pass variant");
-        // Largest work first: an item costs about n (the references it scans).
+        // Largest work first: an item costs about its tile count (the references it scans).
         std::stable_sort(plan.items.begin(), plan.items.end(), [&](const PassItem& a, const PassItem& b) {
-            return plan.sets[a.set].n > plan.sets[b.set].n;
+            return a.tile_end - a.tile_begin > b.tile_end - b.tile_begin;
         });
     }
     if (!plan.any_pass) plan.operand = kOperandNone;
@@ -152,6 +239,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
 scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t stream,
                              void* pinned) {
     char* base = static_cast<char*>(arena);
+    plan.arena_base = arena;
     // Every field passed here holds an arena offset (possibly 0) from plan_layout.
     auto fix = [&](auto*& p) {
         using T = std::remove_reference_t<decltype(*p)>;
@@ -171,6 +259,7 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
                 fix(s.pack_f32);
             }
             fix(s.buf);
+            fix(s.seg_counts);
             fix(s.cand);
             fix(s.theta);
         }
@@ -217,6 +306,9 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     launch_fill_u32(plan.d_scalars, kScalarCount, 0u, stream);
     launch_fill_u32(plan.d_scalars + kScalarListCount, 1, static_cast<uint32_t>(plan.exact_rows.size()), stream);
     unsigned* flags = plan.d_scalars + kScalarFlags;
+    if (plan.count_words)
+        launch_fill_u32(reinterpret_cast<uint32_t*>(static_cast<char*>(plan.arena_base) + plan.counts_offset),
+                        plan.count_words, 0u, stream);
     launch_prepare_all(plan.d_sets, plan.shape, flags, stream);
     if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch prepare");
     if ((err = mark(1)) != hipSuccess) return hip_failure(err, "record event");
@@ -235,6 +327,8 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     }
     if ((err = mark(2)) != hipSuccess) return hip_failure(err, "record event");
     if (plan.any_pass) {
+        launch_select(plan.d_sets, plan.shape, plan.select_widths, stream);
+        if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch candidate selection");
         launch_rerank(plan.d_sets, plan.shape, plan.kp32, plan.kp64, plan.d_list,
                       plan.d_scalars + kScalarListCount, stream);
         if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch rerank");

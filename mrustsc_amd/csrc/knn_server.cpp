@@ -205,7 +205,7 @@ bool scrust_hip_server::start(Slot& slot, Request& r) {
     std::vector<SetSpec> specs{SetSpec{reinterpret_cast<const float*>(base + l.input), r.n, r.d, r.k,
                                        reinterpret_cast<uint32_t*>(base + l.indices),
                                        reinterpret_cast<float*>(base + l.distances)}};
-    scrust_hip_status st = plan_layout(specs, &r.options, r.plan);
+    scrust_hip_status st = plan_layout(specs, &r.options, r.plan, compute_units);
     if (st != SCRUST_HIP_OK) return failed(st, last_error());
     hipError_t err = hipMemcpyAsync(base + l.input, pin + l.input, in_bytes, hipMemcpyHostToDevice, slot.stream);
     if (err != hipSuccess) return failed(SCRUST_HIP_KERNEL, hipGetErrorString(err));
@@ -377,7 +377,7 @@ scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const floa
     // Size the device region now (the layout does not touch the pointers).
     std::vector<SetSpec> specs{SetSpec{embedding, n, d, k, out_indices, out_distances}};
     Plan sizing;
-    st = plan_layout(specs, options, sizing);
+    st = plan_layout(specs, options, sizing, server->compute_units);
     if (st != SCRUST_HIP_OK) return st;
     r->arena_bytes = sizing.arena_bytes;
     r->meta_bytes = sizing.meta_bytes;

@@ -132,7 +132,7 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
                              float* out_distances, scrust_hip_knn_stats* stats) {
     std::vector<SetSpec> specs{SetSpec{x, n, d, k, out_indices, out_distances}};
     Plan plan;
-    scrust_hip_status st = plan_layout(specs, options, plan);
+    scrust_hip_status st = plan_layout(specs, options, plan, ctx->compute_units);
     if (st != SCRUST_HIP_OK) return st;
     SCRUST_CHECK(ctx->arena.reserve(plan.arena_bytes), "allocate workspace");
     SCRUST_CHECK(ctx->meta.reserve(plan.meta_bytes), "allocate metadata");
@@ -308,7 +308,7 @@ scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
         delete batch;
         return st;
     };
-    scrust_hip_status st = plan_layout(specs, options, batch->plan);
+    scrust_hip_status st = plan_layout(specs, options, batch->plan, context->compute_units);
     if (st != SCRUST_HIP_OK) return bail(st);
     Plan& plan = batch->plan;
     hipError_t err = batch->arena.reserve(plan.arena_bytes);

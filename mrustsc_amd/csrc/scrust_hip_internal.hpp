@@ -41,6 +41,10 @@ struct Plan {
     int precision = SCRUST_HIP_BF16;
     int nch_max = 0;
     bool kp32 = false, kp64 = false;
+    unsigned select_widths = 0;  // select_e values present (bit set)
+    size_t count_words = 0;      // segment counts of all datasets (u32 words)
+    size_t counts_offset = 0;    // their arena offset
+    void* arena_base = nullptr;  // set by plan_place
     bool any_pass = false;
     uint64_t pass_flops = 0;
     size_t arena_bytes = 0;
@@ -68,9 +72,10 @@ const char* last_error();
 void clear_error();
 bool device_is_gfx950(int device, hipDeviceProp_t* props_out);
 
-// Lays out a plan for `specs` (no device work).
+// Lays out a plan for `specs` (no device work) for a device of `compute_units` CUs.
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
-                              const scrust_hip_knn_options* options, Plan& plan);
+                              const scrust_hip_knn_options* options, Plan& plan,
+                              int compute_units);
 // Points the plan's descriptors into `arena` and `meta` (sized arena_bytes / meta_bytes)
 // and uploads descriptors, items and the exact-row list on `stream`. With `pinned` (host
 // memory of at least meta_bytes that stays untouched until the copy completes) the upload
