@@ -516,10 +516,10 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
 // Persistent grid over a queue of (dataset, query block, reference range) items, largest
 // first. Every workgroup leaves once the queue is drained.
 template <int OP, int NCH_MAX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-void pass_batch_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __restrict__ items,
-                       uint32_t n_items, uint32_t* __restrict__ counter) {
-    __shared__ PassSmem<OP, NCH_MAX> sm;
+__device__ __forceinline__ void pass_batch_body(const DatasetDesc* __restrict__ sets,
+                                                const PassItem* __restrict__ items,
+                                                uint32_t n_items, uint32_t* __restrict__ counter,
+                                                PassSmem<OP, NCH_MAX>& sm) {
     __shared__ uint32_t next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
@@ -531,6 +531,24 @@ void pass_batch_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __r
         pass_block<OP, NCH_MAX, false>(sets[it.set], it, sm);
         __syncthreads();
     }
+}
+
+template <int OP, int NCH_MAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void pass_batch_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __restrict__ items,
+                       uint32_t n_items, uint32_t* __restrict__ counter) {
+    __shared__ PassSmem<OP, NCH_MAX> sm;
+    pass_batch_body<OP, NCH_MAX>(sets, items, n_items, counter, sm);
+}
+
+// The narrowest class fits two waves per SIMD without spilling once asked to: the runtime
+// chunk count costs it a few registers over the fixed-count kernel.
+template <int OP, int NCH_MAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void pass_batch_narrow_kernel(const DatasetDesc* __restrict__ sets, const PassItem* __restrict__ items,
+                              uint32_t n_items, uint32_t* __restrict__ counter) {
+    __shared__ PassSmem<OP, NCH_MAX> sm;
+    pass_batch_body<OP, NCH_MAX>(sets, items, n_items, counter, sm);
 }
 
 // Final candidate selection, one wave per query row: the KP smallest keys (unordered) of
@@ -671,8 +689,13 @@ bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,
 template <int OP, int NCH_MAX>
 static void run_batch(const DatasetDesc* sets, const PassItem* items, uint32_t n_items,
                       uint32_t* counter, unsigned blocks, hipStream_t stream) {
-    hipLaunchKernelGGL((pass_batch_kernel<OP, NCH_MAX>), dim3(blocks), dim3(256), 0, stream, sets,
-                       items, n_items, counter);
+    constexpr bool kNarrow = (OP == kOperandBf16 && NCH_MAX == 5) || (OP == kOperandF32 && NCH_MAX == 13);
+    if constexpr (kNarrow)
+        hipLaunchKernelGGL((pass_batch_narrow_kernel<OP, NCH_MAX>), dim3(blocks), dim3(256), 0, stream,
+                           sets, items, n_items, counter);
+    else
+        hipLaunchKernelGGL((pass_batch_kernel<OP, NCH_MAX>), dim3(blocks), dim3(256), 0, stream, sets,
+                           items, n_items, counter);
 }
 
 bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t n_items,

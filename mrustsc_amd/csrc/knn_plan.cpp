@@ -219,12 +219,25 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         if (s.on_pass)
             s.seg_counts = reinterpret_cast<uint32_t*>(plan.counts_offset + reinterpret_cast<size_t>(s.seg_counts));
     if (plan.any_pass && !single) {
-        plan.nch_max = batch_chunks(plan.operand, plan.nch_max);
-        if (plan.nch_max == 0) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no batched pass variant");
-        // Largest work first: an item costs about its tile count (the references it scans).
+        // One persistent launch per compiled chunk class (narrow rows keep the occupancy
+        // of their own variant), each over its items largest first: an item costs about
+        // its tile count (the references it scans).
+        auto cls = [&](const PassItem& it) { return batch_chunks(plan.operand, static_cast<int>(plan.sets[it.set].nch)); };
+        for (const PassItem& it : plan.items)
+            if (cls(it) == 0) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no batched pass variant");
         std::stable_sort(plan.items.begin(), plan.items.end(), [&](const PassItem& a, const PassItem& b) {
+            const int ca = cls(a), cb = cls(b);
+            if (ca != cb) return ca < cb;
             return a.tile_end - a.tile_begin > b.tile_end - b.tile_begin;
         });
+        for (size_t i = 0; i < plan.items.size(); i++) {
+            const int c = cls(plan.items[i]);
+            if (plan.groups.empty() || plan.groups.back().nch_max != c)
+                plan.groups.push_back(PassGroup{c, static_cast<uint32_t>(i), 0});
+            plan.groups.back().count++;
+        }
+        if (plan.groups.size() > static_cast<size_t>(kScalarCount - kScalarItemCounter))
+            return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: too many batched pass groups");
     }
     if (!plan.any_pass) plan.operand = kOperandNone;
     plan.shape.operand = plan.operand;
@@ -317,10 +330,14 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
         if (single) {
             ok = launch_pass(plan.d_sets, plan.sets[0], plan.operand, stream);
         } else {
-            const unsigned blocks = static_cast<unsigned>(std::min<size_t>(plan.items.size(), 2ull * compute_units));
-            ok = launch_pass_batch(plan.d_sets, plan.d_items, static_cast<uint32_t>(plan.items.size()),
-                                   plan.d_scalars + kScalarItemCounter, plan.operand, plan.nch_max,
-                                   blocks, stream);
+            ok = true;
+            for (size_t g = 0; g < plan.groups.size() && ok; g++) {
+                const PassGroup& pg = plan.groups[g];
+                const unsigned blocks = static_cast<unsigned>(std::min<size_t>(pg.count, 2ull * compute_units));
+                ok = launch_pass_batch(plan.d_sets, plan.d_items + pg.begin, pg.count,
+                                       plan.d_scalars + kScalarItemCounter + g, plan.operand,
+                                       pg.nch_max, blocks, stream);
+            }
         }
         if (!ok) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no compiled pass variant for this shape");
         if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch candidate pass");

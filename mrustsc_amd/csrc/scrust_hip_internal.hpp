@@ -31,10 +31,17 @@ struct SetSpec {
     float* out_distances;
 };
 
+// Items of one persistent batched pass launch (one compiled chunk class).
+struct PassGroup {
+    int nch_max;
+    uint32_t begin, count;
+};
+
 // Device placement and launch parameters of a set of embeddings.
 struct Plan {
     std::vector<DatasetDesc> sets;  // device pointers filled in
-    std::vector<PassItem> items;    // persistent-pass work items, largest first
+    std::vector<PassItem> items;    // persistent-pass work items, by class, largest first
+    std::vector<PassGroup> groups;  // batched pass launches
     std::vector<uint32_t> exact_rows;  // flat rows of datasets on the exact path
     BatchShape shape{};
     int operand = kOperandNone;
@@ -53,13 +60,13 @@ struct Plan {
     DatasetDesc* d_sets = nullptr;
     PassItem* d_items = nullptr;
     uint32_t* d_list = nullptr;     // exact rows: prefilled exact-path rows, then uncertified
-    uint32_t* d_scalars = nullptr;  // [0] flags, [1] list count, [2] item counter
+    uint32_t* d_scalars = nullptr;  // [0] flags, [1] list count, [2..] item counters
 };
 
 // Scalar slots of Plan::d_scalars.
 constexpr int kScalarFlags = 0;
 constexpr int kScalarListCount = 1;
-constexpr int kScalarItemCounter = 2;
+constexpr int kScalarItemCounter = 2;  // first of one counter per pass group
 constexpr int kScalarCount = 16;
 
 scrust_hip_status fail(scrust_hip_status status, const std::string& message);
