@@ -123,51 +123,73 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
             }
             s.norms[row] = nrm;
             if (op == kOperandBf16) {
-                const int kc = 32 * nch;
-                uint16_t* r = s.pack_ref + row * kc;
-                uint16_t* q = s.pack_query + row * kc;
-                for (int j = 0; j < d; j++) {
-                    const float c = dst[j];
-                    const uint16_t h = bf16_rn(c);
-                    const uint16_t l = bf16_rn(c - bf16_value(h));
-                    r[j] = h;
-                    r[d + j] = l;
-                    r[2 * d + j] = h;
-                    q[j] = h;
-                    q[d + j] = h;
-                    q[2 * d + j] = l;
-                }
+                // Features (see knn_pass.hip), eight per 16-byte store: reference
+                // [h | l | h | nh nl nl2 | 0], query [h | h | l | -1/2 x3 | 0].
                 const uint16_t nh = bf16_rn(nrm);
                 const float rest = nrm - bf16_value(nh);
                 const uint16_t nl = bf16_rn(rest);
                 const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
-                r[3 * d] = nh;
-                r[3 * d + 1] = nl;
-                r[3 * d + 2] = nl2;
-                q[3 * d] = kBf16MinusHalf;
-                q[3 * d + 1] = kBf16MinusHalf;
-                q[3 * d + 2] = kBf16MinusHalf;
-                for (int p = 3 * d + 3; p < kc; p++) {
-                    r[p] = 0;
-                    q[p] = 0;
+                uint4* r = reinterpret_cast<uint4*>(s.pack_ref + row * (32 * nch));
+                uint4* q = reinterpret_cast<uint4*>(s.pack_query + row * (32 * nch));
+                for (int chunk = 0; chunk < 4 * nch; chunk++) {
+                    uint32_t rw[4], qw[4];
+                    for (int e = 0; e < 8; e++) {
+                        const int f = chunk * 8 + e;
+                        uint16_t rv = 0, qv = 0;
+                        if (f < 3 * d) {
+                            const int part = f / d;
+                            const float c = dst[f - part * d];
+                            const uint16_t h = bf16_rn(c);
+                            const uint16_t l = bf16_rn(c - bf16_value(h));
+                            rv = part == 1 ? l : h;
+                            qv = part == 2 ? l : h;
+                        } else if (f < 3 * d + 3) {
+                            rv = f == 3 * d ? nh : (f == 3 * d + 1 ? nl : nl2);
+                            qv = kBf16MinusHalf;
+                        }
+                        if (e & 1) {
+                            rw[e >> 1] |= static_cast<uint32_t>(rv) << 16;
+                            qw[e >> 1] |= static_cast<uint32_t>(qv) << 16;
+                        } else {
+                            rw[e >> 1] = rv;
+                            qw[e >> 1] = qv;
+                        }
+                    }
+                    r[chunk] = make_uint4(rw[0], rw[1], rw[2], rw[3]);
+                    q[chunk] = make_uint4(qw[0], qw[1], qw[2], qw[3]);
                 }
             } else if (op == kOperandF32) {
-                float* p = s.pack_f32 + row * (4 * group);
-                for (int t = 0; t < 4 * group; t++) {
-                    const int g = t / group, c = t % group;
-                    const int f = g * nch + c;
-                    p[t] = c >= nch ? 0.0f : (f < d ? dst[f] : (f == d ? nrm : 0.0f));
+                float4* p4 = reinterpret_cast<float4*>(s.pack_f32 + row * (4 * group));
+                for (int t4 = 0; t4 < group; t4++) {
+                    float v[4];
+                    for (int e = 0; e < 4; e++) {
+                        const int t = t4 * 4 + e;
+                        const int g = t / group, c = t % group;
+                        const int f = g * nch + c;
+                        v[e] = c >= nch ? 0.0f : (f < d ? dst[f] : (f == d ? nrm : 0.0f));
+                    }
+                    p4[t4] = make_float4(v[0], v[1], v[2], v[3]);
                 }
             }
         } else if (op == kOperandBf16) {
-            const int kc = 32 * nch;
-            uint16_t* r = s.pack_ref + row * kc;
-            for (int p = 0; p < kc; p++) r[p] = (p == 3 * d) ? kBf16Inf : 0;
+            uint4* r = reinterpret_cast<uint4*>(s.pack_ref + row * (32 * nch));
+            const int inf_chunk = (3 * d) / 8, inf_word = ((3 * d) % 8) / 2;
+            const uint32_t inf_bits = static_cast<uint32_t>(kBf16Inf) << (16 * ((3 * d) & 1));
+            for (int chunk = 0; chunk < 4 * nch; chunk++) {
+                uint32_t w[4] = {0u, 0u, 0u, 0u};
+                if (chunk == inf_chunk) w[inf_word] = inf_bits;
+                r[chunk] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
         } else if (op == kOperandF32) {
-            float* p = s.pack_f32 + row * (4 * group);
-            for (int t = 0; t < 4 * group; t++) {
-                const int g = t / group, c = t % group;
-                p[t] = (c < nch && g * nch + c == d) ? kInf : 0.0f;
+            float4* p4 = reinterpret_cast<float4*>(s.pack_f32 + row * (4 * group));
+            for (int t4 = 0; t4 < group; t4++) {
+                float v[4];
+                for (int e = 0; e < 4; e++) {
+                    const int t = t4 * 4 + e;
+                    const int g = t / group, c = t % group;
+                    v[e] = (c < nch && g * nch + c == d) ? kInf : 0.0f;
+                }
+                p4[t4] = make_float4(v[0], v[1], v[2], v[3]);
             }
         }
     }
