@@ -39,23 +39,30 @@ void launch_fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t stre
 //    The sum is exact and rounded once; see exact_sum_add.
 // ===========================================================================
 
-constexpr int kColsumRows = 2048;
+constexpr int kColsumWaves = 4;
 
-__global__ __launch_bounds__(64) void colsum_kernel(const DatasetDesc* __restrict__ sets) {
-    __shared__ long long acc[kWave][kAccWords + 1];
+// A block sums colsum_rows(n) rows of 64 columns; its four waves take every fourth row into
+// their own LDS superaccumulators (a dependent chain of at most 512 adds per lane), which
+// are combined and added to the column's global sum.
+__global__ __launch_bounds__(256) void colsum_kernel(const DatasetDesc* __restrict__ sets) {
+    __shared__ long long acc[kColsumWaves][kWave][kAccWords + 1];
     const DatasetDesc& s = sets[blockIdx.z];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const uint32_t col = blockIdx.y * kWave + lane;
-    const size_t r0 = static_cast<size_t>(blockIdx.x) * kColsumRows;
+    const uint32_t rows = colsum_rows(s.n);
+    const size_t r0 = static_cast<size_t>(blockIdx.x) * rows;
     if (r0 >= s.n || blockIdx.y * kWave >= s.d) return;
-    for (int w = 0; w < kAccWords; w++) acc[lane][w] = 0;
-    size_t r1 = r0 + kColsumRows;
+    for (int k = 0; k < kAccWords; k++) acc[w][lane][k] = 0;
+    size_t r1 = r0 + rows;
     if (r1 > s.n) r1 = s.n;
-    if (col < s.d) {
-        for (size_t r = r0; r < r1; r++) exact_sum_add(acc[lane], s.x[r * s.d + col]);
-        for (int w = 0; w < kAccWords; w++) {
-            long long v = acc[lane][w];
-            if (v) atomicAdd(&s.acc[col * kAccWords + w], static_cast<unsigned long long>(v));
+    if (col < s.d)
+        for (size_t r = r0 + w; r < r1; r += kColsumWaves) exact_sum_add(acc[w][lane], s.x[r * s.d + col]);
+    __syncthreads();
+    if (w == 0 && col < s.d) {
+        for (int k = 0; k < kAccWords; k++) {
+            long long v = 0;
+            for (int ww = 0; ww < kColsumWaves; ww++) v += acc[ww][lane][k];
+            if (v) atomicAdd(&s.acc[col * kAccWords + k], static_cast<unsigned long long>(v));
         }
     }
 }
@@ -215,7 +222,7 @@ void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsign
     hipLaunchKernelGGL(clear_sums_kernel, dim3(shape.count), dim3(256), 0, stream, sets);
     if (shape.metric == kMetricEuclidean) {
         dim3 grid(shape.max_row_chunks, shape.max_col_groups, shape.count);
-        hipLaunchKernelGGL(colsum_kernel, grid, dim3(kWave), 0, stream, sets);
+        hipLaunchKernelGGL(colsum_kernel, grid, dim3(kColsumWaves * kWave), 0, stream, sets);
         hipLaunchKernelGGL(colmean_kernel, dim3(shape.max_col_groups, shape.count), dim3(kWave), 0,
                            stream, sets);
     }

@@ -76,12 +76,19 @@ struct BatchShape {
     uint32_t count;           // datasets
     uint32_t total_rows;      // sum of n
     uint32_t total_prep;      // sum of n + 1
-    uint32_t max_row_chunks;  // max ceil(n / colsum rows)
+    uint32_t max_row_chunks;  // max ceil(n / colsum_rows(n))
     uint32_t max_col_groups;  // max ceil(d / 64)
     uint32_t max_d;           // largest d
     int metric;
     int operand;
 };
+
+// Rows per column-sum block: about 512 blocks per dataset, 64 to 2048 rows each.
+__host__ __device__ inline uint32_t colsum_rows(uint32_t n) {
+    uint32_t rows = 64;
+    while (rows < 2048 && rows * 512u < n) rows <<= 1;
+    return rows;
+}
 
 // Dataset owning flat index `row`, given each dataset's first index (ascending).
 __device__ __forceinline__ uint32_t find_set(const DatasetDesc* sets, uint32_t count,

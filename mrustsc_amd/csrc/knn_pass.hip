@@ -152,7 +152,8 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// In-scan compaction of one row (E * 64 == cap). With at least `keep` keys it keeps every
+// In-scan compaction of one row (E * 64 == cap) whose `total` valid keys are in `key`
+// (gather_row). With at least `keep` keys it keeps every
 // key whose score is at most a cut-off v with keep <= count(score <= v) <= keep + keep/2,
 // found by a binary search over the score range the keys span (a handful of steps; the
 // exact KP-th score is only needed by the final selection); ties that defeat the window
@@ -160,12 +161,9 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // segments. Returns the number kept and sets theta (the new append bound, or +inf when
 // the row holds fewer than `keep` keys and is only spread out).
 template <int E>
-__device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint32_t (&c)[4], int keep,
-                                int lane, float& theta) {
+__device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint64_t (&key)[E],
+                                uint32_t total, int keep, int lane, float& theta) {
     constexpr uint32_t kSeg = E * 16;
-    uint64_t key[E];
-    gather_row<E>(rowbuf, c, lane, key);
-    const uint32_t total = c[0] + c[1] + c[2] + c[3];
     uint32_t v = 0xFFFFFFFFu;
     uint32_t kept = total;
     theta = kInf;
@@ -222,14 +220,6 @@ __device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint32_t (
         base += static_cast<uint32_t>(__popcll(m[i]));
     }
     return kept;
-}
-
-__device__ __forceinline__ uint32_t compact_row_by_cap(uint32_t cap, SCRUST_GLOBAL uint64_t* rowbuf,
-                                                       const uint32_t (&c)[4], int keep, int lane,
-                                                       float& theta) {
-    if (cap <= 128) return compact_row<2>(rowbuf, c, keep, lane, theta);
-    if (cap <= 256) return compact_row<4>(rowbuf, c, keep, lane, theta);
-    return compact_row<8>(rowbuf, c, keep, lane, theta);
 }
 
 template <int OP, int NCH>
@@ -432,21 +422,31 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     };
     // Compacts every row of this wave with a segment that could overflow in the next tile
     // (compact_row): the append bound tightens to the kept keys' cut-off score.
-    auto compact = [&]() __attribute__((always_inline)) {
-        uint64_t full = 0;
-        static_for<0, 4>([&](auto QB) {
-            const uint64_t f = __ballot(cnt[QB] > trigger);
-            full |= ((f | f >> 16 | f >> 32 | f >> 48) & 0xFFFFull) << (16 * QB);
-        });
-        while (full) {
-            const int q = __builtin_ctzll(full);
-            full &= full - 1;
-            uint32_t c[4];
-            row_counts(q, c);
-            SCRUST_GLOBAL uint64_t* rowbuf = buf + static_cast<size_t>(qbase + w * kWave + q) * cap;
+    // Rows are compacted one after another by the whole wave; the next row's keys are
+    // gathered before the current row is processed, so their load latency overlaps it.
+    auto compact_rows = [&](auto ec, uint64_t full) __attribute__((always_inline)) {
+        constexpr int E = decltype(ec)::value;
+        int q = __builtin_ctzll(full);
+        full &= full - 1;
+        uint32_t c[4];
+        row_counts(q, c);
+        uint64_t key[E];
+        gather_row<E>(buf + static_cast<size_t>(qbase + w * kWave + q) * cap, c, lane, key);
+        for (;;) {
+            const bool more = full != 0;
+            int qn = 0;
+            uint32_t cn[4] = {0u, 0u, 0u, 0u};
+            uint64_t keyn[E];
+            if (more) {
+                qn = __builtin_ctzll(full);
+                full &= full - 1;
+                row_counts(qn, cn);
+                gather_row<E>(buf + static_cast<size_t>(qbase + w * kWave + qn) * cap, cn, lane, keyn);
+            }
             float theta;
-            const uint32_t kept = compact_row_by_cap(cap, rowbuf, c, kp, lane, theta);
-            // Segment g now holds min(quarter, kept - g * quarter) keys.
+            const uint32_t kept = compact_row<E>(buf + static_cast<size_t>(qbase + w * kWave + q) * cap,
+                                                 key, c[0] + c[1] + c[2] + c[3], kp, lane, theta);
+            // Segment g now holds min(share, kept - g * share) keys.
             const uint32_t share = (kept + 3) / 4;
             const uint32_t start = static_cast<uint32_t>(g) * share;
             const uint32_t mine = kept > start ? (kept - start < share ? kept - start : share) : 0u;
@@ -457,7 +457,26 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
                     if (theta < kInf) phi[QB] = -0.5f * theta;
                 }
             });
+            if (!more) break;
+            q = qn;
+#pragma unroll
+            for (int i = 0; i < 4; i++) c[i] = cn[i];
+#pragma unroll
+            for (int i = 0; i < E; i++) key[i] = keyn[i];
         }
+    };
+    // Compacts every row of this wave with a segment that could overflow in the next tile
+    // (compact_row): the append bound tightens to the kept keys' cut-off score.
+    auto compact = [&]() __attribute__((always_inline)) {
+        uint64_t full = 0;
+        static_for<0, 4>([&](auto QB) {
+            const uint64_t f = __ballot(cnt[QB] > trigger);
+            full |= ((f | f >> 16 | f >> 32 | f >> 48) & 0xFFFFull) << (16 * QB);
+        });
+        if (!full) return;
+        if (cap <= 128) compact_rows(std::integral_constant<int, 2>{}, full);
+        else if (cap <= 256) compact_rows(std::integral_constant<int, 4>{}, full);
+        else compact_rows(std::integral_constant<int, 8>{}, full);
     };
 
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would

@@ -24,7 +24,6 @@ constexpr size_t kAlign = 256;
 
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
-constexpr int kColsumRowsHost = 2048;  // rows per colsum block (knn_kernels.hip)
 constexpr uint32_t kMinPiecePairs = 4;   // shortest reference range of a work item (tile pairs)
 constexpr uint32_t kSelectKeysMax = 1024;  // keys per row the final selection takes (16 per lane)
 
@@ -203,7 +202,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             s.theta = static_cast<float*>(take(sp.n * sizeof(float)));
         }
         plan.sets.push_back(s);
-        plan.shape.max_row_chunks = std::max<uint32_t>(plan.shape.max_row_chunks, static_cast<uint32_t>((sp.n + kColsumRowsHost - 1) / kColsumRowsHost));
+        plan.shape.max_row_chunks = std::max<uint32_t>(plan.shape.max_row_chunks, static_cast<uint32_t>((sp.n + colsum_rows(s.n) - 1) / colsum_rows(s.n)));
         plan.shape.max_col_groups = std::max<uint32_t>(plan.shape.max_col_groups, static_cast<uint32_t>((sp.d + 63) / 64));
         plan.shape.max_d = std::max<uint32_t>(plan.shape.max_d, static_cast<uint32_t>(sp.d));
         row += s.n;
