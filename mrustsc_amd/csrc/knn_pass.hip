@@ -134,22 +134,24 @@ __device__ float select_keys(const uint64_t (&key)[E], int keep, int lane,
     return unordered(v);
 }
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off, kWave));
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off, kWave));
-        v = o > v ? o : v;
-    }
-    return v;
+// Wave-wide min / max on the VALU (DPP): row shifts by 1, 2, 4 and 8 reduce each row of
+// 16 lanes into its lane 15, row broadcasts 15 and 31 carry the rows into lane 63. Lanes
+// without a source keep their own value.
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_extreme_u32(uint32_t v) {
+    auto pick = [](uint32_t a, uint32_t b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    auto step = [&](uint32_t x, auto ctrl, auto row_mask) {
+        const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+            static_cast<int>(x), static_cast<int>(x), decltype(ctrl)::value, decltype(row_mask)::value, 0xF, false));
+        return pick(x, t);
+    };
+    v = step(v, std::integral_constant<int, 0x111>{}, std::integral_constant<int, 0xF>{});  // row_shr:1
+    v = step(v, std::integral_constant<int, 0x112>{}, std::integral_constant<int, 0xF>{});  // row_shr:2
+    v = step(v, std::integral_constant<int, 0x114>{}, std::integral_constant<int, 0xF>{});  // row_shr:4
+    v = step(v, std::integral_constant<int, 0x118>{}, std::integral_constant<int, 0xF>{});  // row_shr:8
+    v = step(v, std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{});  // row_bcast:15
+    v = step(v, std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{});  // row_bcast:31
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
 // In-scan compaction of one row (E * 64 == cap) whose `total` valid keys are in `key`
@@ -177,8 +179,8 @@ __device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint64_t (
                 hi = sc > hi ? sc : hi;
             }
         }
-        lo = wave_min_u32(lo);
-        hi = wave_max_u32(hi);
+        lo = wave_extreme_u32<false>(lo);
+        hi = wave_extreme_u32<true>(hi);
         const uint32_t most = static_cast<uint32_t>(keep + keep / 2);
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
@@ -199,8 +201,9 @@ __device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint64_t (
 #pragma unroll
         for (int i = 0; i < E; i++)
             kept += static_cast<uint32_t>(__popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) <= v)));
-        if (kept > most) {
-            // Ties on the cut-off score: exactly `keep` keys by (score, id).
+        if (kept > most || kept < static_cast<uint32_t>(keep)) {
+            // Ties on the cut-off score (or a window the range bounds missed): exactly
+            // `keep` keys by (score, id).
             theta = select_keys<E>(key, keep, lane, rowbuf, static_cast<uint32_t>(keep) / 4, kSeg);
             return static_cast<uint32_t>(keep);
         }
