@@ -114,6 +114,34 @@ scrust_hip_status scrust_hip_knn_device(scrust_hip_context* context, const float
                                         scrust_hip_knn_stats* stats);
 
 /* ---------------------------------------------------------------------------------------
+ * Connectivities: UMAP's fuzzy simplicial set of a kNN graph, the weighted graph scanpy
+ * stores in obsp["connectivities"]. Replaces scrust_core::neighbors::connectivities
+ * (crates/scrust-core/src/neighbors.rs:208-233; bound as _scrust.connectivities,
+ * crates/scrust-py/src/embedding.rs:79-100). indices / distances are the n x k kNN graph
+ * (self excluded, as scrust_hip_knn returns it). The result is CSR: out_indptr (n + 1),
+ * out_indices and out_values (capacity entries each; 2 n k always suffices), *out_nnz.
+ * Errors: SHAPE for n == 0, k == 0, a neighbour id >= n or too small a capacity (as the
+ * reference); INVALID_PARAMETER for a negative or non-finite distance (the reference does
+ * not check). Weights agree with the reference's to its own audit bar (relative 1e-4, the
+ * exponential being rounded from f64 here and taken from libm there).
+ * ------------------------------------------------------------------------------------- */
+
+/* Host buffers; synchronous. */
+scrust_hip_status scrust_hip_connectivities(scrust_hip_context* context, const uint32_t* indices,
+                                            const float* distances, size_t n, size_t k,
+                                            uint32_t* out_indptr, uint32_t* out_indices,
+                                            float* out_values, size_t capacity, size_t* out_nnz);
+
+/* Device buffers on the context's device (e.g. the outputs of scrust_hip_knn_device);
+ * returns after completion. */
+scrust_hip_status scrust_hip_connectivities_device(scrust_hip_context* context,
+                                                   const uint32_t* indices,
+                                                   const float* distances, size_t n, size_t k,
+                                                   uint32_t* out_indptr, uint32_t* out_indices,
+                                                   float* out_values, size_t capacity,
+                                                   size_t* out_nnz);
+
+/* ---------------------------------------------------------------------------------------
  * Batched builds: many embeddings of uneven n, d and k in one persistent MFMA launch,
  * the whole pipeline captured once in a HIP graph and replayed by scrust_hip_batch_run.
  * No counterpart in the reference (its kNN takes one embedding per call).

@@ -27,6 +27,8 @@ __all__ = [
     "knn",
     "knn_hip",
     "knn_device",
+    "connectivities",
+    "connectivities_device",
     "max_supported_k",
     "Batch",
     "Server",
@@ -172,6 +174,55 @@ def knn_device(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int, dis
         ctypes.c_void_p(indices_ptr), ctypes.c_void_p(distances_ptr), ctypes.byref(st))
     _native.raise_for(status)
     return st.as_dict()
+
+
+def connectivities(indices, distances, *, context: Context | None = None):
+    """UMAP's fuzzy simplicial set of a kNN graph, mirroring ``_scrust.connectivities``
+    (crates/scrust-py/src/embedding.rs:79-100): ``indices`` uint32 and ``distances``
+    float32 arrays of equal 2-D shape (n, k) in, ``(indptr, indices, values, n_cols)`` CSR
+    out (uint32, uint32, float32). Computed on the device."""
+    idx = _graph_array(indices, np.uint32, "indices")
+    dist = _graph_array(distances, np.float32, "distances")
+    if idx.shape != dist.shape:
+        raise ValueError(f"shape mismatch: expected distances shaped {idx.shape}, got {dist.shape}")
+    n, k = idx.shape
+    handle = _handle_or_none(context)
+    cap = max(2 * n * k, 1)
+    indptr = np.empty(n + 1, dtype=np.uint32)
+    cols = np.empty(cap, dtype=np.uint32)
+    vals = np.empty(cap, dtype=np.float32)
+    nnz = ctypes.c_size_t()
+    status = _native.load().scrust_hip_connectivities(
+        handle, idx.ctypes.data_as(ctypes.c_void_p), dist.ctypes.data_as(ctypes.c_void_p), n, k,
+        indptr.ctypes.data_as(ctypes.c_void_p), cols.ctypes.data_as(ctypes.c_void_p),
+        vals.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(nnz))
+    _native.raise_for(status)
+    return indptr, cols[: nnz.value].copy(), vals[: nnz.value].copy(), n
+
+
+def connectivities_device(indices_ptr: int, distances_ptr: int, n: int, k: int, indptr_ptr: int,
+                          out_indices_ptr: int, out_values_ptr: int, capacity: int, *,
+                          context: Context | None = None) -> int:
+    """Connectivities of a device-resident kNN graph into device CSR buffers; returns nnz."""
+    ctx = context if context is not None else _thread_context()
+    nnz = ctypes.c_size_t()
+    status = _native.load().scrust_hip_connectivities_device(
+        ctx.handle, ctypes.c_void_p(indices_ptr), ctypes.c_void_p(distances_ptr), n, k,
+        ctypes.c_void_p(indptr_ptr), ctypes.c_void_p(out_indices_ptr), ctypes.c_void_p(out_values_ptr),
+        capacity, ctypes.byref(nnz))
+    _native.raise_for(status)
+    return nnz.value
+
+
+def _graph_array(a, dtype, name):
+    """A 2-D array of `dtype` (convert.rs:70-93: a wrong dtype or rank is a ValueError)."""
+    if not isinstance(a, np.ndarray):
+        raise ValueError(f"{name} must be a numpy array")
+    if a.dtype != dtype:
+        raise ValueError(f"{name} must have dtype {np.dtype(dtype).name}, got {a.dtype}")
+    if a.ndim != 2:
+        raise ValueError(f"{name} must be 2-dimensional, got {a.ndim} dimensions")
+    return np.ascontiguousarray(a)
 
 
 class Batch:

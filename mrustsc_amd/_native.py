@@ -38,6 +38,8 @@ EXPORTS = (
     "scrust_hip_knn",
     "scrust_hip_knn_ex",
     "scrust_hip_knn_device",
+    "scrust_hip_connectivities",
+    "scrust_hip_connectivities_device",
     "scrust_hip_batch_create",
     "scrust_hip_batch_run",
     "scrust_hip_batch_destroy",
@@ -188,6 +190,10 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_knn_ex.restype = ctypes.c_int
     lib.scrust_hip_knn_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
     lib.scrust_hip_knn_device.restype = ctypes.c_int
+    lib.scrust_hip_connectivities.argtypes = [vp, vp, vp, sz, sz, vp, vp, vp, sz, ctypes.POINTER(sz)]
+    lib.scrust_hip_connectivities.restype = ctypes.c_int
+    lib.scrust_hip_connectivities_device.argtypes = [vp, vp, vp, sz, sz, vp, vp, vp, sz, ctypes.POINTER(sz)]
+    lib.scrust_hip_connectivities_device.restype = ctypes.c_int
     lib.scrust_hip_batch_create.argtypes = [vp, ctypes.POINTER(Dataset), sz, opts, ctypes.POINTER(vp)]
     lib.scrust_hip_batch_create.restype = ctypes.c_int
     lib.scrust_hip_batch_run.argtypes = [vp, ctypes.POINTER(BatchStats)]

@@ -156,7 +156,7 @@ __device__ __forceinline__ uint32_t wave_extreme_u32(uint32_t v) {
 
 // In-scan compaction of one row (E * 64 == cap) whose `total` valid keys are in `key`
 // (gather_row). With at least `keep` keys it keeps every
-// key whose score is at most a cut-off v with keep <= count(score <= v) <= keep + keep/2,
+// key whose score is at most a cut-off v with keep <= count(score <= v) <= keep + keep/8,
 // found by a binary search over the score range the keys span (a handful of steps; the
 // exact KP-th score is only needed by the final selection); ties that defeat the window
 // fall back to the exact selection. Keys are written back spread evenly over the four
@@ -181,7 +181,7 @@ __device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint64_t (
         }
         lo = wave_extreme_u32<false>(lo);
         hi = wave_extreme_u32<true>(hi);
-        const uint32_t most = static_cast<uint32_t>(keep + keep / 2);
+        const uint32_t most = static_cast<uint32_t>(keep + keep / 8);
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
             uint32_t n_le = 0;

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "knn_connect.hpp"
 #include "knn_launch.hpp"
 #include "scrust_hip_internal.hpp"
 
@@ -278,6 +279,111 @@ scrust_hip_status scrust_hip_knn_ex(scrust_hip_context* context, const float* em
 scrust_hip_status scrust_hip_knn(scrust_hip_context* context, const float* embedding, size_t n,
                                  size_t d, size_t k, uint32_t* out_indices, float* out_distances) {
     return scrust_hip_knn_ex(context, embedding, n, d, k, nullptr, out_indices, out_distances, nullptr);
+}
+
+namespace {
+
+using scrust_hip::fail;
+
+// Argument checks of connectivities (neighbors.rs:209-230) on the sizes and pointers.
+scrust_hip_status check_graph(const void* indices, const void* distances, size_t n, size_t k,
+                              const void* indptr, const void* out_indices, const void* out_values,
+                              const size_t* out_nnz) {
+    if (n == 0 || k == 0)
+        return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected a neighbour graph with at least one cell and one neighbour, got " +
+                                          std::to_string(n) + " x " + std::to_string(k));
+    if (n >= 0xFFFFFFFFull)
+        return scrust_hip::invalid("n", "below 2^32 - 1", std::to_string(n));
+    if (!indices || !distances || !indptr || !out_indices || !out_values || !out_nnz)
+        return scrust_hip::invalid("buffers", "non-null", "a null pointer");
+    return SCRUST_HIP_OK;
+}
+
+// Runs the pipeline on device buffers and reads nnz and the validation flags.
+scrust_hip_status run_connectivities(scrust_hip_context* ctx, const uint32_t* indices,
+                                     const float* distances, size_t n, size_t k, uint32_t* indptr,
+                                     uint32_t* out_indices, float* out_values, size_t capacity,
+                                     size_t* out_nnz) {
+    using namespace scrust_hip;
+    const ConnLayout l = conn_layout(n, k, conn_temp_bytes(n, k));
+    SCRUST_CHECK(ctx->conn.reserve(l.total), "allocate connectivities workspace");
+    hipStream_t s = ctx->stream;
+    SCRUST_CHECK(conn_enqueue(indices, distances, n, k, ctx->conn.ptr, l, indptr, out_indices, out_values,
+                              capacity, s),
+                 "launch connectivities");
+    uint32_t flags = 0, nnz = 0;
+    SCRUST_CHECK(hipMemcpyAsync(&flags, static_cast<char*>(ctx->conn.ptr) + l.flags, sizeof flags,
+                                hipMemcpyDeviceToHost, s),
+                 "read flags");
+    SCRUST_CHECK(hipMemcpyAsync(&nnz, indptr + n, sizeof nnz, hipMemcpyDeviceToHost, s), "read nnz");
+    SCRUST_CHECK(hipStreamSynchronize(s), "wait for connectivities");
+    if (flags & kConnFlagBadId)
+        return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected neighbour ids below " + std::to_string(n) +
+                                          ", got an out-of-range neighbour id");
+    if (flags & kConnFlagBadDistance) return invalid("distances", "finite and >= 0", "a negative or non-finite value");
+    if (nnz > capacity)
+        return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected a capacity of " + std::to_string(nnz) +
+                                          " entries, got " + std::to_string(capacity));
+    *out_nnz = nnz;
+    return SCRUST_HIP_OK;
+}
+
+}  // namespace
+
+scrust_hip_status scrust_hip_connectivities_device(scrust_hip_context* context,
+                                                   const uint32_t* indices,
+                                                   const float* distances, size_t n, size_t k,
+                                                   uint32_t* out_indptr, uint32_t* out_indices,
+                                                   float* out_values, size_t capacity,
+                                                   size_t* out_nnz) {
+    scrust_hip::clear_error();
+    scrust_hip_status st = check_graph(indices, distances, n, k, out_indptr, out_indices, out_values, out_nnz);
+    if (st != SCRUST_HIP_OK) return st;
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    return run_connectivities(context, indices, distances, n, k, out_indptr, out_indices, out_values,
+                              capacity, out_nnz);
+}
+
+scrust_hip_status scrust_hip_connectivities(scrust_hip_context* context, const uint32_t* indices,
+                                            const float* distances, size_t n, size_t k,
+                                            uint32_t* out_indptr, uint32_t* out_indices,
+                                            float* out_values, size_t capacity, size_t* out_nnz) {
+    using namespace scrust_hip;
+    clear_error();
+    scrust_hip_status st = check_graph(indices, distances, n, k, out_indptr, out_indices, out_values, out_nnz);
+    if (st != SCRUST_HIP_OK) return st;
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    hipStream_t s = context->stream;
+    const size_t nk = n * k, full = 2 * nk;
+    SCRUST_CHECK(context->conn_in.reserve(nk * (sizeof(uint32_t) + sizeof(float))), "allocate graph");
+    SCRUST_CHECK(context->conn_out.reserve((n + 1) * sizeof(uint32_t) + full * (sizeof(uint32_t) + sizeof(float)) + 512),
+                 "allocate result");
+    uint32_t* d_idx = static_cast<uint32_t*>(context->conn_in.ptr);
+    float* d_dist = reinterpret_cast<float*>(d_idx + nk);
+    uint32_t* d_indptr = static_cast<uint32_t*>(context->conn_out.ptr);
+    uint32_t* d_cols = d_indptr + (n + 1);
+    float* d_vals = reinterpret_cast<float*>(d_cols + full);
+    SCRUST_CHECK(hipMemcpyAsync(d_idx, indices, nk * sizeof(uint32_t), hipMemcpyHostToDevice, s), "copy indices");
+    SCRUST_CHECK(hipMemcpyAsync(d_dist, distances, nk * sizeof(float), hipMemcpyHostToDevice, s), "copy distances");
+    size_t nnz = 0;
+    st = run_connectivities(context, d_idx, d_dist, n, k, d_indptr, d_cols, d_vals, full, &nnz);
+    if (st != SCRUST_HIP_OK) return st;
+    if (nnz > capacity)
+        return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected a capacity of " + std::to_string(nnz) +
+                                          " entries, got " + std::to_string(capacity));
+    SCRUST_CHECK(hipMemcpyAsync(out_indptr, d_indptr, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                 "copy indptr");
+    SCRUST_CHECK(hipMemcpyAsync(out_indices, d_cols, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                 "copy column ids");
+    SCRUST_CHECK(hipMemcpyAsync(out_values, d_vals, nnz * sizeof(float), hipMemcpyDeviceToHost, s),
+                 "copy weights");
+    SCRUST_CHECK(hipStreamSynchronize(s), "wait for copies");
+    *out_nnz = nnz;
+    return SCRUST_HIP_OK;
 }
 
 scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,

@@ -104,6 +104,8 @@ struct scrust_hip_context {
     hipEvent_t events[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     scrust_hip::Buffer arena, meta;                       // single-call plan workspace
     scrust_hip::Buffer input, out_indices, out_distances;  // host-input entry point
+    scrust_hip::Buffer conn;                              // connectivities workspace
+    scrust_hip::Buffer conn_in, conn_out;                 // its host-input staging
     ~scrust_hip_context();
 };
 

@@ -57,6 +57,10 @@ def _load():
     lib.baseline_expansion_knn.restype = ctypes.c_int
     lib.oracle_default_threads.argtypes = []
     lib.oracle_default_threads.restype = ctypes.c_int
+    lib.oracle_ndarray_sum_f32.argtypes = [f32p, sz]
+    lib.oracle_ndarray_sum_f32.restype = ctypes.c_float
+    lib.oracle_connectivities.argtypes = [u32p, f32p, sz, sz, u32p, u32p, f32p, sz, f32p, f32p]
+    lib.oracle_connectivities.restype = ctypes.c_long
     _lib = lib
     return lib
 
@@ -156,6 +160,35 @@ def baseline_knn(x, k: int, *, tile_rows: int = 0, queries=None,
     if rc != 0:
         raise ValueError(f"invalid arguments: n={n} d={d} k={k}")
     return idx, dist
+
+
+def ndarray_sum(values) -> float:
+    """ndarray 0.16.1's sum of a contiguous f32 array (eight-way unrolled fold)."""
+    v = np.ascontiguousarray(values, dtype=np.float32).ravel()
+    return float(_load().oracle_ndarray_sum_f32(_ptr(v, ctypes.c_float), v.size))
+
+
+def connectivities(indices, distances):
+    """UMAP's fuzzy simplicial set of a kNN graph (neighbors.rs:193-372):
+    (indptr uint32[n+1], indices uint32[nnz], values float32[nnz], sigmas, rhos)."""
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    dist = np.ascontiguousarray(distances, dtype=np.float32)
+    if idx.ndim != 2 or idx.shape != dist.shape:
+        raise ValueError(f"indices {idx.shape} and distances {dist.shape} must be equal 2-D shapes")
+    n, k = idx.shape
+    cap = 2 * n * k
+    indptr = np.empty(n + 1, dtype=np.uint32)
+    cols = np.empty(max(cap, 1), dtype=np.uint32)
+    vals = np.empty(max(cap, 1), dtype=np.float32)
+    sigmas = np.empty(n, dtype=np.float32)
+    rhos = np.empty(n, dtype=np.float32)
+    nnz = _load().oracle_connectivities(_ptr(idx, ctypes.c_uint32), _ptr(dist, ctypes.c_float), n, k,
+                                        _ptr(indptr, ctypes.c_uint32), _ptr(cols, ctypes.c_uint32),
+                                        _ptr(vals, ctypes.c_float), cap, _ptr(sigmas, ctypes.c_float),
+                                        _ptr(rhos, ctypes.c_float))
+    if nnz < 0:
+        raise ValueError(f"invalid kNN graph: n={n} k={k}")
+    return indptr, cols[:nnz].copy(), vals[:nnz].copy(), sigmas, rhos
 
 
 def threads_env_default() -> int:

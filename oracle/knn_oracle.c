@@ -619,3 +619,147 @@ int baseline_expansion_knn(const float* x, size_t n, size_t d, size_t k, size_t 
     free(norms);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Fuzzy simplicial set: crates/scrust-core/src/neighbors.rs:193-372.        */
+/* ------------------------------------------------------------------------ */
+
+/* ndarray 0.16.1 (Cargo.lock:736-737), numeric_util::unrolled_fold as used by
+ * ArrayBase::sum on contiguous data: p0..p7 take elements 8i+j, the result is
+ * ((((0 + (p0+p4)) + (p1+p5)) + (p2+p6)) + (p3+p7)) followed by the tail in order. */
+float oracle_ndarray_sum_f32(const float* x, size_t len) {
+    float p[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    size_t i = 0;
+    while (len - i >= 8) {
+        for (int j = 0; j < 8; j++) p[j] = p[j] + x[i + j];
+        i += 8;
+    }
+    float acc = 0.0f;
+    acc = acc + (p[0] + p[4]);
+    acc = acc + (p[1] + p[5]);
+    acc = acc + (p[2] + p[6]);
+    acc = acc + (p[3] + p[7]);
+    for (; i < len; i++) acc = acc + x[i];
+    return acc;
+}
+
+typedef struct {
+    uint32_t col;
+    float w;
+} conn_entry;
+
+static int conn_by_col(const void* a, const void* b) {
+    const uint32_t x = ((const conn_entry*)a)->col, y = ((const conn_entry*)b)->col;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* neighbors.rs:243-295: rho = first strictly positive distance; 64 bisection steps on
+ * sigma with the cardinality summed in f32 and compared in f64 against log2(k + 1);
+ * sigma floored at 1e-3 times the row mean over k + 1 entries (rho > 0) or the mean of
+ * all distances (rho == 0). */
+static void smooth_row(const float* row, size_t k, double target, float mean_distance,
+                       float* sigma_out, float* rho_out) {
+    float rho = 0.0f;
+    for (size_t j = 0; j < k; j++)
+        if (row[j] > 0.0f) {
+            rho = row[j];
+            break;
+        }
+    float low = 0.0f, high = FLT_MAX, mid = 1.0f;
+    for (int it = 0; it < 64; it++) {
+        float cardinality = 0.0f;
+        for (size_t j = 0; j < k; j++) {
+            const float offset = row[j] - rho;
+            cardinality = cardinality + (offset > 0.0f ? expf(-offset / mid) : 1.0f);
+        }
+        const double c = (double)cardinality;
+        if (fabs(c - target) < 1e-5) break;
+        if (c > target) {
+            high = mid;
+            mid = (low + high) / 2.0f;
+        } else {
+            low = mid;
+            mid = high >= FLT_MAX ? mid * 2.0f : (low + high) / 2.0f;
+        }
+    }
+    const float reference = rho > 0.0f ? oracle_ndarray_sum_f32(row, k) / (float)(k + 1) : mean_distance;
+    const float floor_value = 1e-3f * reference;
+    *sigma_out = mid > floor_value ? mid : floor_value;
+    *rho_out = rho;
+}
+
+long oracle_connectivities(const uint32_t* indices, const float* distances, size_t n, size_t k,
+                           uint32_t* indptr, uint32_t* out_indices, float* out_values,
+                           size_t capacity, float* sigmas, float* rhos) {
+    if (n == 0 || k == 0) return -1;
+    for (size_t i = 0; i < n * k; i++)
+        if (indices[i] >= n) return -1;
+    const double target = log2((double)(k + 1));
+    const float mean_distance = oracle_ndarray_sum_f32(distances, n * k) / (float)(n * k);
+    float* sg = (float*)malloc(n * sizeof(float));
+    float* rh = (float*)malloc(n * sizeof(float));
+    conn_entry* fwd = (conn_entry*)malloc(n * k * sizeof(conn_entry));
+    size_t* flen = (size_t*)calloc(n, sizeof(size_t));
+    size_t* indeg = (size_t*)calloc(n + 1, sizeof(size_t));
+    for (size_t i = 0; i < n; i++) {
+        smooth_row(distances + i * k, k, target, mean_distance, &sg[i], &rh[i]);
+        /* neighbors.rs:298-327: weights of the directed graph, row sorted by column. */
+        conn_entry* row = fwd + i * k;
+        size_t m = 0;
+        for (size_t j = 0; j < k; j++) {
+            const uint32_t nb = indices[i * k + j];
+            if (nb == i) continue;
+            const float offset = distances[i * k + j] - rh[i];
+            const float w = (offset <= 0.0f || sg[i] == 0.0f) ? 1.0f : expf(-offset / sg[i]);
+            if (w != 0.0f) row[m++] = (conn_entry){nb, w};
+        }
+        qsort(row, m, sizeof(conn_entry), conn_by_col);
+        flen[i] = m;
+        for (size_t j = 0; j < m; j++) indeg[row[j].col + 1]++;
+    }
+    /* neighbors.rs:330-372: the transpose, then the sorted union a + a^T - a * a^T. */
+    for (size_t i = 0; i < n; i++) indeg[i + 1] += indeg[i];
+    conn_entry* back = (conn_entry*)malloc((indeg[n] + 1) * sizeof(conn_entry));
+    size_t* fill = (size_t*)calloc(n, sizeof(size_t));
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < flen[i]; j++) {
+            const uint32_t c = fwd[i * k + j].col;
+            back[indeg[c] + fill[c]++] = (conn_entry){(uint32_t)i, fwd[i * k + j].w};
+        }
+    long nnz = 0;
+    int ok = 1;
+    indptr[0] = 0;
+    for (size_t i = 0; i < n && ok; i++) {
+        conn_entry* b = back + indeg[i];
+        const size_t bl = indeg[i + 1] - indeg[i];
+        qsort(b, bl, sizeof(conn_entry), conn_by_col);
+        const conn_entry* f = fwd + i * k;
+        size_t x = 0, y = 0;
+        while (x < flen[i] || y < bl) {
+            uint32_t col;
+            if (x < flen[i] && y < bl) col = f[x].col < b[y].col ? f[x].col : b[y].col;
+            else col = x < flen[i] ? f[x].col : b[y].col;
+            float weight = 0.0f, reverse = 0.0f;
+            if (x < flen[i] && f[x].col == col) weight = f[x++].w;
+            if (y < bl && b[y].col == col) reverse = b[y++].w;
+            if ((size_t)nnz >= capacity) {
+                ok = 0;
+                break;
+            }
+            out_indices[nnz] = col;
+            out_values[nnz] = weight + reverse - weight * reverse;
+            nnz++;
+        }
+        indptr[i + 1] = (uint32_t)nnz;
+    }
+    if (sigmas) memcpy(sigmas, sg, n * sizeof(float));
+    if (rhos) memcpy(rhos, rh, n * sizeof(float));
+    free(sg);
+    free(rh);
+    free(fwd);
+    free(flen);
+    free(indeg);
+    free(back);
+    free(fill);
+    return ok ? nnz : -1;
+}

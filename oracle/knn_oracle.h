@@ -62,6 +62,20 @@ int baseline_expansion_knn(const float* x, size_t n, size_t d, size_t k, size_t 
 /* Threads used when `threads` <= 0: every online host core. */
 int oracle_default_threads(void);
 
+/* ndarray 0.16.1's ArrayBase::sum over a contiguous f32 slice
+ * (numeric_util::unrolled_fold: eight running sums, folded pairwise, then the tail). */
+float oracle_ndarray_sum_f32(const float* x, size_t len);
+
+/* UMAP's fuzzy simplicial set of a kNN graph, restating
+ * crates/scrust-core/src/neighbors.rs:193-372 (connectivities, smooth_knn_distances,
+ * directed_weights, symmetrise). indices/distances are n x k (self excluded). Writes the
+ * CSR row pointers (n + 1), column ids and weights (capacity entries each; 2 n k always
+ * suffices) and, when not NULL, sigmas and rhos (n each). Returns nnz, or -1 on invalid
+ * arguments (an id >= n, n == 0, k == 0) or too small a capacity. */
+long oracle_connectivities(const uint32_t* indices, const float* distances, size_t n, size_t k,
+                           uint32_t* indptr, uint32_t* out_indices, float* out_values,
+                           size_t capacity, float* sigmas, float* rhos);
+
 #ifdef __cplusplus
 }
 #endif
