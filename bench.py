@@ -164,6 +164,55 @@ def parity_block(x, k, got_ids, got_dist, rows, metric="euclidean"):
             "oracle_seconds": time.time() - t0}
 
 
+def connectivities_block(single, ctx, with_oracle: bool) -> dict:
+    """The fuzzy simplicial set of the graph just built, device-resident in and out;
+    against the oracle (the reference's algorithm restated in C, one thread) when asked."""
+    import numpy as np
+    import torch
+
+    import mrustsc_amd
+
+    n, k = single.n, single.k
+    cap = 2 * n * k
+    indptr = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    cols = torch.empty(cap, dtype=torch.int32, device="cuda")
+    vals = torch.empty(cap, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def run():
+        return mrustsc_amd.connectivities_device(single.ids.data_ptr(), single.dist.data_ptr(), n, k,
+                                                 indptr.data_ptr(), cols.data_ptr(), vals.data_ptr(),
+                                                 cap, context=ctx)
+    run()
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        nnz = run()
+        walls.append(time.perf_counter() - t0)
+    best = min(walls)
+    block = {"ms": best * 1e3, "cells_per_s": n / best, "nnz": nnz,
+             "note": "device-resident kNN graph in, CSR out; best of 3 after 1 warm-up"}
+    if with_oracle:
+        import oracle
+
+        ids, dist = single.result()
+        t0 = time.perf_counter()
+        w_indptr, w_cols, w_vals, _, _ = oracle.connectivities(ids, dist)
+        sec = time.perf_counter() - t0
+        g_indptr = indptr.cpu().numpy().view(np.uint32)
+        g_cols = cols.cpu().numpy()[:nnz].view(np.uint32)
+        g_vals = vals.cpu().numpy()[:nnz]
+        same = bool(np.array_equal(g_indptr, w_indptr) and np.array_equal(g_cols, w_cols))
+        rel = (np.abs(g_vals.astype(np.float64) - w_vals) / np.maximum(np.abs(w_vals), 1e-30)) if same else None
+        block["parity"] = {"pattern_equal": same,
+                           "max_relative_error": float(rel.max()) if same else None,
+                           "bitwise_fraction": float(np.mean(g_vals == w_vals)) if same else None,
+                           "bar": "pattern equal, relative <= 1e-4, >= 90% bit-identical"}
+        block["cpu_baseline"] = {"value": n / sec, "unit": "cells/s", "cores": 1, "kind": "port",
+                                 "sample": f"all {n} rows (neighbors.rs:193-372 restated in C)"}
+    return block
+
+
 def cpu_baseline(x, k, rows=None, repeats=1):
     import numpy as np
 
@@ -214,6 +263,7 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
                        "algorithmic_flops": flops,
                        "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs"}
     got_ids, got_dist = single.result()
+    out["connectivities"] = connectivities_block(single, ctx, not args.no_cpu)
 
     if not args.no_fp32:
         walls32, st32 = single.timed(args.fp32_steps, 1, "fp32")
@@ -347,12 +397,20 @@ def run_serving_config(args):
         pool.append(synthetic.clustered(n, 50, seed=100 + i, n_clusters=int(rng.integers(5, 31))))
     log(f"[bench] serving pool: {len(pool)} embeddings, n {min(p.shape[0] for p in pool)}"
         f"-{max(p.shape[0] for p in pool)}")
+    import torch
+
+    resident = [torch.from_numpy(x).to("cuda") for x in pool]  # embeddings kept on the device
+    torch.cuda.synchronize()
+
+    def submit(server, i):
+        t = resident[i]
+        return server.submit_device(t.data_ptr(), t.shape[0], t.shape[1], 15, keep_alive=t)
     out = base_line(args, 0.0, 0.0)
     with mrustsc_amd.Server(slots=4) as server:
         # Saturation: a closed burst of requests, all submitted at once.
         order = rng.integers(0, len(pool), size=64)
         t0 = time.perf_counter()
-        tickets = [server.submit(pool[i], 15) for i in order]
+        tickets = [submit(server, i) for i in order]
         for t in tickets:
             server.wait(t)
         sat = len(order) / (time.perf_counter() - t0)
@@ -371,7 +429,7 @@ def run_serving_config(args):
                 delay = (int(when) - time.monotonic_ns()) * 1e-9
                 if delay > 0:
                     time.sleep(delay)
-                tickets.append((int(when), server.submit(pool[i], 15)))
+                tickets.append((int(when), submit(server, i)))
             lat = []
             for when, t in tickets:
                 _, _, info = server.wait(t)
@@ -390,7 +448,8 @@ def run_serving_config(args):
                 "p99_ms": mid["p99_ms"], "p50_ms": mid["p50_ms"],
                 "saturation_rps": sat, "loads": loads, "server": stats,
                 "config": {"workload": "Poisson arrivals, n log-uniform 1,000-200,000, d=50, "
-                                       "k=15, host embeddings, 4 slots", "pool": len(pool)}})
+                                       "k=15, device-resident embeddings, results to host, 4 slots",
+                           "pool": len(pool)}})
     return out
 
 

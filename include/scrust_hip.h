@@ -220,6 +220,14 @@ scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const floa
                                            const scrust_hip_knn_options* options,
                                            uint32_t* out_indices, float* out_distances,
                                            uint64_t* ticket);
+/* As scrust_hip_server_submit with a device-resident embedding (a pointer on the server's
+ * device, e.g. a PCA result kept on the GPU): no staging and no upload; the results still
+ * land in the host buffers. The embedding must stay valid until the wait returns. */
+scrust_hip_status scrust_hip_server_submit_device(scrust_hip_server* server,
+                                                  const float* embedding, size_t n, size_t d,
+                                                  size_t k, const scrust_hip_knn_options* options,
+                                                  uint32_t* out_indices, float* out_distances,
+                                                  uint64_t* ticket);
 /* Waits up to timeout_ms (< 0: forever) for a ticket; returns the build's status (KERNEL
  * with a "timed out" message when it is still running; the ticket stays valid). */
 scrust_hip_status scrust_hip_server_wait(scrust_hip_server* server, uint64_t ticket,

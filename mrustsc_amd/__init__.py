@@ -303,6 +303,21 @@ class Server:
         self._pending[ticket.value] = (x, ids, dist)
         return ticket.value
 
+    def submit_device(self, embedding_ptr: int, n: int, d: int, k: int, *, metric: str = "euclidean",
+                      precision: str = "bf16", keep_alive=None) -> int:
+        """Submits a device-resident embedding (raw device pointer on the server's device);
+        results come back to host arrays. ``keep_alive`` (e.g. the owning tensor) is held
+        until the ticket is waited for."""
+        ids = np.empty((int(n), int(k)), dtype=np.uint32)
+        dist = np.empty((int(n), int(k)), dtype=np.float32)
+        opts = _options(metric, precision)
+        ticket = ctypes.c_uint64()
+        _native.raise_for(self._lib.scrust_hip_server_submit_device(
+            self._handle, ctypes.c_void_p(embedding_ptr), int(n), int(d), int(k), ctypes.byref(opts),
+            ids.ctypes.data, dist.ctypes.data, ctypes.byref(ticket)))
+        self._pending[ticket.value] = (keep_alive, ids, dist)
+        return ticket.value
+
     def wait(self, ticket: int, timeout_ms: int = -1):
         """(ids, distances, info) of a ticket; info holds the request's timestamps."""
         info = _native.RequestInfo()

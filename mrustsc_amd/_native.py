@@ -45,6 +45,7 @@ EXPORTS = (
     "scrust_hip_batch_destroy",
     "scrust_hip_server_create",
     "scrust_hip_server_submit",
+    "scrust_hip_server_submit_device",
     "scrust_hip_server_wait",
     "scrust_hip_server_stats_get",
     "scrust_hip_server_destroy",
@@ -204,6 +205,8 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_server_create.restype = ctypes.c_int
     lib.scrust_hip_server_submit.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
     lib.scrust_hip_server_submit.restype = ctypes.c_int
+    lib.scrust_hip_server_submit_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+    lib.scrust_hip_server_submit_device.restype = ctypes.c_int
     lib.scrust_hip_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(RequestInfo)]
     lib.scrust_hip_server_wait.restype = ctypes.c_int
     lib.scrust_hip_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]

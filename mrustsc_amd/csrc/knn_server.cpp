@@ -87,6 +87,7 @@ class RangeAllocator {
 struct Request {
     uint64_t ticket = 0;
     const float* x = nullptr;
+    bool device_input = false;  // x is a device pointer (no staging, no upload)
     size_t n = 0, d = 0, k = 0;
     scrust_hip_knn_options options{};
     uint32_t* out_indices = nullptr;
@@ -115,10 +116,11 @@ struct Layout {
     size_t input, indices, distances, arena, meta, total;
 };
 
-Layout region_layout(size_t n, size_t d, size_t k, size_t arena_bytes, size_t meta_bytes) {
+Layout region_layout(size_t n, size_t d, size_t k, size_t arena_bytes, size_t meta_bytes,
+                     bool device_input) {
     Layout l{};
     l.input = 0;
-    l.indices = l.input + round_region(n * d * sizeof(float));
+    l.indices = l.input + (device_input ? 0 : round_region(n * d * sizeof(float)));
     l.distances = l.indices + round_region(n * k * sizeof(uint32_t));
     l.arena = l.distances + round_region(n * k * sizeof(float));
     l.meta = l.arena + round_region(arena_bytes);
@@ -181,9 +183,9 @@ bool scrust_hip_server::start(Slot& slot, Request& r) {
         r.error = msg;
         return false;
     };
-    const Layout l = region_layout(r.n, r.d, r.k, r.arena_bytes, r.meta_bytes);
+    const Layout l = region_layout(r.n, r.d, r.k, r.arena_bytes, r.meta_bytes, r.device_input);
     char* base = static_cast<char*>(pool) + r.offset;
-    const size_t in_bytes = r.n * r.d * sizeof(float);
+    const size_t in_bytes = r.device_input ? 0 : r.n * r.d * sizeof(float);
     const size_t idx_bytes = r.n * r.k * sizeof(uint32_t);
     const size_t dist_bytes = r.n * r.k * sizeof(float);
     // Pinned staging: [input | indices | distances] at the region's offsets, then the
@@ -201,13 +203,15 @@ bool scrust_hip_server::start(Slot& slot, Request& r) {
         slot.pinned_bytes = grow;
     }
     char* pin = static_cast<char*>(slot.pinned);
-    std::memcpy(pin + l.input, r.x, in_bytes);
-    std::vector<SetSpec> specs{SetSpec{reinterpret_cast<const float*>(base + l.input), r.n, r.d, r.k,
+    if (in_bytes) std::memcpy(pin + l.input, r.x, in_bytes);
+    const float* x = r.device_input ? r.x : reinterpret_cast<const float*>(base + l.input);
+    std::vector<SetSpec> specs{SetSpec{x, r.n, r.d, r.k,
                                        reinterpret_cast<uint32_t*>(base + l.indices),
                                        reinterpret_cast<float*>(base + l.distances)}};
     scrust_hip_status st = plan_layout(specs, &r.options, r.plan, compute_units);
     if (st != SCRUST_HIP_OK) return failed(st, last_error());
-    hipError_t err = hipMemcpyAsync(base + l.input, pin + l.input, in_bytes, hipMemcpyHostToDevice, slot.stream);
+    hipError_t err = hipSuccess;
+    if (in_bytes) err = hipMemcpyAsync(base + l.input, pin + l.input, in_bytes, hipMemcpyHostToDevice, slot.stream);
     if (err != hipSuccess) return failed(SCRUST_HIP_KERNEL, hipGetErrorString(err));
     st = plan_place(r.plan, base + l.arena, base + l.meta, slot.stream, pin + meta_at);
     if (st != SCRUST_HIP_OK) return failed(st, last_error());
@@ -225,7 +229,7 @@ bool scrust_hip_server::start(Slot& slot, Request& r) {
 }
 
 void scrust_hip_server::finish(Slot& slot, Request& r) {
-    const Layout l = region_layout(r.n, r.d, r.k, r.arena_bytes, r.meta_bytes);
+    const Layout l = region_layout(r.n, r.d, r.k, r.arena_bytes, r.meta_bytes, r.device_input);
     const char* pin = static_cast<const char*>(slot.pinned);
     uint32_t scalars[kScalarCount];
     std::memcpy(scalars, pin + l.arena, sizeof scalars);
@@ -356,11 +360,11 @@ scrust_hip_status scrust_hip_server_create(int device, const scrust_hip_server_c
     return SCRUST_HIP_OK;
 }
 
-scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const float* embedding,
-                                           size_t n, size_t d, size_t k,
-                                           const scrust_hip_knn_options* options,
-                                           uint32_t* out_indices, float* out_distances,
-                                           uint64_t* ticket) {
+namespace {
+
+scrust_hip_status submit_request(scrust_hip_server* server, const float* embedding, bool device_input,
+                                 size_t n, size_t d, size_t k, const scrust_hip_knn_options* options,
+                                 uint32_t* out_indices, float* out_distances, uint64_t* ticket) {
     clear_error();
     scrust_hip_status st = validate(embedding, n, d, k, out_indices, out_distances, options);
     if (st != SCRUST_HIP_OK) return st;
@@ -368,6 +372,7 @@ scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const floa
     if (!ticket) return invalid("ticket", "a non-null pointer", "null");
     auto r = std::make_unique<Request>();
     r->x = embedding;
+    r->device_input = device_input;
     r->n = n;
     r->d = d;
     r->k = k;
@@ -381,7 +386,7 @@ scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const floa
     if (st != SCRUST_HIP_OK) return st;
     r->arena_bytes = sizing.arena_bytes;
     r->meta_bytes = sizing.meta_bytes;
-    r->bytes = region_layout(n, d, k, r->arena_bytes, r->meta_bytes).total;
+    r->bytes = region_layout(n, d, k, r->arena_bytes, r->meta_bytes, device_input).total;
     std::lock_guard<std::mutex> lk(server->mu);
     if (r->bytes > server->pool_bytes) {
         server->rejected++;
@@ -397,6 +402,24 @@ scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const floa
     server->submitted++;
     server->cv_work.notify_all();
     return SCRUST_HIP_OK;
+}
+
+}  // namespace
+
+scrust_hip_status scrust_hip_server_submit(scrust_hip_server* server, const float* embedding,
+                                           size_t n, size_t d, size_t k,
+                                           const scrust_hip_knn_options* options,
+                                           uint32_t* out_indices, float* out_distances,
+                                           uint64_t* ticket) {
+    return submit_request(server, embedding, false, n, d, k, options, out_indices, out_distances, ticket);
+}
+
+scrust_hip_status scrust_hip_server_submit_device(scrust_hip_server* server, const float* embedding,
+                                                  size_t n, size_t d, size_t k,
+                                                  const scrust_hip_knn_options* options,
+                                                  uint32_t* out_indices, float* out_distances,
+                                                  uint64_t* ticket) {
+    return submit_request(server, embedding, true, n, d, k, options, out_indices, out_distances, ticket);
 }
 
 scrust_hip_status scrust_hip_server_wait(scrust_hip_server* server, uint64_t ticket,

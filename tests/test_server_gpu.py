@@ -81,3 +81,19 @@ def test_a_bad_request_fails_alone(context):
     want_ids, want_dist = mrustsc_amd.knn_hip(good, 10, context=context)
     np.testing.assert_array_equal(ids, want_ids)
     np.testing.assert_array_equal(dist, want_dist)
+
+
+def test_device_resident_requests_equal_host_requests(context):
+    torch = pytest.importorskip("torch")
+    reqs = _requests(8, seed=5)
+    alone = [mrustsc_amd.knn_hip(x, k, context=context) for x, k in reqs]
+    tensors = [torch.from_numpy(x).to("cuda") for x, _ in reqs]
+    torch.cuda.synchronize()
+    print("[case] server device-resident requests", flush=True)
+    with mrustsc_amd.Server(slots=3) as server:
+        tickets = [server.submit_device(t.data_ptr(), t.shape[0], t.shape[1], k, keep_alive=t)
+                   for t, (_, k) in zip(tensors, reqs)]
+        for (ids_a, dist_a), t in zip(alone, tickets):
+            ids, dist, _ = server.wait(t)
+            np.testing.assert_array_equal(ids, ids_a)
+            np.testing.assert_array_equal(dist, dist_a)
