@@ -300,6 +300,16 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
         h2h.append(time.perf_counter() - t0)
     out["host_to_host"] = {"cells_per_s": n / min(h2h[1:]), "best_ms": min(h2h[1:]) * 1e3,
                            "note": "knn_hip from host memory, best of 2 after 1 warm-up"}
+    # The device half of pp.neighbors: graph + connectivities, host embedding in, both obsp
+    # matrices out, the graph kept on the device between the steps.
+    nb = []
+    for i in range(3):
+        t0 = time.perf_counter()
+        mrustsc_amd.neighbors(x, n_neighbors=k + 1, context=ctx)
+        nb.append(time.perf_counter() - t0)
+    out["pp_neighbors_host_to_host"] = {"cells_per_s": n / min(nb[1:]), "best_ms": min(nb[1:]) * 1e3,
+                                        "note": "mrustsc_amd.neighbors: kNN graph + connectivities, "
+                                                "best of 2 after 1 warm-up"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, k, cpu_rows, cpu_repeats)
     del single

@@ -141,6 +141,17 @@ scrust_hip_status scrust_hip_connectivities_device(scrust_hip_context* context,
                                                    float* out_values, size_t capacity,
                                                    size_t* out_nnz);
 
+/* The device half of pp.neighbors (python/scrust/pp/_basics.py:186-215): the kNN graph and
+ * its connectivities in one call, the graph kept on the device between the two steps. The
+ * kNN outputs are scrust_hip_knn_ex's; the CSR is scrust_hip_connectivities'. The caller
+ * forms obsp["distances"] from the kNN outputs (fixed-width rows: indptr = 0, k, 2k, ...). */
+scrust_hip_status scrust_hip_neighbors(scrust_hip_context* context, const float* embedding,
+                                      size_t n, size_t d, size_t k,
+                                      const scrust_hip_knn_options* options,
+                                      uint32_t* out_indices, float* out_distances,
+                                      uint32_t* conn_indptr, uint32_t* conn_indices,
+                                      float* conn_values, size_t capacity, size_t* conn_nnz);
+
 /* ---------------------------------------------------------------------------------------
  * Batched builds: many embeddings of uneven n, d and k in one persistent MFMA launch,
  * the whole pipeline captured once in a HIP graph and replayed by scrust_hip_batch_run.

@@ -29,6 +29,7 @@ __all__ = [
     "knn_device",
     "connectivities",
     "connectivities_device",
+    "neighbors",
     "max_supported_k",
     "Batch",
     "Server",
@@ -212,6 +213,44 @@ def connectivities_device(indices_ptr: int, distances_ptr: int, n: int, k: int, 
         capacity, ctypes.byref(nnz))
     _native.raise_for(status)
     return nnz.value
+
+
+def neighbors(embedding, *, n_neighbors: int = 15, use_rep: str = "X_pca", metric: str = "euclidean",
+              precision: str = "bf16", context: Context | None = None) -> dict:
+    """What ``sr.pp.neighbors`` stores (python/scrust/pp/_basics.py:186-215), from an
+    embedding array: ``obsp`` entries "distances" and "connectivities" as
+    (indptr, indices, values, shape) CSR parts and ``uns["neighbors"]``. The kNN graph and
+    its connectivities are built in one device call (``scrust_hip_neighbors``); scanpy's
+    n_neighbors counts the cell itself, so the graph has n_neighbors - 1 columns."""
+    if n_neighbors < 2:
+        raise ValueError(f"n_neighbors must be at least 2, got {n_neighbors}")
+    x = _embedding(embedding)
+    n, d = x.shape
+    k = int(n_neighbors) - 1
+    opts = _options(metric, precision)
+    handle = _handle_or_none(context)
+    ids = np.empty((n, k), dtype=np.uint32)
+    dist = np.empty((n, k), dtype=np.float32)
+    cap = max(2 * n * k, 1)
+    indptr = np.empty(n + 1, dtype=np.uint32)
+    cols = np.empty(cap, dtype=np.uint32)
+    vals = np.empty(cap, dtype=np.float32)
+    nnz = ctypes.c_size_t()
+    status = _native.load().scrust_hip_neighbors(
+        handle, x.ctypes.data, n, d, k, ctypes.byref(opts), ids.ctypes.data, dist.ctypes.data,
+        indptr.ctypes.data, cols.ctypes.data, vals.ctypes.data, cap, ctypes.byref(nnz))
+    _native.raise_for(status)
+    # knn returns one fixed-width row per cell, so the CSR offsets are the row starts.
+    dist_indptr = np.arange(0, n * k + 1, k, dtype=np.uint32)
+    return {
+        "obsp": {
+            "distances": (dist_indptr, ids.ravel(), dist.ravel(), (n, n)),
+            "connectivities": (indptr, cols[: nnz.value].copy(), vals[: nnz.value].copy(), (n, n)),
+        },
+        "uns": {"neighbors": {"connectivities_key": "connectivities", "distances_key": "distances",
+                              "params": {"n_neighbors": int(n_neighbors), "method": "umap",
+                                         "use_rep": use_rep}}},
+    }
 
 
 def _graph_array(a, dtype, name):

@@ -386,6 +386,56 @@ scrust_hip_status scrust_hip_connectivities(scrust_hip_context* context, const u
     return SCRUST_HIP_OK;
 }
 
+scrust_hip_status scrust_hip_neighbors(scrust_hip_context* context, const float* embedding, size_t n,
+                                      size_t d, size_t k, const scrust_hip_knn_options* options,
+                                      uint32_t* out_indices, float* out_distances,
+                                      uint32_t* conn_indptr, uint32_t* conn_indices,
+                                      float* conn_values, size_t capacity, size_t* conn_nnz) {
+    using namespace scrust_hip;
+    clear_error();
+    scrust_hip_status st = validate(embedding, n, d, k, out_indices, out_distances, options);
+    if (st != SCRUST_HIP_OK) return st;
+    if (!conn_indptr || !conn_indices || !conn_values || !conn_nnz)
+        return invalid("buffers", "non-null", "a null pointer");
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    hipStream_t s = context->stream;
+    const size_t nk = n * k, full = 2 * nk;
+    SCRUST_CHECK(context->input.reserve(n * d * sizeof(float)), "allocate input");
+    SCRUST_CHECK(context->out_indices.reserve(nk * sizeof(uint32_t)), "allocate output");
+    SCRUST_CHECK(context->out_distances.reserve(nk * sizeof(float)), "allocate output");
+    SCRUST_CHECK(context->conn_out.reserve((n + 1) * sizeof(uint32_t) + full * (sizeof(uint32_t) + sizeof(float)) + 512),
+                 "allocate result");
+    SCRUST_CHECK(hipMemcpyAsync(context->input.ptr, embedding, n * d * sizeof(float), hipMemcpyHostToDevice, s),
+                 "copy input");
+    uint32_t* d_ids = static_cast<uint32_t*>(context->out_indices.ptr);
+    float* d_dist = static_cast<float*>(context->out_distances.ptr);
+    st = run_single(context, static_cast<const float*>(context->input.ptr), n, d, k, options, d_ids, d_dist,
+                    nullptr);
+    if (st != SCRUST_HIP_OK) return st;
+    // The graph stays on the device for the fuzzy simplicial set.
+    uint32_t* d_indptr = static_cast<uint32_t*>(context->conn_out.ptr);
+    uint32_t* d_cols = d_indptr + (n + 1);
+    float* d_vals = reinterpret_cast<float*>(d_cols + full);
+    size_t nnz = 0;
+    st = run_connectivities(context, d_ids, d_dist, n, k, d_indptr, d_cols, d_vals, full, &nnz);
+    if (st != SCRUST_HIP_OK) return st;
+    if (nnz > capacity)
+        return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected a capacity of " + std::to_string(nnz) +
+                                          " entries, got " + std::to_string(capacity));
+    SCRUST_CHECK(hipMemcpyAsync(out_indices, d_ids, nk * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "copy indices");
+    SCRUST_CHECK(hipMemcpyAsync(out_distances, d_dist, nk * sizeof(float), hipMemcpyDeviceToHost, s), "copy distances");
+    SCRUST_CHECK(hipMemcpyAsync(conn_indptr, d_indptr, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                 "copy indptr");
+    SCRUST_CHECK(hipMemcpyAsync(conn_indices, d_cols, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                 "copy column ids");
+    SCRUST_CHECK(hipMemcpyAsync(conn_values, d_vals, nnz * sizeof(float), hipMemcpyDeviceToHost, s), "copy weights");
+    SCRUST_CHECK(hipStreamSynchronize(s), "wait for copies");
+    *conn_nnz = nnz;
+    return SCRUST_HIP_OK;
+}
+
 scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
                                           const scrust_hip_dataset* datasets, size_t count,
                                           const scrust_hip_knn_options* options,

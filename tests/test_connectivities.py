@@ -252,3 +252,29 @@ def test_device_rejects_bad_ids_and_distances(context):
     with pytest.raises(ValueError):
         mrustsc_amd.connectivities(ids, neg, context=context)
     mrustsc_amd.connectivities(ids, dist, context=context)  # the context still works
+
+
+def test_neighbors_rejects_fewer_than_two_neighbours():
+    with pytest.raises(ValueError, match="at least 2"):
+        mrustsc_amd.neighbors(generic(), n_neighbors=1)
+
+
+@pytest.mark.gpu
+def test_neighbors_equals_knn_then_connectivities(context):
+    # The pp.neighbors device call: the graph (n_neighbors - 1 columns) and its
+    # connectivities, as the two separate calls give them.
+    x = synthetic.clustered(8000, 50, seed=21)
+    print("[case] neighbors 8000 x 50", flush=True)
+    out = mrustsc_amd.neighbors(x, n_neighbors=15, context=context)
+    ids, dist = mrustsc_amd.knn_hip(x, 14, context=context)
+    indptr, cols, vals, _ = mrustsc_amd.connectivities(ids, dist, context=context)
+    d_indptr, d_cols, d_vals, shape = out["obsp"]["distances"]
+    assert shape == (8000, 8000)
+    np.testing.assert_array_equal(d_indptr, np.arange(0, 8000 * 14 + 1, 14))
+    np.testing.assert_array_equal(d_cols, ids.ravel())
+    np.testing.assert_array_equal(d_vals, dist.ravel())
+    c_indptr, c_cols, c_vals, _ = out["obsp"]["connectivities"]
+    np.testing.assert_array_equal(c_indptr, indptr)
+    np.testing.assert_array_equal(c_cols, cols)
+    np.testing.assert_array_equal(c_vals, vals)
+    assert out["uns"]["neighbors"]["params"] == {"n_neighbors": 15, "method": "umap", "use_rep": "X_pca"}
