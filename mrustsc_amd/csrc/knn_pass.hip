@@ -228,12 +228,24 @@ __device__ uint32_t compact_row(SCRUST_GLOBAL uint64_t* rowbuf, const uint64_t (
 template <int OP, int NCH>
 struct Operand;
 
+// LDS tile layout. A ds_read_b128 is serviced for lanes 0-31 and then lanes 32-63, each
+// half over 128 four-byte banks, so a read is conflict-free when the 32 lanes of a half
+// address 32 distinct 16-byte slots modulo 32. Lanes 16g..16g+15 read rows lc = 0..15 at
+// lane group g's part, slot lc * stride + g * gstride + c: with stride = 2 (mod 4) the
+// rows cover the 16 even residues and an odd gstride moves group g + 1 onto the odd ones.
+__host__ __device__ constexpr int lds_gstride(int group_u4) { return group_u4 | 1; }
+__host__ __device__ constexpr int lds_stride(int group_u4) {
+    return (3 * lds_gstride(group_u4) + group_u4 + 1) / 4 * 4 + 2;
+}
+
 template <int NCH>
 struct Operand<kOperandBf16, NCH> {
     typedef bf16x8 Frag;
-    // 16-byte chunks per packed row (one more per LDS row for bank spread).
+    // 16-byte chunks per packed row, per lane group.
     static __device__ __forceinline__ int row_u4(int nch) { return 4 * nch; }
+    static __device__ __forceinline__ int group_u4(int nch) { return nch; }
     static constexpr int kRowU4Max = 4 * NCH;
+    static constexpr int kGroupU4Max = NCH;
     static constexpr int kLoadsMax = NCH;  // 64 rows x 4 nch chunks / 256 threads
 };
 
@@ -241,13 +253,15 @@ template <int NCH>
 struct Operand<kOperandF32, NCH> {
     typedef float Frag;
     static __device__ __forceinline__ int row_u4(int nch) { return (nch + 3) / 4 * 4; }
+    static __device__ __forceinline__ int group_u4(int nch) { return (nch + 3) / 4; }
     static constexpr int kRowU4Max = (NCH + 3) / 4 * 4;
+    static constexpr int kGroupU4Max = (NCH + 3) / 4;
     static constexpr int kLoadsMax = kRowU4Max / 4;
 };
 
 template <int OP, int NCH>
 struct PassSmem {
-    u32x4 tile[2][kTileRefs * (Operand<OP, NCH>::kRowU4Max + 1)];
+    u32x4 tile[2][kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max)];
 };
 
 // One work item: query rows [block * 256, block * 256 + 256) of dataset s against the
@@ -259,7 +273,9 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     typedef Operand<OP, NCH> Op;
     const int nch = FIXED ? NCH : static_cast<int>(s.nch);
     const int row_u4 = FIXED ? Op::kRowU4Max : Op::row_u4(nch);
-    const int stride = row_u4 + 1;
+    const int group_u4 = FIXED ? Op::kGroupU4Max : Op::group_u4(nch);
+    const int gstride = lds_gstride(group_u4);
+    const int stride = lds_stride(group_u4);
     const int loads = FIXED ? Op::kLoadsMax : row_u4 / 4;
     const int group = row_u4;  // fp32: floats per lane group (4 groups per row)
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, lc = lane & 15;
@@ -316,12 +332,14 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // one tile's MFMA time).
     u32x4 pa[Op::kLoadsMax], pb[Op::kLoadsMax];
     // Thread chunk i of a tile is u = tid + 256 i: reference row u / row_u4, part
-    // u % row_u4, at LDS slot row * stride + part = u + u / row_u4.
+    // p = u % row_u4 of lane group p / group_u4, at LDS slot
+    // row * stride + (p / group_u4) * gstride + p % group_u4.
     int lds_at[Op::kLoadsMax];
 #pragma unroll
     for (int i = 0; i < Op::kLoadsMax; i++) {
         const int u = tid + 256 * i;
-        lds_at[i] = u + u / row_u4;
+        const int p = u % row_u4;
+        lds_at[i] = (u / row_u4) * stride + (p / group_u4) * gstride + p % group_u4;
     }
     // Global -> registers P for tile T; registers P -> LDS buffer B. Plain unrolled loops
     // with constant indices keep the sets in registers.
@@ -342,14 +360,14 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 if (FIXED || c < nch) {
-                    const bf16x8 b = __builtin_bit_cast(bf16x8, brow[g * nch + c]);
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, brow[g * gstride + c]);
 #pragma unroll
                     for (int qb = 0; qb < 4; qb++)
                         acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[qb][c], acc[qb], 0, 0, 0);
                 }
             }
         } else {
-            const float4* bq = reinterpret_cast<const float4*>(brow) + g * (group / 4);
+            const float4* bq = reinterpret_cast<const float4*>(brow) + g * gstride;
 #pragma unroll
             for (int c4 = 0; c4 < Op::kRowU4Max / 4; c4++) {
                 if (FIXED || c4 * 4 < nch) {
