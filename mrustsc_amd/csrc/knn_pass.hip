@@ -429,17 +429,60 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
 
     // Tile t: its MFMAs and filter on LDS buffer cur, then the next tile's registers to the
     // other buffer, a barrier, and the compaction of rows whose buffers could overflow.
+    // bf16: a block's reference fragments, read from LDS one block ahead of their MFMAs.
+    auto load_frags = [&](auto& b, int cur, int cb) __attribute__((always_inline)) {
+        const u32x4* brow = &sm.tile[cur][(cb * 16 + lc) * stride];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+            if (FIXED || c < nch) b[c] = __builtin_bit_cast(bf16x8, brow[g * gstride + c]);
+    };
+    auto mfma_frags = [&](f32x4 (&acc)[4], const auto& b) __attribute__((always_inline)) {
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) acc[qb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            if (FIXED || c < nch) {
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++)
+                    acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[c], a[qb][c], acc[qb], 0, 0, 0);
+            }
+        }
+    };
     auto consume = [&](uint32_t t, int cur) __attribute__((always_inline)) {
         f32x4 acc0[4], acc1[4];
-        // Software pipeline: the MFMAs of block cb are in flight while block cb-1 is filtered.
-        mfma_block(acc0, cur, 0);
-        mfma_block(acc1, cur, 1);
-        filter(acc0, t, 0);
-        mfma_block(acc0, cur, 2);
-        filter(acc1, t, 1);
-        mfma_block(acc1, cur, 3);
-        filter(acc0, t, 2);
-        filter(acc1, t, 3);
+        if constexpr (OP == kOperandBf16 && FIXED) {
+            // Software pipeline: block cb+1's fragments are read while block cb's MFMAs
+            // run, and block cb-1 is filtered while they run.
+            // Scheduling barriers keep the compiler from sinking each read to its use.
+            bf16x8 b0[NCH], b1[NCH];
+            load_frags(b0, cur, 0);
+            load_frags(b1, cur, 1);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_frags(acc0, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            load_frags(b0, cur, 2);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_frags(acc1, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            load_frags(b1, cur, 3);
+            __builtin_amdgcn_sched_barrier(0);
+            filter(acc0, t, 0);
+            mfma_frags(acc0, b0);
+            filter(acc1, t, 1);
+            mfma_frags(acc1, b1);
+            filter(acc0, t, 2);
+            filter(acc1, t, 3);
+        } else {
+            // Software pipeline: the MFMAs of block cb are in flight while block cb-1 is filtered.
+            mfma_block(acc0, cur, 0);
+            mfma_block(acc1, cur, 1);
+            filter(acc0, t, 0);
+            mfma_block(acc0, cur, 2);
+            filter(acc1, t, 1);
+            mfma_block(acc1, cur, 3);
+            filter(acc0, t, 2);
+            filter(acc1, t, 3);
+        }
     };
     // Compacts every row of this wave with a segment that could overflow in the next tile
     // (compact_row): the append bound tightens to the kept keys' cut-off score.
