@@ -31,6 +31,7 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 FP32_MATRIX_PEAK_TFLOPS = 157.3  # MI355X fp32 MFMA
 METRIC = "kNN-graph cells/s at 1M x 50, k=15 (BF16+rerank vs FP32); p99 request latency"
 
@@ -53,6 +54,7 @@ def parse_args():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cosine", action="store_true")
+    p.add_argument("--no-tsne", action="store_true", help="skip the t-SNE input distances block")
     p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
     p.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
@@ -213,6 +215,67 @@ def connectivities_block(single, ctx, with_oracle: bool) -> dict:
     return block
 
 
+def tsne_block(ctx, with_oracle: bool, n: int = 20000, d: int = 50) -> dict:
+    """t-SNE's input distances (tsne.rs:86, :173-188) at the reference's exact-t-SNE cell
+    cap: device-resident points in, the n x n float32 matrix out. Bound by the n^2 * 4
+    bytes it writes; the rate below is over the whole call (centring, tiles, flag
+    readback), a lower bound on the tile kernel's."""
+    import numpy as np
+    import torch
+
+    import mrustsc_amd
+    from mrustsc_amd import synthetic
+
+    x = synthetic.clustered(n, d, seed=2)
+    dx = torch.from_numpy(x).to("cuda")
+    out = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def run():
+        mrustsc_amd.tsne_squared_distances_device(dx.data_ptr(), n, d, out.data_ptr(), context=ctx)
+    run()
+    walls = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        walls.append(time.perf_counter() - t0)
+    best = min(walls)
+    written = n * n * 4.0
+    block = {"n_cells": n, "n_dims": d, "call_ms": best * 1e3,
+             "achieved_gb_per_s": written / best / 1e9,
+             "roofline": {"bound": "hbm", "achieved": written / best / 1e9, "peak": HBM_PEAK_GBPS,
+                          "unit": "GB/s", "frac": written / best / 1e9 / HBM_PEAK_GBPS,
+                          "algorithmic_bytes": written,
+                          "algorithmic_bytes_per_unit": "4 per matrix entry written; n^2 entries"},
+             "tflops": 2.0 * n * n * d / best / 1e12,
+             "note": "device-resident, best of 5 after 1 warm-up; wall clock of the whole call"}
+    if with_oracle:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        rows = (0, 512)
+        t0 = time.perf_counter()
+        want = oracle.tsne_squared_distances(x, rows=rows, threads=threads)
+        sec = time.perf_counter() - t0
+        got = out[rows[0]:rows[1]].cpu().numpy().astype(np.float64)
+        c = x.astype(np.float64) - x.astype(np.float64).mean(0)
+        norms = (c * c).sum(1)
+        lim = (2 * d + 8) * 2.0 ** -24 * (norms[rows[0]:rows[1], None] + norms[None, :]) * 1.0001
+        err = np.abs(got - want)
+        block["parity"] = {"rows_checked": rows[1] - rows[0], "within_bound": bool(np.all(err <= lim)),
+                           "max_error_over_bound": float((err / lim).max()),
+                           "bitwise_fraction": float(np.mean(got == want)),
+                           "bar": "|device - oracle| <= (2 d + 8) 2^-24 (|a|^2 + |b|^2)"}
+        block["cpu_baseline"] = {"value": (rows[1] - rows[0]) * n / sec, "unit": "entries/s",
+                                 "cores": threads, "kind": "port",
+                                 "sample": f"rows {rows[0]}-{rows[1]} of the {n} x {n} matrix "
+                                           "(tsne.rs:173-188 restated in C)",
+                                 "device_entries_per_s": n * n / best}
+    del out, dx
+    torch.cuda.synchronize()
+    return block
+
+
 def cpu_baseline(x, k, rows=None, repeats=1):
     import numpy as np
 
@@ -310,6 +373,8 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
     out["pp_neighbors_host_to_host"] = {"cells_per_s": n / min(nb[1:]), "best_ms": min(nb[1:]) * 1e3,
                                         "note": "mrustsc_amd.neighbors: kNN graph + connectivities, "
                                                 "best of 2 after 1 warm-up"}
+    if not args.no_tsne and name == "c3":
+        out["tsne_squared_distances"] = tsne_block(ctx, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, k, cpu_rows, cpu_repeats)
     del single

@@ -153,6 +153,26 @@ scrust_hip_status scrust_hip_neighbors(scrust_hip_context* context, const float*
                                       float* conn_values, size_t capacity, size_t* conn_nnz);
 
 /* ---------------------------------------------------------------------------------------
+ * t-SNE input distances: the pairwise squared-distance matrix exact t-SNE starts from.
+ * Replaces squared_distances(&centred(&points)) (crates/scrust-core/src/tsne.rs:86,
+ * :173-188): column means subtracted in f32, then (|a|^2 + |b|^2) - 2 a.b clamped at 0, the
+ * Gram block on the fp32 MFMA. out is n x n float32 row-major (the diagonal included).
+ * The reference's expansion order is its gemm's, so entries agree with it to the
+ * expansion's rounding bound, (2 d + 8) 2^-24 (|a|^2 + |b|^2), not bit for bit.
+ * Errors: SHAPE for d == 0 (tsne.rs:120-122) or n == 0; INVALID_PARAMETER for
+ * n > 20000, the exact formulation's cap (tsne.rs:40, :146-152), or non-finite input.
+ * ------------------------------------------------------------------------------------- */
+
+/* Host buffers; synchronous. */
+scrust_hip_status scrust_hip_tsne_squared_distances(scrust_hip_context* context, const float* points,
+                                                    size_t n, size_t d, float* out);
+
+/* Device buffers on the context's device; returns after completion. */
+scrust_hip_status scrust_hip_tsne_squared_distances_device(scrust_hip_context* context,
+                                                           const float* points, size_t n, size_t d,
+                                                           float* out);
+
+/* ---------------------------------------------------------------------------------------
  * Batched builds: many embeddings of uneven n, d and k in one persistent MFMA launch,
  * the whole pipeline captured once in a HIP graph and replayed by scrust_hip_batch_run.
  * No counterpart in the reference (its kNN takes one embedding per call).

@@ -30,6 +30,8 @@ __all__ = [
     "connectivities",
     "connectivities_device",
     "neighbors",
+    "tsne_squared_distances",
+    "tsne_squared_distances_device",
     "max_supported_k",
     "Batch",
     "Server",
@@ -251,6 +253,33 @@ def neighbors(embedding, *, n_neighbors: int = 15, use_rep: str = "X_pca", metri
                               "params": {"n_neighbors": int(n_neighbors), "method": "umap",
                                          "use_rep": use_rep}}},
     }
+
+
+def tsne_squared_distances(points, *, context: Context | None = None) -> np.ndarray:
+    """The pairwise squared distances exact t-SNE starts from,
+    ``squared_distances(&centred(&points))`` (crates/scrust-core/src/tsne.rs:86, :173-188):
+    a float32 (n, d) array in, the float32 (n, n) matrix out (diagonal included), computed on
+    the device. Entries agree with the reference's to the expansion's rounding bound
+    (2 d + 8) 2^-24 (|a|^2 + |b|^2). At most 20,000 cells (tsne.rs:146-152)."""
+    if not isinstance(points, np.ndarray) or points.ndim != 2 or points.dtype != np.float32:
+        raise ValueError("points must be a 2-D numpy array of float32")
+    x = np.ascontiguousarray(points)
+    n, d = x.shape
+    handle = _handle_or_none(context)
+    out = np.empty((n, n), dtype=np.float32)
+    status = _native.load().scrust_hip_tsne_squared_distances(handle, x.ctypes.data, n, d, out.ctypes.data)
+    _native.raise_for(status)
+    return out
+
+
+def tsne_squared_distances_device(points_ptr: int, n: int, d: int, out_ptr: int, *,
+                                  context: Context | None = None) -> None:
+    """Device-resident variant: an (n, d) float32 input and an (n, n) float32 output, both
+    device pointers on the context's device."""
+    ctx = context if context is not None else _thread_context()
+    status = _native.load().scrust_hip_tsne_squared_distances_device(
+        ctx.handle, ctypes.c_void_p(points_ptr), n, d, ctypes.c_void_p(out_ptr))
+    _native.raise_for(status)
 
 
 def _graph_array(a, dtype, name):

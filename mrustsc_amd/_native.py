@@ -41,6 +41,8 @@ EXPORTS = (
     "scrust_hip_connectivities",
     "scrust_hip_connectivities_device",
     "scrust_hip_neighbors",
+    "scrust_hip_tsne_squared_distances",
+    "scrust_hip_tsne_squared_distances_device",
     "scrust_hip_batch_create",
     "scrust_hip_batch_run",
     "scrust_hip_batch_destroy",
@@ -196,6 +198,10 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_connectivities.restype = ctypes.c_int
     lib.scrust_hip_connectivities_device.argtypes = [vp, vp, vp, sz, sz, vp, vp, vp, sz, ctypes.POINTER(sz)]
     lib.scrust_hip_connectivities_device.restype = ctypes.c_int
+    lib.scrust_hip_tsne_squared_distances.argtypes = [vp, vp, sz, sz, vp]
+    lib.scrust_hip_tsne_squared_distances.restype = ctypes.c_int
+    lib.scrust_hip_tsne_squared_distances_device.argtypes = [vp, vp, sz, sz, vp]
+    lib.scrust_hip_tsne_squared_distances_device.restype = ctypes.c_int
     lib.scrust_hip_neighbors.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, vp, vp, vp, sz, ctypes.POINTER(sz)]
     lib.scrust_hip_neighbors.restype = ctypes.c_int
     lib.scrust_hip_batch_create.argtypes = [vp, ctypes.POINTER(Dataset), sz, opts, ctypes.POINTER(vp)]

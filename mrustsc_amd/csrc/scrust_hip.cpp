@@ -17,6 +17,7 @@
 
 #include "knn_connect.hpp"
 #include "knn_launch.hpp"
+#include "knn_tsne.hpp"
 #include "scrust_hip_internal.hpp"
 
 namespace scrust_hip {
@@ -433,6 +434,80 @@ scrust_hip_status scrust_hip_neighbors(scrust_hip_context* context, const float*
     SCRUST_CHECK(hipMemcpyAsync(conn_values, d_vals, nnz * sizeof(float), hipMemcpyDeviceToHost, s), "copy weights");
     SCRUST_CHECK(hipStreamSynchronize(s), "wait for copies");
     *conn_nnz = nnz;
+    return SCRUST_HIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+using scrust_hip::fail;
+
+// Argument checks of the t-SNE input distances: no features is a shape error
+// (tsne.rs:120-122), more cells than the exact formulation allows an invalid parameter
+// (tsne.rs:146-152).
+scrust_hip_status check_tsne(const void* points, size_t n, size_t d, const void* out) {
+    if (d == 0) return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected at least one feature, got zero features");
+    if (n == 0) return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected at least one cell, got zero cells");
+    if (n > scrust_hip::kTsneMaxCells)
+        return scrust_hip::invalid("n_cells", "at most 20000 for the exact O(n^2) formulation", std::to_string(n));
+    if (d > 8192) return scrust_hip::invalid("points", "at most 8192 features", std::to_string(d));
+    if (!points || !out) return scrust_hip::invalid("buffers", "non-null", "a null pointer");
+    return SCRUST_HIP_OK;
+}
+
+// Centring and the distance tiles on device buffers; reads the non-finite flag.
+scrust_hip_status run_tsne(scrust_hip_context* ctx, const float* points, size_t n, size_t d, float* out) {
+    using namespace scrust_hip;
+    const TsneLayout l = tsne_layout(n, d);
+    SCRUST_CHECK(ctx->tsne.reserve(l.total), "allocate t-SNE workspace");
+    hipStream_t s = ctx->stream;
+    SCRUST_CHECK(tsne_enqueue(points, n, d, ctx->tsne.ptr, l, out, s), "launch t-SNE distances");
+    uint32_t flags = 0;
+    SCRUST_CHECK(hipMemcpyAsync(&flags, static_cast<char*>(ctx->tsne.ptr) + l.flags, sizeof flags,
+                                hipMemcpyDeviceToHost, s),
+                 "read flags");
+    SCRUST_CHECK(hipStreamSynchronize(s), "wait for t-SNE distances");
+    if (flags & kTsneFlagNonFinite) return invalid("points", "finite values", "NaN or infinity");
+    return SCRUST_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+scrust_hip_status scrust_hip_tsne_squared_distances_device(scrust_hip_context* context, const float* points,
+                                                           size_t n, size_t d, float* out) {
+    using namespace scrust_hip;
+    clear_error();
+    scrust_hip_status st = check_tsne(points, n, d, out);
+    if (st != SCRUST_HIP_OK) return st;
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    return run_tsne(context, points, n, d, out);
+}
+
+scrust_hip_status scrust_hip_tsne_squared_distances(scrust_hip_context* context, const float* points,
+                                                    size_t n, size_t d, float* out) {
+    using namespace scrust_hip;
+    clear_error();
+    scrust_hip_status st = check_tsne(points, n, d, out);
+    if (st != SCRUST_HIP_OK) return st;
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    hipStream_t s = context->stream;
+    const size_t cells = n * n;
+    SCRUST_CHECK(context->input.reserve(n * d * sizeof(float)), "allocate input");
+    SCRUST_CHECK(context->tsne_out.reserve(cells * sizeof(float)), "allocate output");
+    SCRUST_CHECK(hipMemcpyAsync(context->input.ptr, points, n * d * sizeof(float), hipMemcpyHostToDevice, s),
+                 "copy input");
+    float* d_out = static_cast<float*>(context->tsne_out.ptr);
+    st = run_tsne(context, static_cast<const float*>(context->input.ptr), n, d, d_out);
+    if (st != SCRUST_HIP_OK) return st;
+    SCRUST_CHECK(hipMemcpyAsync(out, d_out, cells * sizeof(float), hipMemcpyDeviceToHost, s), "copy distances");
+    SCRUST_CHECK(hipStreamSynchronize(s), "wait for copies");
     return SCRUST_HIP_OK;
 }
 

@@ -61,6 +61,8 @@ def _load():
     lib.oracle_ndarray_sum_f32.restype = ctypes.c_float
     lib.oracle_connectivities.argtypes = [u32p, f32p, sz, sz, u32p, u32p, f32p, sz, f32p, f32p]
     lib.oracle_connectivities.restype = ctypes.c_long
+    lib.oracle_tsne_squared_distances.argtypes = [f32p, sz, sz, sz, sz, f32p, ctypes.c_int]
+    lib.oracle_tsne_squared_distances.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -189,6 +191,20 @@ def connectivities(indices, distances):
     if nnz < 0:
         raise ValueError(f"invalid kNN graph: n={n} k={k}")
     return indptr, cols[:nnz].copy(), vals[:nnz].copy(), sigmas, rhos
+
+
+def tsne_squared_distances(points, *, rows=None, threads: int = 0) -> np.ndarray:
+    """t-SNE's input distances (tsne.rs:86, :173-188): the (n, n) float32 matrix, or the
+    rows ``range(*rows)`` of it."""
+    x = _f32(points)
+    n, d = x.shape
+    begin, end = (0, n) if rows is None else rows
+    out = np.empty((max(end - begin, 0), n), dtype=np.float32)
+    rc = _load().oracle_tsne_squared_distances(_ptr(x, ctypes.c_float), n, d, begin, end,
+                                               _ptr(out, ctypes.c_float), threads)
+    if rc != 0:
+        raise ValueError(f"invalid arguments: n={n} d={d} rows={rows}")
+    return out
 
 
 def threads_env_default() -> int:

@@ -763,3 +763,75 @@ long oracle_connectivities(const uint32_t* indices, const float* distances, size
     free(fill);
     return ok ? nnz : -1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* t-SNE input distances: squared_distances(&centred(&points)),             */
+/* crates/scrust-core/src/tsne.rs:86, :173-188.                              */
+/*                                                                          */
+/* centred (:173-177): column sums in f32, times f32(1/n) (candle's affine   */
+/* casts the factor to f32), subtracted in f32. squared_distances           */
+/* (:181-188): norms = row sums of squares, gram = points . points^T, then  */
+/* (norm_i + norm_j) - 2 gram_ij, clamped at 0. candle's reductions and     */
+/* gemm's blocking fix summation orders this restatement does not follow:   */
+/* column sums are exact and rounded once, row sums and dot products run    */
+/* in dimension order with every product rounded. Entries meet the          */
+/* reference's to the expansion's rounding bound, not bit for bit.          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const float* coords;
+    const float* norms;
+    size_t n, d, row_begin;
+    float* out;
+} tsne_ctx;
+
+static void tsne_row(void* arg, size_t item, void* scratch) {
+    (void)scratch;
+    const tsne_ctx* c = (const tsne_ctx*)arg;
+    const size_t i = c->row_begin + item;
+    const float* a = c->coords + i * c->d;
+    float* dst = c->out + item * c->n;
+    for (size_t j = 0; j < c->n; j++) {
+        const float* b = c->coords + j * c->d;
+        float g = 0.0f;
+        for (size_t t = 0; t < c->d; t++) {
+            const float p = a[t] * b[t];
+            g = g + p;
+        }
+        const float sum = c->norms[i] + c->norms[j];
+        const float v = sum - 2.0f * g;
+        dst[j] = v > 0.0f ? v : 0.0f;
+    }
+}
+
+int oracle_tsne_squared_distances(const float* x, size_t n, size_t d, size_t row_begin,
+                                  size_t row_end, float* out, int threads) {
+    if (n == 0 || d == 0 || row_begin > row_end || row_end > n) return -1;
+    float* coords = (float*)malloc(n * d * sizeof(float));
+    float* norms = (float*)malloc(n * sizeof(float));
+    float* mean = (float*)malloc(d * sizeof(float));
+    if (!coords || !norms || !mean) {
+        free(coords);
+        free(norms);
+        free(mean);
+        return -1;
+    }
+    const float inv_n = (float)(1.0 / (double)n);
+    for (size_t j = 0; j < d; j++) mean[j] = (float)oracle_exact_sum_f32(x + j, n, d) * inv_n;
+    for (size_t i = 0; i < n; i++) {
+        float s = 0.0f;
+        for (size_t j = 0; j < d; j++) {
+            const float c = x[i * d + j] - mean[j];
+            coords[i * d + j] = c;
+            const float sq = c * c;
+            s = s + sq;
+        }
+        norms[i] = s;
+    }
+    tsne_ctx ctx = {coords, norms, n, d, row_begin, out};
+    parallel_for(tsne_row, &ctx, row_end - row_begin, 0, threads);
+    free(coords);
+    free(norms);
+    free(mean);
+    return 0;
+}

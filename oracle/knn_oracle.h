@@ -76,6 +76,12 @@ long oracle_connectivities(const uint32_t* indices, const float* distances, size
                            uint32_t* indptr, uint32_t* out_indices, float* out_values,
                            size_t capacity, float* sigmas, float* rhos);
 
+/* The t-SNE input distances, squared_distances(&centred(&points))
+ * (crates/scrust-core/src/tsne.rs:86, :173-188), for rows [row_begin, row_end): out holds
+ * (row_end - row_begin) x n floats. Returns 0, or -1 on invalid arguments. */
+int oracle_tsne_squared_distances(const float* x, size_t n, size_t d, size_t row_begin,
+                                  size_t row_end, float* out, int threads);
+
 #ifdef __cplusplus
 }
 #endif
