@@ -75,6 +75,27 @@ __device__ __forceinline__ float direct_square(const float* __restrict__ a,
     return s < snap_threshold(d, na, nb) ? 0.0f : s;
 }
 
+// direct_square with the loads widened to VEC floats (rows 4 * VEC-byte aligned: VEC divides
+// d); the fma chain is the same, in dimension order.
+template <int VEC>
+__device__ __forceinline__ float direct_square_vec(const float* __restrict__ a,
+                                                   const float* __restrict__ b, int d, float na,
+                                                   float nb) {
+    typedef float vec_t __attribute__((ext_vector_type(VEC)));
+    const vec_t* av = reinterpret_cast<const vec_t*>(a);
+    const vec_t* bv = reinterpret_cast<const vec_t*>(b);
+    float s = 0.0f;
+    for (int j = 0; j < d / VEC; j++) {
+        const vec_t x = av[j], y = bv[j];
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const float delta = x[e] - y[e];
+            s = __builtin_fmaf(delta, delta, s);
+        }
+    }
+    return s < snap_threshold(d, na, nb) ? 0.0f : s;
+}
+
 // f64 recomputation for boundary near-ties: f64 differences of the same f32
 // coordinates, f64 fma in dimension order.
 __device__ __forceinline__ double direct_square_f64(const float* __restrict__ a,

@@ -262,7 +262,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     const bool present = raw != kNoKey;
     const bool valid = present && id != q && id < n;
     float dsq = kInf;
-    if (valid) dsq = direct_square(xq, s.coords + static_cast<size_t>(id) * d, d, nq, s.norms[id]);
+    if (valid) {
+        const float* xr = s.coords + static_cast<size_t>(id) * d;
+        // Rows are 16- or 8-byte aligned when 4 or 2 divides d (the arena is 256-byte aligned).
+        if ((d & 3) == 0) dsq = direct_square_vec<4>(xq, xr, d, nq, s.norms[id]);
+        else if ((d & 1) == 0) dsq = direct_square_vec<2>(xq, xr, d, nq, s.norms[id]);
+        else dsq = direct_square(xq, xr, d, nq, s.norms[id]);
+    }
     uint64_t key = valid ? make_key(dsq, id) : kNoKey;
     key = segment_sort<KP>(key, lane);
 
