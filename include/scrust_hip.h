@@ -152,6 +152,18 @@ scrust_hip_status scrust_hip_neighbors(scrust_hip_context* context, const float*
                                       uint32_t* conn_indptr, uint32_t* conn_indices,
                                       float* conn_values, size_t capacity, size_t* conn_nnz);
 
+/* scrust_hip_neighbors with every buffer on the context's device: an embedding already there
+ * (the PCA output kept on the device; pp.neighbors reads obsm["X_pca"] at
+ * python/scrust/pp/_basics.py:198) in, both graphs out, nothing crossing PCIe but the nnz.
+ * Returns after completion. No counterpart in the reference (its arrays live in host memory). */
+scrust_hip_status scrust_hip_neighbors_device(scrust_hip_context* context, const float* embedding,
+                                             size_t n, size_t d, size_t k,
+                                             const scrust_hip_knn_options* options,
+                                             uint32_t* out_indices, float* out_distances,
+                                             uint32_t* conn_indptr, uint32_t* conn_indices,
+                                             float* conn_values, size_t capacity,
+                                             size_t* conn_nnz);
+
 /* ---------------------------------------------------------------------------------------
  * t-SNE input distances: the pairwise squared-distance matrix exact t-SNE starts from.
  * Replaces squared_distances(&centred(&points)) (crates/scrust-core/src/tsne.rs:86,

@@ -30,6 +30,7 @@ __all__ = [
     "connectivities",
     "connectivities_device",
     "neighbors",
+    "neighbors_device",
     "tsne_squared_distances",
     "tsne_squared_distances_device",
     "max_supported_k",
@@ -253,6 +254,29 @@ def neighbors(embedding, *, n_neighbors: int = 15, use_rep: str = "X_pca", metri
                               "params": {"n_neighbors": int(n_neighbors), "method": "umap",
                                          "use_rep": use_rep}}},
     }
+
+
+def neighbors_device(embedding_ptr: int, n: int, d: int, n_neighbors: int, out_indices_ptr: int,
+                     out_distances_ptr: int, conn_indptr_ptr: int, conn_indices_ptr: int,
+                     conn_values_ptr: int, capacity: int, *, metric: str = "euclidean",
+                     precision: str = "bf16", context: Context | None = None) -> int:
+    """The device half of ``pp.neighbors`` with everything on the device: an (n, d) float32
+    embedding already there (the PCA output kept on the device) in; the kNN graph
+    (n x (n_neighbors - 1) uint32 ids and float32 distances, the rows of obsp["distances"])
+    and the connectivities CSR (n + 1 offsets, ``capacity`` column ids and weights;
+    2 n (n_neighbors - 1) always suffice) out, as device pointers. Returns nnz."""
+    if n_neighbors < 2:
+        raise ValueError(f"n_neighbors must be at least 2, got {n_neighbors}")
+    opts = _options(metric, precision)
+    ctx = context if context is not None else _thread_context()
+    nnz = ctypes.c_size_t()
+    status = _native.load().scrust_hip_neighbors_device(
+        ctx.handle, ctypes.c_void_p(embedding_ptr), n, d, int(n_neighbors) - 1, ctypes.byref(opts),
+        ctypes.c_void_p(out_indices_ptr), ctypes.c_void_p(out_distances_ptr),
+        ctypes.c_void_p(conn_indptr_ptr), ctypes.c_void_p(conn_indices_ptr),
+        ctypes.c_void_p(conn_values_ptr), capacity, ctypes.byref(nnz))
+    _native.raise_for(status)
+    return nnz.value
 
 
 def tsne_squared_distances(points, *, context: Context | None = None) -> np.ndarray:

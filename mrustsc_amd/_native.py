@@ -41,6 +41,7 @@ EXPORTS = (
     "scrust_hip_connectivities",
     "scrust_hip_connectivities_device",
     "scrust_hip_neighbors",
+    "scrust_hip_neighbors_device",
     "scrust_hip_tsne_squared_distances",
     "scrust_hip_tsne_squared_distances_device",
     "scrust_hip_batch_create",
@@ -204,6 +205,9 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_tsne_squared_distances_device.restype = ctypes.c_int
     lib.scrust_hip_neighbors.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, vp, vp, vp, sz, ctypes.POINTER(sz)]
     lib.scrust_hip_neighbors.restype = ctypes.c_int
+    lib.scrust_hip_neighbors_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, vp, vp, vp, sz,
+                                                ctypes.POINTER(sz)]
+    lib.scrust_hip_neighbors_device.restype = ctypes.c_int
     lib.scrust_hip_batch_create.argtypes = [vp, ctypes.POINTER(Dataset), sz, opts, ctypes.POINTER(vp)]
     lib.scrust_hip_batch_create.restype = ctypes.c_int
     lib.scrust_hip_batch_run.argtypes = [vp, ctypes.POINTER(BatchStats)]

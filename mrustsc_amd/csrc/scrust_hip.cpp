@@ -511,6 +511,27 @@ scrust_hip_status scrust_hip_tsne_squared_distances(scrust_hip_context* context,
     return SCRUST_HIP_OK;
 }
 
+scrust_hip_status scrust_hip_neighbors_device(scrust_hip_context* context, const float* embedding,
+                                             size_t n, size_t d, size_t k,
+                                             const scrust_hip_knn_options* options,
+                                             uint32_t* out_indices, float* out_distances,
+                                             uint32_t* conn_indptr, uint32_t* conn_indices,
+                                             float* conn_values, size_t capacity, size_t* conn_nnz) {
+    using namespace scrust_hip;
+    clear_error();
+    scrust_hip_status st = validate(embedding, n, d, k, out_indices, out_distances, options);
+    if (st != SCRUST_HIP_OK) return st;
+    if (!conn_indptr || !conn_indices || !conn_values || !conn_nnz)
+        return invalid("buffers", "non-null", "a null pointer");
+    if (!context) return fail(SCRUST_HIP_NO_GPU, "no HIP device context");
+    if (hipSetDevice(context->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(context->device));
+    st = run_single(context, embedding, n, d, k, options, out_indices, out_distances, nullptr);
+    if (st != SCRUST_HIP_OK) return st;
+    return run_connectivities(context, out_indices, out_distances, n, k, conn_indptr, conn_indices,
+                              conn_values, capacity, conn_nnz);
+}
+
 scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
                                           const scrust_hip_dataset* datasets, size_t count,
                                           const scrust_hip_knn_options* options,

@@ -278,3 +278,38 @@ def test_neighbors_equals_knn_then_connectivities(context):
     np.testing.assert_array_equal(c_cols, cols)
     np.testing.assert_array_equal(c_vals, vals)
     assert out["uns"]["neighbors"]["params"] == {"n_neighbors": 15, "method": "umap", "use_rep": "X_pca"}
+
+
+@pytest.mark.gpu
+def test_device_resident_neighbors_equal_the_host_call(context):
+    # The embedding stays on the device (a PCA output kept there) and so do both graphs.
+    import torch
+
+    x = synthetic.clustered(6000, 50, seed=23)
+    n, k = 6000, 14
+    print("[case] neighbors_device 6000 x 50", flush=True)
+    host = mrustsc_amd.neighbors(x, n_neighbors=15, context=context)
+    dx = torch.from_numpy(x).to("cuda")
+    ids = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    cap = 2 * n * k
+    indptr = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    cols = torch.empty(cap, dtype=torch.int32, device="cuda")
+    vals = torch.empty(cap, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    nnz = mrustsc_amd.neighbors_device(dx.data_ptr(), n, 50, 15, ids.data_ptr(), dist.data_ptr(),
+                                       indptr.data_ptr(), cols.data_ptr(), vals.data_ptr(), cap,
+                                       context=context)
+    d_indptr, d_cols, d_vals, _ = host["obsp"]["distances"]
+    np.testing.assert_array_equal(ids.cpu().numpy().view(np.uint32).ravel(), d_cols)
+    np.testing.assert_array_equal(dist.cpu().numpy().ravel(), d_vals)
+    c_indptr, c_cols, c_vals, _ = host["obsp"]["connectivities"]
+    assert nnz == len(c_cols)
+    np.testing.assert_array_equal(indptr.cpu().numpy().view(np.uint32), c_indptr)
+    np.testing.assert_array_equal(cols.cpu().numpy()[:nnz].view(np.uint32), c_cols)
+    np.testing.assert_array_equal(vals.cpu().numpy()[:nnz], c_vals)
+
+
+def test_device_resident_neighbors_checks_arguments_first():
+    with pytest.raises(ValueError, match="at least 2"):
+        mrustsc_amd.neighbors_device(0, 10, 5, 1, 0, 0, 0, 0, 0, 0)
