@@ -30,6 +30,9 @@ __device__ __forceinline__ SCRUST_GLOBAL T* global_ptr(T* p) {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// A native 16-byte vector (HIP's uint4 is a class with unions, which keeps arrays of it
+// out of registers).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // Keys: (squared distance, id) packed so that u64 order is (d^2, id) order.

@@ -89,13 +89,14 @@ __global__ void clear_sums_kernel(const DatasetDesc* __restrict__ sets) {
 //    fma in dimension order (knn.rs:148-156). Cosine: x / |x| with the norm in f64.
 //    Operand layouts are described in knn_pass.hip. Rows n.. of a dataset on the pass
 //    (up to a whole pair of 64-row tiles) write sentinel reference rows (norm feature +inf).
+//    prepare_kernel (one thread per row) serves the exact path and the fp32 pass;
+//    prepare_rows_kernel (64 rows per block, coalesced) the bf16 pass.
 // ===========================================================================
 
 __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t count,
                                uint32_t total_prep, int metric, int operand,
                                unsigned* __restrict__ flags) {
     const uint32_t flat = blockIdx.x * blockDim.x + threadIdx.x;
-    constexpr uint16_t kBf16Inf = 0x7F80u, kBf16MinusHalf = 0xBF00u;
     float nrm = 0.0f;
     bool finite = true;
     uint32_t set = 0;
@@ -107,7 +108,9 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
         const int d = static_cast<int>(s.d);
         const int nch = static_cast<int>(s.nch), group = static_cast<int>(s.group);
         const int op = s.on_pass ? operand : kOperandNone;
-        if (row < s.n) {
+        if (op == kOperandBf16) {
+            // prepare_rows_kernel's (coalesced, 64 rows per block).
+        } else if (row < s.n) {
             has_norm = true;
             const float* src = s.x + row * d;
             float* dst = s.coords + row * d;
@@ -129,43 +132,7 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
                 nrm = __builtin_fmaf(c, c, nrm);
             }
             s.norms[row] = nrm;
-            if (op == kOperandBf16) {
-                // Features (see knn_pass.hip), eight per 16-byte store: reference
-                // [h | l | h | nh nl nl2 | 0], query [h | h | l | -1/2 x3 | 0].
-                const uint16_t nh = bf16_rn(nrm);
-                const float rest = nrm - bf16_value(nh);
-                const uint16_t nl = bf16_rn(rest);
-                const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
-                uint4* r = reinterpret_cast<uint4*>(s.pack_ref + row * (32 * nch));
-                uint4* q = reinterpret_cast<uint4*>(s.pack_query + row * (32 * nch));
-                for (int chunk = 0; chunk < 4 * nch; chunk++) {
-                    uint32_t rw[4], qw[4];
-                    for (int e = 0; e < 8; e++) {
-                        const int f = chunk * 8 + e;
-                        uint16_t rv = 0, qv = 0;
-                        if (f < 3 * d) {
-                            const int part = f / d;
-                            const float c = dst[f - part * d];
-                            const uint16_t h = bf16_rn(c);
-                            const uint16_t l = bf16_rn(c - bf16_value(h));
-                            rv = part == 1 ? l : h;
-                            qv = part == 2 ? l : h;
-                        } else if (f < 3 * d + 3) {
-                            rv = f == 3 * d ? nh : (f == 3 * d + 1 ? nl : nl2);
-                            qv = kBf16MinusHalf;
-                        }
-                        if (e & 1) {
-                            rw[e >> 1] |= static_cast<uint32_t>(rv) << 16;
-                            qw[e >> 1] |= static_cast<uint32_t>(qv) << 16;
-                        } else {
-                            rw[e >> 1] = rv;
-                            qw[e >> 1] = qv;
-                        }
-                    }
-                    r[chunk] = make_uint4(rw[0], rw[1], rw[2], rw[3]);
-                    q[chunk] = make_uint4(qw[0], qw[1], qw[2], qw[3]);
-                }
-            } else if (op == kOperandF32) {
+            if (op == kOperandF32) {
                 float4* p4 = reinterpret_cast<float4*>(s.pack_f32 + row * (4 * group));
                 for (int t4 = 0; t4 < group; t4++) {
                     float v[4];
@@ -177,15 +144,6 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
                     }
                     p4[t4] = make_float4(v[0], v[1], v[2], v[3]);
                 }
-            }
-        } else if (op == kOperandBf16) {
-            uint4* r = reinterpret_cast<uint4*>(s.pack_ref + row * (32 * nch));
-            const int inf_chunk = (3 * d) / 8, inf_word = ((3 * d) % 8) / 2;
-            const uint32_t inf_bits = static_cast<uint32_t>(kBf16Inf) << (16 * ((3 * d) & 1));
-            for (int chunk = 0; chunk < 4 * nch; chunk++) {
-                uint32_t w[4] = {0u, 0u, 0u, 0u};
-                if (chunk == inf_chunk) w[inf_word] = inf_bits;
-                r[chunk] = make_uint4(w[0], w[1], w[2], w[3]);
             }
         } else if (op == kOperandF32) {
             float4* p4 = reinterpret_cast<float4*>(s.pack_f32 + row * (4 * group));
@@ -217,6 +175,123 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
     if (!finite) atomicOr(flags, kFlagNonFinite);
 }
 
+// Datasets on the bf16 pass: 64 rows per block. The rows' inputs are read into LDS with
+// coalesced loads, one thread per row centres (or scales) its row there and sums its norm
+// (the same arithmetic as prepare_kernel), and the coordinates and packed operands leave
+// with coalesced stores: consecutive threads write consecutive 16-byte chunks. Rows
+// n .. ref_rows - 1 get sentinel reference rows (norm feature +inf).
+constexpr int kPrepRows = 64;
+
+__global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __restrict__ sets,
+                                                           int metric, unsigned* __restrict__ flags) {
+    __shared__ float xs[kPrepRows * kMaxPassD];
+    __shared__ double mean_s[kMaxPassD];
+    __shared__ float nrm_s[kPrepRows];
+    constexpr uint16_t kBf16Inf = 0x7F80u, kBf16MinusHalf = 0xBF00u;
+    const DatasetDesc& s = sets[blockIdx.z];
+    if (!s.on_pass) return;
+    const uint32_t n = s.n, ref_rows = (n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
+    const uint32_t r0 = blockIdx.x * kPrepRows;
+    if (r0 >= ref_rows) return;
+    const int tid = threadIdx.x;
+    const int d = static_cast<int>(s.d), nch = static_cast<int>(s.nch);
+    const uint32_t rows = r0 < n ? (n - r0 < kPrepRows ? n - r0 : kPrepRows) : 0u;
+    const SCRUST_GLOBAL float* src = global_ptr(s.x) + static_cast<size_t>(r0) * d;
+    for (uint32_t i = tid; i < rows * d; i += 256) xs[i] = src[i];
+    if (metric != kMetricCosine)
+        for (int j = tid; j < d; j += 256) mean_s[j] = s.mean[j];
+    __syncthreads();
+    if (tid < kPrepRows) {
+        float nrm = 0.0f;
+        bool finite = true;
+        if (static_cast<uint32_t>(tid) < rows) {
+            float* row = xs + tid * d;
+            double length = 0.0;
+            if (metric == kMetricCosine) {
+                double acc = 0.0;
+                for (int j = 0; j < d; j++) acc = __builtin_fma((double)row[j], (double)row[j], acc);
+                length = __builtin_sqrt(acc);
+            }
+            for (int j = 0; j < d; j++) {
+                const float v = row[j];
+                finite = finite && __builtin_isfinite(v);
+                float c;
+                if (metric == kMetricCosine)
+                    c = length > 0.0 ? static_cast<float>(static_cast<double>(v) / length) : 0.0f;
+                else
+                    c = static_cast<float>(static_cast<double>(v) - mean_s[j]);
+                row[j] = c;
+                nrm = __builtin_fmaf(c, c, nrm);
+            }
+        }
+        nrm_s[tid] = nrm;
+        // Max squared norm (norms are >= 0: their bits order as unsigned ints).
+        unsigned bits = __float_as_uint(nrm);
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned other = __shfl_xor(bits, off, kWave);
+            bits = other > bits ? other : bits;
+        }
+        if (tid == 0 && rows > 0) atomicMax(s.max_norm_bits, bits);
+        if (__ballot(!finite) != 0ull && tid == 0) atomicOr(flags, kFlagNonFinite);
+    }
+    __syncthreads();
+    SCRUST_GLOBAL float* coords = global_ptr(s.coords) + static_cast<size_t>(r0) * d;
+    for (uint32_t i = tid; i < rows * d; i += 256) coords[i] = xs[i];
+    if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[r0 + tid] = nrm_s[tid];
+    // Packed operands (see knn_pass.hip): reference [h | l | h | nh nl nl2 | 0], query
+    // [h | h | l | -1/2 x3 | 0], 32 nch features per row, eight per 16-byte chunk.
+    const int chunks = 4 * nch;
+    const uint32_t prep_rows = ref_rows - r0 < kPrepRows ? ref_rows - r0 : kPrepRows;
+    SCRUST_GLOBAL u32x4* rp = global_ptr(reinterpret_cast<u32x4*>(s.pack_ref)) + static_cast<size_t>(r0) * chunks;
+    SCRUST_GLOBAL u32x4* qp = global_ptr(reinterpret_cast<u32x4*>(s.pack_query)) + static_cast<size_t>(r0) * chunks;
+    for (uint32_t u = tid; u < prep_rows * chunks; u += 256) {
+        const uint32_t row = u / chunks;
+        const int chunk = static_cast<int>(u - row * chunks);
+        uint32_t rw[4], qw[4];
+        if (row < rows) {
+            const float* c_row = xs + row * d;
+            const float nrm = nrm_s[row];
+            const uint16_t nh = bf16_rn(nrm);
+            const float rest = nrm - bf16_value(nh);
+            const uint16_t nl = bf16_rn(rest);
+            const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                uint16_t rv[2], qv[2];
+#pragma unroll
+                for (int h2 = 0; h2 < 2; h2++) {
+                    const int f = chunk * 8 + e + h2;
+                    rv[h2] = 0;
+                    qv[h2] = 0;
+                    if (f < 3 * d) {
+                        const int part = (f >= d) + (f >= 2 * d);
+                        const float c = c_row[f - part * d];
+                        const uint16_t h = bf16_rn(c);
+                        const uint16_t l = bf16_rn(c - bf16_value(h));
+                        rv[h2] = part == 1 ? l : h;
+                        qv[h2] = part == 2 ? l : h;
+                    } else if (f < 3 * d + 3) {
+                        rv[h2] = f == 3 * d ? nh : (f == 3 * d + 1 ? nl : nl2);
+                        qv[h2] = kBf16MinusHalf;
+                    }
+                }
+                rw[e >> 1] = static_cast<uint32_t>(rv[0]) | static_cast<uint32_t>(rv[1]) << 16;
+                qw[e >> 1] = static_cast<uint32_t>(qv[0]) | static_cast<uint32_t>(qv[1]) << 16;
+            }
+            rp[u] = u32x4{rw[0], rw[1], rw[2], rw[3]};
+            qp[u] = u32x4{qw[0], qw[1], qw[2], qw[3]};
+        } else {
+            // Sentinel reference row: +inf at the norm feature 3d, zero elsewhere.
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                const int f = chunk * 8 + e;
+                rw[e >> 1] = (f == 3 * d ? kBf16Inf : 0u) | (f + 1 == 3 * d ? static_cast<uint32_t>(kBf16Inf) << 16 : 0u);
+            }
+            rp[u] = u32x4{rw[0], rw[1], rw[2], rw[3]};
+        }
+    }
+}
+
 void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsigned* flags,
                         hipStream_t stream) {
     hipLaunchKernelGGL(clear_sums_kernel, dim3(shape.count), dim3(256), 0, stream, sets);
@@ -229,6 +304,9 @@ void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsign
     const unsigned blocks = (shape.total_prep + 255) / 256;
     hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
                        shape.total_prep, shape.metric, shape.operand, flags);
+    if (shape.operand == kOperandBf16 && shape.max_prep_blocks > 0)
+        hipLaunchKernelGGL(prepare_rows_kernel, dim3(shape.max_prep_blocks, 1, shape.count), dim3(256), 0,
+                           stream, sets, shape.metric, flags);
 }
 
 // ===========================================================================

@@ -79,6 +79,7 @@ struct BatchShape {
     uint32_t max_row_chunks;  // max ceil(n / colsum_rows(n))
     uint32_t max_col_groups;  // max ceil(d / 64)
     uint32_t max_d;           // largest d
+    uint32_t max_prep_blocks; // max ceil(ref_rows / 64) over datasets on the bf16 pass
     int metric;
     int operand;
 };

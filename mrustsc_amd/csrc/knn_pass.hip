@@ -43,10 +43,6 @@ namespace scrust_hip {
 
 constexpr int kTileRefs = 64;  // references per LDS tile (4 column blocks of 16)
 
-// A native 16-byte vector (HIP's uint4 is a class with unions, which keeps arrays of it
-// out of registers).
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (I < N) {

@@ -205,6 +205,9 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         plan.shape.max_row_chunks = std::max<uint32_t>(plan.shape.max_row_chunks, static_cast<uint32_t>((sp.n + colsum_rows(s.n) - 1) / colsum_rows(s.n)));
         plan.shape.max_col_groups = std::max<uint32_t>(plan.shape.max_col_groups, static_cast<uint32_t>((sp.d + 63) / 64));
         plan.shape.max_d = std::max<uint32_t>(plan.shape.max_d, static_cast<uint32_t>(sp.d));
+        if (pass && plan.operand == kOperandBf16)
+            plan.shape.max_prep_blocks = std::max<uint32_t>(plan.shape.max_prep_blocks,
+                                                            static_cast<uint32_t>((ref_rows + 63) / 64));
         row += s.n;
         prep += pass ? static_cast<uint32_t>(ref_rows) : s.n;
     }
