@@ -58,6 +58,8 @@ def parse_args():
     p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
     p.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
+    p.add_argument("--gemm-ceiling", action="store_true",
+                   help="time the platform's bf16 GEMM (torch / hipBLASLt) as a practical MFMA ceiling")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer three times and exit: the "
                         "known-size copy that calibrates the HBM byte counters")
@@ -75,6 +77,32 @@ def measured_traffic(n: int, d: int, k: int):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def gemm_ceiling() -> int:
+    """The platform library's bf16 GEMM on this GPU (torch.matmul, hipBLASLt underneath): the
+    practical MFMA rate under sustained load, beside the 2.5 PFLOP/s nominal peak. The last
+    shape is the pass's contraction written out as a plain GEMM (K = 160 packed features)."""
+    import torch
+
+    for m, n, k in ((8192, 8192, 8192), (16384, 16384, 4096), (65536, 4096, 160)):
+        a = torch.randn(m, k, dtype=torch.bfloat16, device="cuda")
+        b = torch.randn(k, n, dtype=torch.bfloat16, device="cuda")
+        for _ in range(3):
+            c = a @ b
+        torch.cuda.synchronize()
+        iters = max(10, int(2.0e15 * 0.5 / (2 * m * n * k)))  # about half a second of work
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            c = a @ b
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        print(json.dumps({"gemm": [m, n, k], "dtype": "bf16", "iters": iters, "ms": ms,
+                          "tflops": 2.0 * m * n * k / ms / 1e9}), flush=True)
+        del a, b, c
+    return 0
 
 
 def copy_probe(mib: int) -> int:
@@ -537,6 +565,8 @@ def main() -> int:
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.gemm_ceiling:
+        return gemm_ceiling()
 
     import torch  # noqa: F401  (loads the HIP runtime the library then shares)
 

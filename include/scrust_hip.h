@@ -56,7 +56,10 @@ typedef struct scrust_hip_knn_options {
     /* Candidate buffer of the pass, slots per query row: 0 = tuned or default (256),
      * else 128, 256 or 512. Every setting gives the same result. */
     uint32_t tile;
-    uint32_t reserved;
+    /* Row order of the candidate pass: 0 = automatic (rows grouped by nearest pivot for
+     * single embeddings of at least 65,536 cells, so that each query block meets its
+     * neighbours early), 1 = input order, 2 = grouped. Every setting gives the same result. */
+    uint32_t order;
 } scrust_hip_knn_options;
 
 typedef struct scrust_hip_knn_stats {

@@ -122,23 +122,28 @@ def _embedding(embedding) -> np.ndarray:
     return np.ascontiguousarray(embedding)
 
 
-def _options(metric: str, precision: str, tile: int = 0) -> KnnOptions:
+ORDERS = {"auto": 0, "input": 1, "grouped": 2}
+
+
+def _options(metric: str, precision: str, tile: int = 0, order: str = "auto") -> KnnOptions:
     if metric not in _native.METRICS:
         raise ValueError(f"metric must be one of {sorted(_native.METRICS)}, got {metric!r}")
     if precision not in _native.PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_native.PRECISIONS)}, got {precision!r}")
-    return KnnOptions(_native.METRICS[metric], _native.PRECISIONS[precision], int(tile), 0)
+    if order not in ORDERS:
+        raise ValueError(f"order must be one of {sorted(ORDERS)}, got {order!r}")
+    return KnnOptions(_native.METRICS[metric], _native.PRECISIONS[precision], int(tile), ORDERS[order])
 
 
 def knn_hip(embedding, k: int, *, metric: str = "euclidean", precision: str = "bf16",
-            context: Context | None = None, stats: bool = False, tile: int = 0):
+            context: Context | None = None, stats: bool = False, tile: int = 0, order: str = "auto"):
     """kNN on the device from host memory. Returns (ids, distances) or, with
     ``stats=True``, (ids, distances, stats dict)."""
     x = _embedding(embedding)
     if isinstance(k, bool) or int(k) != k or k < 0:
         raise ValueError(f"k must be a non-negative integer, got {k!r}")
     k = int(k)
-    opts = _options(metric, precision, tile)
+    opts = _options(metric, precision, tile, order)
     handle = _handle_or_none(context)
     n, d = x.shape
     ids = np.empty((n, k), dtype=np.uint32)
@@ -167,10 +172,10 @@ def knn(embedding, k: int, device: str = "auto"):
 
 def knn_device(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int, distances_ptr: int,
                *, metric: str = "euclidean", precision: str = "bf16",
-               context: Context | None = None, tile: int = 0) -> dict:
+               context: Context | None = None, tile: int = 0, order: str = "auto") -> dict:
     """kNN on device-resident buffers (raw device pointers, e.g. ``tensor.data_ptr()``).
     Returns the call's stats."""
-    opts = _options(metric, precision, tile)
+    opts = _options(metric, precision, tile, order)
     ctx = context if context is not None else _thread_context()
     st = KnnStats()
     status = _native.load().scrust_hip_knn_device(

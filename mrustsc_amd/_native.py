@@ -61,7 +61,7 @@ class KnnOptions(ctypes.Structure):
         ("metric", ctypes.c_uint32),
         ("precision", ctypes.c_uint32),
         ("tile", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("order", ctypes.c_uint32),
     ]
 
 

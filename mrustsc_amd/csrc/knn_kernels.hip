@@ -112,8 +112,10 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
             // prepare_rows_kernel's (coalesced, 64 rows per block).
         } else if (row < s.n) {
             has_norm = true;
-            const float* src = s.x + row * d;
-            float* dst = s.coords + row * d;
+            // Prepare rows are in pass order; coordinates and norms stay in input order.
+            const size_t in_row = s.perm && op != kOperandNone ? s.perm[row] : row;
+            const float* src = s.x + in_row * d;
+            float* dst = s.coords + in_row * d;
             double length = 0.0;
             if (metric == kMetricCosine) {
                 double acc = 0.0;
@@ -131,7 +133,7 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
                 dst[j] = c;
                 nrm = __builtin_fmaf(c, c, nrm);
             }
-            s.norms[row] = nrm;
+            s.norms[in_row] = nrm;
             if (op == kOperandF32) {
                 float4* p4 = reinterpret_cast<float4*>(s.pack_f32 + row * (4 * group));
                 for (int t4 = 0; t4 < group; t4++) {
@@ -196,8 +198,17 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
     const int tid = threadIdx.x;
     const int d = static_cast<int>(s.d), nch = static_cast<int>(s.nch);
     const uint32_t rows = r0 < n ? (n - r0 < kPrepRows ? n - r0 : kPrepRows) : 0u;
-    const SCRUST_GLOBAL float* src = global_ptr(s.x) + static_cast<size_t>(r0) * d;
-    for (uint32_t i = tid; i < rows * d; i += 256) xs[i] = src[i];
+    // Rows in pass order: pass position r0 + t holds input row perm[r0 + t].
+    const SCRUST_GLOBAL float* x = global_ptr(s.x);
+    const SCRUST_GLOBAL uint32_t* perm = s.perm ? global_ptr(s.perm) : nullptr;
+    if (perm) {
+        for (uint32_t i = tid; i < rows * d; i += 256) {
+            const uint32_t t = i / d;
+            xs[i] = x[static_cast<size_t>(perm[r0 + t]) * d + (i - t * d)];
+        }
+    } else {
+        for (uint32_t i = tid; i < rows * d; i += 256) xs[i] = x[static_cast<size_t>(r0) * d + i];
+    }
     if (metric != kMetricCosine)
         for (int j = tid; j < d; j += 256) mean_s[j] = s.mean[j];
     __syncthreads();
@@ -235,9 +246,17 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
         if (__ballot(!finite) != 0ull && tid == 0) atomicOr(flags, kFlagNonFinite);
     }
     __syncthreads();
-    SCRUST_GLOBAL float* coords = global_ptr(s.coords) + static_cast<size_t>(r0) * d;
-    for (uint32_t i = tid; i < rows * d; i += 256) coords[i] = xs[i];
-    if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[r0 + tid] = nrm_s[tid];
+    SCRUST_GLOBAL float* coords = global_ptr(s.coords);
+    if (perm) {
+        for (uint32_t i = tid; i < rows * d; i += 256) {
+            const uint32_t t = i / d;
+            coords[static_cast<size_t>(perm[r0 + t]) * d + (i - t * d)] = xs[i];
+        }
+        if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[perm[r0 + tid]] = nrm_s[tid];
+    } else {
+        for (uint32_t i = tid; i < rows * d; i += 256) coords[static_cast<size_t>(r0) * d + i] = xs[i];
+        if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[r0 + tid] = nrm_s[tid];
+    }
     // Packed operands (see knn_pass.hip): reference [h | l | h | nh nl nl2 | 0], query
     // [h | h | l | -1/2 x3 | 0], 32 nch features per row, eight per 16-byte chunk.
     const int chunks = 4 * nch;
@@ -334,10 +353,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     const float* xq = s.coords + static_cast<size_t>(q) * d;
     const float nq = s.norms[q];
 
+    // Candidates and thresholds are stored in pass order; candidate ids are pass positions.
+    const uint32_t pq = row_ok && s.inv_perm ? s.inv_perm[q] : q;
     uint64_t raw = kNoKey;
-    if (row_ok) raw = s.cand[static_cast<size_t>(q) * KP + j];
-    const uint32_t id = key_id(raw);
+    if (row_ok) raw = s.cand[static_cast<size_t>(pq) * KP + j];
     const bool present = raw != kNoKey;
+    uint32_t id = key_id(raw);
+    if (present && s.perm && id < n) id = s.perm[id];
     const bool valid = present && id != q && id < n;
     float dsq = kInf;
     if (valid) {
@@ -402,7 +424,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
                                    2.0 * s.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
                                    s.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
         const double cd = (d + 3) * u * 1.01;
-        const double lower = (static_cast<double>(s.theta[q]) + nqd - err) * (1.0 - cd);
+        const double lower = (static_cast<double>(s.theta[pq]) + nqd - err) * (1.0 - cd);
         const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);
         const double need = (t + tau) > snap_max ? (t + tau) : snap_max;
         certified = lower > need;

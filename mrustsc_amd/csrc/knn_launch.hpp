@@ -48,6 +48,12 @@ struct DatasetDesc {
     unsigned* max_norm_bits;      // max squared norm (as bits)
     uint32_t* out_indices;        // n x k
     float* out_distances;         // n x k
+    // Pass order (null: input order): perm[p] = input row at pass position p, inv_perm the
+    // inverse. Operands, buffers, candidates and thresholds are in pass order; coordinates,
+    // norms and outputs in input order; candidate ids are pass positions until the rerank.
+    const uint32_t* perm;
+    const uint32_t* inv_perm;
+    uint32_t rotate;              // 1: a block's scan starts at its own part of the order
     uint32_t n, d, k, kp, nch, group, cap;
     uint32_t row_begin;           // first flat row of this dataset
     uint32_t prep_begin;          // first flat prepare row (ref_rows per pass dataset)

@@ -322,6 +322,13 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // The reference operand holds whole pairs of tiles (rows n.. are sentinels), so a
     // tile is one contiguous run of tile_u4 16-byte chunks; ranges hold whole pairs.
     const uint32_t t_begin = item.tile_begin, t_end = item.tile_end;
+    // The scan starts item.pad tiles into the range and wraps (a rotation of the order in
+    // which the range's tiles are visited; item.pad is even, so pairs stay pairs).
+    const uint32_t rot = item.pad, span = t_end - t_begin;
+    auto phys = [&](uint32_t t) __attribute__((always_inline)) {
+        const uint32_t u = t + rot;
+        return u >= t_end ? u - span : u;
+    };
     const uint32_t tile_u4 = kTileRefs * row_u4;
     // Two register sets stage tiles t+1 and t+2 while tile t is consumed from LDS, so a
     // tile's loads are issued two tiles before they are stored (HBM/L3 latency exceeds
@@ -342,7 +349,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
 #define SCRUST_LOAD_TILE(P, T)                                                    \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
         if (FIXED || i < loads)                                                   \
-            P[i] = ref_rows[static_cast<size_t>(T) * tile_u4 + tid + 256 * i];   \
+            P[i] = ref_rows[static_cast<size_t>(phys(T)) * tile_u4 + tid + 256 * i];   \
     }
 #define SCRUST_STORE_TILE(P, B)                                                   \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
@@ -549,12 +556,12 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     __syncthreads();
     for (uint32_t t = t_begin; t < t_end; t += 2) {
         SCRUST_LOAD_TILE(pa, t + 2 < last_load ? t + 2 : last_load)
-        consume(t, 0);
+        consume(phys(t), 0);
         SCRUST_STORE_TILE(pb, 1)
         __syncthreads();
         compact();
         SCRUST_LOAD_TILE(pb, t + 3 < last_load ? t + 3 : last_load)
-        consume(t + 1, 1);
+        consume(phys(t + 1), 1);
         SCRUST_STORE_TILE(pa, 0)
         __syncthreads();
         compact();
@@ -588,7 +595,10 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
     item.tile_begin = 2 * item.split * s.chunk_pairs;
     const uint32_t stop = (item.split + 1) * s.chunk_pairs;
     item.tile_end = 2 * (stop < pairs ? stop : pairs);
-    item.pad = 0;
+    // In spatial order, a block's scan starts at the tiles of its own group of 16 blocks
+    // (4096 rows, about one cell), so its rows meet their neighbours first; the 16 blocks
+    // share the start, and its tiles stay in L2 for all of them.
+    item.pad = s.rotate ? (4u * (item.block & ~15u)) % (item.tile_end - item.tile_begin) : 0u;
     pass_block<OP, NCH, true>(s, item, sm);
 }
 

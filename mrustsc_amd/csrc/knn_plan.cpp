@@ -186,6 +186,17 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             }
             s.npad = static_cast<uint32_t>(npad);
             s.splits = splits;
+            // Spatial order (single datasets): automatic from kOrderMinCells cells, or as
+            // the options ask; the scan rotation needs the whole reference set in one range.
+            const uint32_t order_knob = options ? options->order : 0u;
+            if (single && (order_knob == 2u || (order_knob == 0u && sp.n >= kOrderMinCells))) {
+                plan.ordered = true;
+                plan.order = order_layout(sp.n, sp.d);
+                s.perm = static_cast<const uint32_t*>(take(sp.n * sizeof(uint32_t)));
+                s.inv_perm = static_cast<const uint32_t*>(take(sp.n * sizeof(uint32_t)));
+                plan.order_offset = reinterpret_cast<size_t>(take(plan.order.total));
+                s.rotate = splits == 1 ? 1u : 0u;
+            }
             s.select_e = next_pow2(std::max<uint32_t>(2, (splits * s.cap + 63) / 64));
             plan.select_widths |= s.select_e;
             if (plan.operand == kOperandBf16) {
@@ -277,6 +288,10 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
             fix(s.seg_counts);
             fix(s.cand);
             fix(s.theta);
+            if (s.perm) {
+                fix(s.perm);
+                fix(s.inv_perm);
+            }
         }
     }
     char* m = static_cast<char*>(meta);
@@ -324,6 +339,12 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     if (plan.count_words)
         launch_fill_u32(reinterpret_cast<uint32_t*>(static_cast<char*>(plan.arena_base) + plan.counts_offset),
                         plan.count_words, 0u, stream);
+    if (plan.ordered) {
+        const DatasetDesc& s = plan.sets[0];
+        err = order_enqueue(s.x, s.n, s.d, static_cast<char*>(plan.arena_base) + plan.order_offset, plan.order,
+                            const_cast<uint32_t*>(s.perm), const_cast<uint32_t*>(s.inv_perm), stream);
+        if (err != hipSuccess) return hip_failure(err, "launch row order");
+    }
     launch_prepare_all(plan.d_sets, plan.shape, flags, stream);
     if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch prepare");
     if ((err = mark(1)) != hipSuccess) return hip_failure(err, "record event");

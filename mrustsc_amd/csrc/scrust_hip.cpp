@@ -70,6 +70,8 @@ scrust_hip_status validate(const void* embedding, size_t n, size_t d, size_t k,
             return invalid("precision", "bf16, fp32 or exact", std::to_string(options->precision));
         if (options->tile != 0 && options->tile != 128 && options->tile != 256 && options->tile != 512)
             return invalid("tile", "0, 128, 256 or 512", std::to_string(options->tile));
+        if (options->order > 2)
+            return invalid("order", "0 (automatic), 1 (input) or 2 (grouped)", std::to_string(options->order));
     }
     return SCRUST_HIP_OK;
 }

@@ -8,6 +8,7 @@
 
 #include "../../include/scrust_hip.h"
 #include "knn_launch.hpp"
+#include "knn_order.hpp"
 
 namespace scrust_hip {
 
@@ -53,6 +54,10 @@ struct Plan {
     size_t counts_offset = 0;    // their arena offset
     void* arena_base = nullptr;  // set by plan_place
     bool any_pass = false;
+    // Spatial order of a single dataset's pass (knn_order.hip): its workspace's arena offset.
+    bool ordered = false;
+    OrderLayout order{};
+    size_t order_offset = 0;
     uint64_t pass_flops = 0;
     size_t arena_bytes = 0;
     size_t meta_bytes = 0;

@@ -264,3 +264,58 @@ def test_every_candidate_buffer_size_matches_the_oracle(context, precision, tile
     want_ids, want_dist = oracle.knn(x, k)
     np.testing.assert_array_equal(ids, want_ids)
     np.testing.assert_array_equal(dist, want_dist)
+
+
+# --- the pass's row order ---------------------------------------------------------------
+# Rows grouped by nearest pivot (automatic from 65,536 cells, forced here on small inputs)
+# change only the order the pass visits pairs in: every case must still equal the oracle.
+
+
+def grouped_cases():
+    rng = np.random.default_rng(17)
+    dup = rng.normal(size=(600, 12)).astype(np.float32)
+    dup[100:140] = dup[5]
+    dup[300:303] = dup[299]
+    return {
+        "random": (rng.normal(size=(300, 32)).astype(np.float32), 16, "euclidean"),
+        "duplicates": (dup, 15, "euclidean"),
+        "lattice": (synthetic.lattice(8), 6, "euclidean"),
+        "line": (synthetic.line(500), 9, "euclidean"),
+        "clustered": (synthetic.clustered(6000, 50, seed=31), 15, "euclidean"),
+        "d100_k30": (synthetic.clustered(3000, 100, seed=32), 30, "euclidean"),
+        "cosine": (synthetic.clustered(4000, 50, seed=33), 15, "cosine"),
+        "near_ties": (np.array([[0, 0], [1, 1e-4], [1, 1e-5], [-51, -1e-4], [49, -1e-5]],
+                               dtype=np.float32), 2, "euclidean"),
+    }
+
+
+@pytest.mark.parametrize("precision", ("bf16", "fp32"))
+@pytest.mark.parametrize("case", sorted(grouped_cases()))
+def test_grouped_row_order_matches_the_oracle(context, precision, case):
+    x, k, metric = grouped_cases()[case]
+    print(f"[case] grouped order {case} precision={precision}", flush=True)
+    ids, dist = mrustsc_amd.knn_hip(x, k, precision=precision, metric=metric, context=context,
+                                    order="grouped")
+    want_ids, want_dist = oracle.knn(x, k, metric=metric)
+    np.testing.assert_array_equal(ids, want_ids)
+    np.testing.assert_array_equal(dist, want_dist)
+
+
+@pytest.mark.parametrize("precision", ("bf16", "fp32"))
+def test_automatic_order_at_scale_equals_input_order_and_the_oracle(context, precision):
+    # 70,000 cells: the automatic setting groups the rows and rotates each block's scan.
+    x = synthetic.clustered(70000, 50, seed=34)
+    print(f"[case] automatic order n=70000 precision={precision}", flush=True)
+    got = mrustsc_amd.knn_hip(x, 15, precision=precision, context=context)
+    plain = mrustsc_amd.knn_hip(x, 15, precision=precision, context=context, order="input")
+    np.testing.assert_array_equal(got[0], plain[0])
+    np.testing.assert_array_equal(got[1], plain[1])
+    queries = np.random.default_rng(1).choice(x.shape[0], 300, replace=False)
+    want_ids, want_dist = oracle.knn(x, 15, queries=queries)
+    np.testing.assert_array_equal(got[0][queries], want_ids)
+    np.testing.assert_array_equal(got[1][queries], want_dist)
+
+
+def test_unknown_order_is_rejected():
+    with pytest.raises(ValueError, match="order"):
+        mrustsc_amd.knn_hip(np.zeros((10, 2), dtype=np.float32), 3, order="sorted")
