@@ -57,8 +57,9 @@ typedef struct scrust_hip_knn_options {
      * else 128, 256 or 512. Every setting gives the same result. */
     uint32_t tile;
     /* Row order of the candidate pass: 0 = automatic (rows grouped by nearest pivot for
-     * single embeddings of at least 65,536 cells, so that each query block meets its
-     * neighbours early), 1 = input order, 2 = grouped. Every setting gives the same result. */
+     * single embeddings of at least 65,536 cells whose query blocks each scan all references
+     * in one range, so that each block meets its neighbours early), 1 = input order,
+     * 2 = grouped. Every setting gives the same result. */
     uint32_t order;
 } scrust_hip_knn_options;
 

@@ -186,10 +186,12 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             }
             s.npad = static_cast<uint32_t>(npad);
             s.splits = splits;
-            // Spatial order (single datasets): automatic from kOrderMinCells cells, or as
-            // the options ask; the scan rotation needs the whole reference set in one range.
+            // Spatial order (single datasets): automatic from kOrderMinCells cells when a
+            // block scans all references in one range (the scan then starts at the block's
+            // own part of the order; a range that starts elsewhere meets the cells in sorted
+            // order, which appends more than a random order), or as the options ask.
             const uint32_t order_knob = options ? options->order : 0u;
-            if (single && (order_knob == 2u || (order_knob == 0u && sp.n >= kOrderMinCells))) {
+            if (single && (order_knob == 2u || (order_knob == 0u && sp.n >= kOrderMinCells && splits == 1))) {
                 plan.ordered = true;
                 plan.order = order_layout(sp.n, sp.d);
                 s.perm = static_cast<const uint32_t*>(take(sp.n * sizeof(uint32_t)));
