@@ -97,3 +97,17 @@ def test_device_resident_requests_equal_host_requests(context):
             ids, dist, _ = server.wait(t)
             np.testing.assert_array_equal(ids, ids_a)
             np.testing.assert_array_equal(dist, dist_a)
+
+
+def test_a_large_request_in_the_mix_equals_its_input_order_build(context):
+    # 80,000 cells: the server's build groups the rows (automatic order); the result equals
+    # a build in input order.
+    reqs = [(synthetic.clustered(80000, 50, seed=7), 15)] + _requests(6, seed=3)
+    plain = [mrustsc_amd.knn_hip(x, k, context=context, order="input") for x, k in reqs]
+    print("[case] server mix with an 80,000-cell request", flush=True)
+    with mrustsc_amd.Server(slots=4) as server:
+        tickets = [server.submit(x, k) for x, k in reqs]
+        for (ids_a, dist_a), t in zip(plain, tickets):
+            ids, dist, _ = server.wait(t)
+            np.testing.assert_array_equal(ids, ids_a)
+            np.testing.assert_array_equal(dist, dist_a)
