@@ -112,10 +112,10 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
             // prepare_rows_kernel's (coalesced, 64 rows per block).
         } else if (row < s.n) {
             has_norm = true;
-            // Prepare rows are in pass order; coordinates and norms stay in input order.
+            // Prepare rows, coordinates and norms are in pass order; the input in input order.
             const size_t in_row = s.perm && op != kOperandNone ? s.perm[row] : row;
             const float* src = s.x + in_row * d;
-            float* dst = s.coords + in_row * d;
+            float* dst = s.coords + row * d;
             double length = 0.0;
             if (metric == kMetricCosine) {
                 double acc = 0.0;
@@ -133,7 +133,7 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
                 dst[j] = c;
                 nrm = __builtin_fmaf(c, c, nrm);
             }
-            s.norms[in_row] = nrm;
+            s.norms[row] = nrm;
             if (op == kOperandF32) {
                 float4* p4 = reinterpret_cast<float4*>(s.pack_f32 + row * (4 * group));
                 for (int t4 = 0; t4 < group; t4++) {
@@ -246,17 +246,11 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
         if (__ballot(!finite) != 0ull && tid == 0) atomicOr(flags, kFlagNonFinite);
     }
     __syncthreads();
-    SCRUST_GLOBAL float* coords = global_ptr(s.coords);
-    if (perm) {
-        for (uint32_t i = tid; i < rows * d; i += 256) {
-            const uint32_t t = i / d;
-            coords[static_cast<size_t>(perm[r0 + t]) * d + (i - t * d)] = xs[i];
-        }
-        if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[perm[r0 + tid]] = nrm_s[tid];
-    } else {
-        for (uint32_t i = tid; i < rows * d; i += 256) coords[static_cast<size_t>(r0) * d + i] = xs[i];
-        if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[r0 + tid] = nrm_s[tid];
-    }
+    // Coordinates and norms are stored in pass order too (the rerank reads a row's
+    // candidates, which sit near it in that order).
+    SCRUST_GLOBAL float* coords = global_ptr(s.coords) + static_cast<size_t>(r0) * d;
+    for (uint32_t i = tid; i < rows * d; i += 256) coords[i] = xs[i];
+    if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[r0 + tid] = nrm_s[tid];
     // Packed operands (see knn_pass.hip): reference [h | l | h | nh nl nl2 | 0], query
     // [h | h | l | -1/2 x3 | 0], 32 nch features per row, eight per 16-byte chunk.
     const int chunks = 4 * nch;
@@ -347,29 +341,30 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     const uint32_t flat = flat0 < total_rows ? flat0 : total_rows - 1;
     const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
     const bool row_ok = flat0 < total_rows && s.on_pass && s.kp == KP;
-    const uint32_t q = flat - s.row_begin;
+    // Rows are visited in pass order: p indexes coordinates, norms, candidates and
+    // thresholds; q = perm[p] is the input row the output belongs to. Candidate ids are
+    // pass positions; keys carry input ids, so order and ties are the input's.
+    const uint32_t p = flat - s.row_begin;
     const uint32_t n = s.n;
+    const uint32_t q = row_ok && s.perm ? s.perm[p] : p;
     const int d = static_cast<int>(s.d);
-    const float* xq = s.coords + static_cast<size_t>(q) * d;
-    const float nq = s.norms[q];
+    const float* xq = s.coords + static_cast<size_t>(p) * d;
+    const float nq = s.norms[p];
 
-    // Candidates and thresholds are stored in pass order; candidate ids are pass positions.
-    const uint32_t pq = row_ok && s.inv_perm ? s.inv_perm[q] : q;
     uint64_t raw = kNoKey;
-    if (row_ok) raw = s.cand[static_cast<size_t>(pq) * KP + j];
+    if (row_ok) raw = s.cand[static_cast<size_t>(p) * KP + j];
     const bool present = raw != kNoKey;
-    uint32_t id = key_id(raw);
-    if (present && s.perm && id < n) id = s.perm[id];
-    const bool valid = present && id != q && id < n;
+    const uint32_t c = key_id(raw);
+    const bool valid = present && c != p && c < n;
     float dsq = kInf;
     if (valid) {
-        const float* xr = s.coords + static_cast<size_t>(id) * d;
+        const float* xr = s.coords + static_cast<size_t>(c) * d;
         // Rows are 16- or 8-byte aligned when 4 or 2 divides d (the arena is 256-byte aligned).
-        if ((d & 3) == 0) dsq = direct_square_vec<4>(xq, xr, d, nq, s.norms[id]);
-        else if ((d & 1) == 0) dsq = direct_square_vec<2>(xq, xr, d, nq, s.norms[id]);
-        else dsq = direct_square(xq, xr, d, nq, s.norms[id]);
+        if ((d & 3) == 0) dsq = direct_square_vec<4>(xq, xr, d, nq, s.norms[c]);
+        else if ((d & 1) == 0) dsq = direct_square_vec<2>(xq, xr, d, nq, s.norms[c]);
+        else dsq = direct_square(xq, xr, d, nq, s.norms[c]);
     }
-    uint64_t key = valid ? make_key(dsq, id) : kNoKey;
+    uint64_t key = valid ? make_key(dsq, s.perm ? s.perm[c] : c) : kNoKey;
     key = segment_sort<KP>(key, lane);
 
     // Segment-local bookkeeping (every lane of a segment computes the same values).
@@ -394,8 +389,10 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
         const int lo = band_mask ? __builtin_ctzll(band_mask) - seg : 0;
         const int hi = band_mask ? 63 - __builtin_clzll(band_mask) - seg : -1;
         double d64 = 0.0;
-        if (tie && in_band && my_d != 0.0f)
-            d64 = direct_square_f64(xq, s.coords + static_cast<size_t>(my_id) * d, d);
+        if (tie && in_band && my_d != 0.0f) {
+            const uint32_t mp = s.inv_perm ? s.inv_perm[my_id] : my_id;
+            d64 = direct_square_f64(xq, s.coords + static_cast<size_t>(mp) * d, d);
+        }
         int rank = 0;
         for (int i = 0; i < KP; i++) {
             const double od = __shfl(d64, seg + i, kWave);
@@ -424,7 +421,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
                                    2.0 * s.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
                                    s.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
         const double cd = (d + 3) * u * 1.01;
-        const double lower = (static_cast<double>(s.theta[pq]) + nqd - err) * (1.0 - cd);
+        const double lower = (static_cast<double>(s.theta[p]) + nqd - err) * (1.0 - cd);
         const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);
         const double need = (t + tau) > snap_max ? (t + tau) : snap_max;
         certified = lower > need;
@@ -437,8 +434,9 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
                 metric == kMetricCosine ? 0.5f * my_d : __builtin_sqrtf(my_d);
         }
     } else if (j == 0) {
+        // Listed as the input row (the exact kernel works in input rows).
         const uint32_t slot = atomicAdd(fail_count, 1u);
-        fail_list[slot] = flat;
+        fail_list[slot] = s.row_begin + q;
     }
 }
 
@@ -512,19 +510,22 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
         const uint32_t q = flat - s.row_begin;
         const int d = static_cast<int>(s.d), k = static_cast<int>(s.k);
         const uint32_t n = s.n;
-        for (int j = tid; j < d; j += blockDim.x) xq[j] = s.coords[static_cast<size_t>(q) * d + j];
+        // Coordinates and norms of an ordered dataset are in pass order: pq is the row's
+        // pass position, reference r (a pass position) has input id perm[r].
+        const uint32_t pq = s.inv_perm ? s.inv_perm[q] : q;
+        for (int j = tid; j < d; j += blockDim.x) xq[j] = s.coords[static_cast<size_t>(pq) * d + j];
         if (tid == 0) {
             kept = 0;
             theta = kNoKey;
             dropped = 0;
         }
         __syncthreads();
-        const float nq = s.norms[q];
+        const float nq = s.norms[pq];
         for (uint32_t r0 = 0; r0 < n; r0 += blockDim.x) {
             const uint32_t r = r0 + tid;
-            if (r < n && r != q) {
+            if (r < n && r != pq) {
                 const float v = direct_square(xq, s.coords + static_cast<size_t>(r) * d, d, nq, s.norms[r]);
-                const uint64_t key = make_key(v, r);
+                const uint64_t key = make_key(v, s.perm ? s.perm[r] : r);
                 if (key < theta) {
                     const uint32_t slot = atomicAdd(&kept, 1u);
                     keys[slot] = key;
@@ -571,7 +572,8 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
                         const uint64_t key = keys[lo + i];
                         const float v = key_value(key);
                         const uint32_t id = key_id(key);
-                        const double v64 = v == 0.0f ? 0.0 : direct_square_f64(xq, s.coords + static_cast<size_t>(id) * d, d);
+                        const uint32_t at = s.inv_perm ? s.inv_perm[id] : id;
+                        const double v64 = v == 0.0f ? 0.0 : direct_square_f64(xq, s.coords + static_cast<size_t>(at) * d, d);
                         int slot = i;
                         while (slot > 0 && (bd[slot - 1] > v64 || (bd[slot - 1] == v64 && bid[slot - 1] > id))) {
                             bd[slot] = bd[slot - 1];

@@ -49,8 +49,9 @@ struct DatasetDesc {
     uint32_t* out_indices;        // n x k
     float* out_distances;         // n x k
     // Pass order (null: input order): perm[p] = input row at pass position p, inv_perm the
-    // inverse. Operands, buffers, candidates and thresholds are in pass order; coordinates,
-    // norms and outputs in input order; candidate ids are pass positions until the rerank.
+    // inverse. Coordinates, norms, operands, buffers, candidates and thresholds are in pass
+    // order, the input and the outputs in input order; ids are pass positions until the
+    // rerank and the exact kernel key them by input id.
     const uint32_t* perm;
     const uint32_t* inv_perm;
     uint32_t rotate;              // 1: a block's scan starts at its own part of the order
