@@ -30,6 +30,12 @@ namespace {
 constexpr int kTsneTile = 128;  // output rows and columns per workgroup
 constexpr int kTsneKc = 32;     // features per LDS stage
 constexpr int kSumWaves = 4;
+// Mirrored stores go through LDS, 32 columns of a wave's 64 x 64 block at a time, as rows
+// of 64 values at a stride of 68 floats (16-byte aligned; lane groups fall in distinct banks).
+constexpr int kMirrorCols = 32, kMirrorStride = 68;
+constexpr int kTsneLds = 2 * kTsneTile * (kTsneKc + 1) > 4 * kMirrorCols * kMirrorStride
+                             ? 2 * kTsneTile * (kTsneKc + 1)
+                             : 4 * kMirrorCols * kMirrorStride;
 
 // Column sums: a block takes colsum_rows(n) rows of 64 columns; each wave sums every
 // fourth row into its own LDS superaccumulator, the four are combined and added to the
@@ -105,9 +111,13 @@ __global__ __launch_bounds__(256) void tsne_tile_kernel(const float* __restrict_
                                                         const float* __restrict__ norms, uint32_t n,
                                                         uint32_t dpad, float* __restrict__ out) {
     if (blockIdx.x < blockIdx.y) return;
-    // Row stride 33: the 16 rows x 4 features one read touches land in 64 distinct banks.
-    __shared__ float sa[kTsneTile][kTsneKc + 1];
-    __shared__ float sb[kTsneTile][kTsneKc + 1];
+    // One LDS buffer: the two operand stages during the contraction, then the transposed
+    // staging of the mirrored stores. Operand row stride 33: the 16 rows x 4 features one
+    // read touches land in 64 distinct banks.
+    __shared__ float lds[kTsneLds];
+    typedef float Stage[kTsneKc + 1];
+    Stage* sa = reinterpret_cast<Stage*>(lds);
+    Stage* sb = reinterpret_cast<Stage*>(lds + kTsneTile * (kTsneKc + 1));
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1, lr = lane & 15, lk = lane >> 4;
     const uint32_t r0 = blockIdx.y * kTsneTile, c0 = blockIdx.x * kTsneTile;
@@ -198,6 +208,54 @@ __global__ __launch_bounds__(256) void tsne_tile_kernel(const float* __restrict_
         }
     };
     const bool interior = r0 + kTsneTile <= n && c0 + kTsneTile <= n;
+    if (interior && n % 4 == 0 && mirror) {
+        // Direct block: one float4 per (rb, cb). Mirrored block: each half of the wave's
+        // columns is written to LDS transposed (column-major rows of 64), then read back
+        // as float4 runs along the mirrored rows: 16 lanes store one 256-byte row segment.
+        float* T = lds + w * (kMirrorCols * kMirrorStride);
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            const uint32_t row = rbase + rb * 16;
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float sum = nrow[rb] + (r == 0 ? ncol[cb].x : r == 1 ? ncol[cb].y : r == 2 ? ncol[cb].z : ncol[cb].w);
+                    const float e = sum - 2.0f * acc[rb][cb][r];
+                    v[r] = e > 0.0f ? e : 0.0f;
+                    acc[rb][cb][r] = v[r];
+                }
+                *reinterpret_cast<float4*>(out + static_cast<size_t>(row) * n + cbase + cb * 16) =
+                    make_float4(v[0], v[1], v[2], v[3]);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            __syncthreads();
+#pragma unroll
+            for (int cbh = 0; cbh < 2; cbh++) {
+                const int cb = 2 * h + cbh;
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        T[(cbh * 16 + 4 * lk + r) * kMirrorStride + rb * 16 + lr] = acc[rb][cb][r];
+            }
+            __syncthreads();
+            // 32 columns x 16 float4: column c = i / 16, rows 4 (i % 16) .. + 3 of the wave's 64.
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int i = lane + 64 * m;
+                const int c = i >> 4, q4 = i & 15;
+                const float4 val = *reinterpret_cast<const float4*>(T + c * kMirrorStride + 4 * q4);
+                const uint32_t col = c0 + wc * 64 + h * 32 + c;
+                const uint32_t row = r0 + wr * 64 + 4 * q4;
+                *reinterpret_cast<float4*>(out + static_cast<size_t>(col) * n + row) = val;
+            }
+        }
+        return;
+    }
     if (n % 4 == 0) {
         if (interior) finish(std::false_type{}, std::true_type{});
         else finish(std::true_type{}, std::true_type{});
