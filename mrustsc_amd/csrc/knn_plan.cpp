@@ -329,7 +329,8 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
 }
 
 scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
-                               hipStream_t stream, hipEvent_t* events) {
+                               hipStream_t stream, hipEvent_t* events, hipStream_t side,
+                               hipEvent_t fork, hipEvent_t join) {
     auto mark = [&](int i) -> hipError_t { return events ? hipEventRecord(events[i], stream) : hipSuccess; };
     hipError_t err = mark(0);
     if (err != hipSuccess) return hip_failure(err, "record event");
@@ -356,12 +357,23 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
             ok = launch_pass(plan.d_sets, plan.sets[0], plan.operand, stream);
         } else {
             ok = true;
-            for (size_t g = 0; g < plan.groups.size() && ok; g++) {
+            // With a side stream, every class but the first runs on it, in parallel with the
+            // first: a class's persistent workgroups take the slots the other's tail frees.
+            const bool fan = side && fork && join && plan.groups.size() > 1;
+            if (fan) {
+                if ((err = hipEventRecord(fork, stream)) != hipSuccess) return hip_failure(err, "record event");
+                if ((err = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return hip_failure(err, "wait event");
+            }
+            for (size_t g = plan.groups.size(); g-- > 0 && ok;) {
                 const PassGroup& pg = plan.groups[g];
                 const unsigned blocks = static_cast<unsigned>(std::min<size_t>(pg.count, 2ull * compute_units));
                 ok = launch_pass_batch(plan.d_sets, plan.d_items + pg.begin, pg.count,
                                        plan.d_scalars + kScalarItemCounter + g, plan.operand,
-                                       pg.nch_max, blocks, stream);
+                                       pg.nch_max, blocks, fan && g > 0 ? side : stream);
+            }
+            if (fan) {
+                if ((err = hipEventRecord(join, side)) != hipSuccess) return hip_failure(err, "record event");
+                if ((err = hipStreamWaitEvent(stream, join, 0)) != hipSuccess) return hip_failure(err, "wait event");
             }
         }
         if (!ok) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no compiled pass variant for this shape");

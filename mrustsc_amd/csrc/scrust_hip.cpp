@@ -181,6 +181,9 @@ scrust_hip_batch::~scrust_hip_batch() {
     if (context) (void)hipSetDevice(context->device);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,10 +576,15 @@ scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
     if (st != SCRUST_HIP_OK) return bail(st);
     err = hipStreamSynchronize(s);
     if (err != hipSuccess) return bail(hip_failure(err, "upload batch metadata"));
+    // The side stream of the pass's parallel branches (default priority).
+    err = hipStreamCreateWithFlags(&batch->side, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&batch->fork, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&batch->join, hipEventDisableTiming);
+    if (err != hipSuccess) return bail(hip_failure(err, "create batch stream"));
     // Capture the whole pipeline once; runs replay it.
     err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (err != hipSuccess) return bail(hip_failure(err, "begin graph capture"));
-    st = plan_enqueue(plan, false, context->compute_units, s, nullptr);
+    st = plan_enqueue(plan, false, context->compute_units, s, nullptr, batch->side, batch->fork, batch->join);
     hipGraph_t graph = nullptr;
     err = hipStreamEndCapture(s, &graph);
     if (st != SCRUST_HIP_OK) {

@@ -98,7 +98,9 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
 // (one dataset); otherwise the persistent batched pass. Events (6, optional) mark the
 // stage boundaries.
 scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
-                               hipStream_t stream, hipEvent_t* events);
+                               hipStream_t stream, hipEvent_t* events,
+                               hipStream_t side = nullptr, hipEvent_t fork = nullptr,
+                               hipEvent_t join = nullptr);
 
 }  // namespace scrust_hip
 
@@ -121,5 +123,9 @@ struct scrust_hip_batch {
     scrust_hip::Buffer arena, meta;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // A second stream and its fork / join events: the pass launches of different chunk
+    // classes are captured as parallel branches of the graph.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     ~scrust_hip_batch();
 };
