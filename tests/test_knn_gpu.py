@@ -319,3 +319,19 @@ def test_automatic_order_at_scale_equals_input_order_and_the_oracle(context, pre
 def test_unknown_order_is_rejected():
     with pytest.raises(ValueError, match="order"):
         mrustsc_amd.knn_hip(np.zeros((10, 2), dtype=np.float32), 3, order="sorted")
+
+
+def test_four_million_cells_match_the_oracle_on_sampled_rows(context):
+    # Past 2^22 rows: every index computation on the path in 64 bits where it must be
+    # (buffers of npad x cap keys, candidate and coordinate offsets).
+    n = 4_000_000
+    x = synthetic.clustered(n, 50, seed=35)
+    print(f"[case] n={n} d=50 k=15 bf16 (sampled rows)", flush=True)
+    ids, dist, stats = mrustsc_amd.knn_hip(x, 15, context=context, stats=True)
+    print(stats, flush=True)
+    queries = np.random.default_rng(2).choice(n, 40, replace=False)
+    queries = np.concatenate([queries, [0, n - 1]])
+    want_ids, want_dist = oracle.knn(x, 15, queries=queries)
+    np.testing.assert_array_equal(ids[queries], want_ids)
+    np.testing.assert_array_equal(dist[queries], want_dist)
+    assert stats["fallback_rows"] <= n // 1000
