@@ -352,7 +352,11 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
                        "kernel": f"pass_kernel<1, {(3 * d + 3 + 31) // 32}> "
                                  "(bf16 operands, chunks of 32)",
                        "algorithmic_flops": flops,
-                       "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs"}
+                       "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs",
+                       # The MFMA work issued: the hi/lo split's 3 products per feature plus
+                       # the norm terms, padded to chunks of 32, over n x npad pairs.
+                       "executed_tflops": 2.0 * n * ((n + 255) // 256 * 256) * 32 * ((3 * d + 3 + 31) // 32)
+                                          / (dev["pass_ms"] * 1e-3) / 1e12}
     got_ids, got_dist = single.result()
     out["connectivities"] = connectivities_block(single, ctx, not args.no_cpu)
 
