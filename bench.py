@@ -440,29 +440,40 @@ def run_batch_config(args):
         f"{sum(s['n'] for s in specs)} cells")
     cells = sum(s["n"] for s in specs)
     flops = sum(2.0 * s["n"] * s["n"] * s["d"] for s in specs)
-    out = {}
-    with mrustsc_amd.Batch(entries, context=ctx) as batch:
+
+    def time_batch(batch, label):
         for i in range(args.warmup):
             st = batch.run()
-            log(f"[bench] batch warmup {i}: {st}")
+            log(f"[bench] {label} batch warmup {i}: {st}")
         walls, devs = [], []
         for i in range(args.steps):
             t0 = time.perf_counter()
             st = batch.run()
             walls.append(time.perf_counter() - t0)
             devs.append(st["device_ms"])
-            log(f"[bench] batch step {i}: device {st['device_ms']:.2f} ms")
-        sec = sum(walls) / len(walls)
-        # Each dataset against its own standalone run.
-        equal = 0
-        t_single = 0.0
-        for sp, (x, dx), (di, dd) in zip(specs, data, outs):
+            log(f"[bench] {label} batch step {i}: device {st['device_ms']:.2f} ms")
+        return sum(walls) / len(walls), devs
+
+    # The creation shape first, then the batch tuner's choice (it times every shape of the
+    # whole graph and keeps the fastest).
+    with mrustsc_amd.Batch(entries, context=ctx) as batch:
+        untuned_sec, untuned_devs = time_batch(batch, "untuned")
+        base_ids = [di.cpu().numpy().view(np.uint32).copy() for di, _ in outs]
+        base_dist = [dd.cpu().numpy().copy() for _, dd in outs]
+        t0 = time.perf_counter()
+        tuned = batch.tune()
+        tune_sec = time.perf_counter() - t0
+        log(f"[bench] batch tuner: {tuned} in {tune_sec:.1f} s")
+        out = {}
+        sec, devs = time_batch(batch, "tuned")
+        # Each dataset against its own standalone run and the untuned batch.
+        equal = equal_untuned = 0
+        for sp, (x, dx), (di, dd), b_ids, b_dist in zip(specs, data, outs, base_ids, base_dist):
             ids = di.cpu().numpy().view(np.uint32)
             dist = dd.cpu().numpy()
-            s0 = time.perf_counter()
             w_ids, w_dist = mrustsc_amd.knn_hip(x, sp["k"], context=ctx)
-            t_single += time.perf_counter() - s0
             equal += int(np.array_equal(ids, w_ids) and np.array_equal(dist, w_dist))
+            equal_untuned += int(np.array_equal(ids, b_ids) and np.array_equal(dist, b_dist))
         # The same 64 builds as separate device-resident calls, for comparison.
         seq = []
         for _ in range(2):
@@ -480,6 +491,12 @@ def run_batch_config(args):
                          "datasets": 64, "cells": cells}
         out["device_ms_best"] = min(devs)
         out["datasets_equal_to_standalone"] = equal
+        out["datasets_equal_to_untuned"] = equal_untuned
+        out["tuning"] = {**tuned, "seconds": tune_sec, "untuned_ms": untuned_sec * 1e3,
+                         "untuned_device_ms_best": min(untuned_devs),
+                         "tuned_over_untuned": untuned_sec / sec,
+                         "note": "batch tuner (longest range per work item x candidate buffer), "
+                                 "run once before the timed replays, untimed"}
         out["sequential_calls_ms"] = min(seq) * 1e3
         out["batch_over_sequential"] = min(seq) / sec
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,

@@ -53,8 +53,10 @@ typedef struct scrust_hip_context scrust_hip_context;
 typedef struct scrust_hip_knn_options {
     uint32_t metric;    /* scrust_hip_metric */
     uint32_t precision; /* scrust_hip_precision */
-    /* Candidate buffer of the pass, slots per query row: 0 = tuned or default (256),
-     * else 128, 256 or 512. Every setting gives the same result. */
+    /* Candidate buffer of the pass, slots per query row: 0 = automatic (for a single build
+     * the tuning table's work shape for its bucket when the table has one, else 256 slots
+     * with the plan's own reference ranges), else 128, 256 or 512. Every setting gives the
+     * same result. */
     uint32_t tile;
     /* Row order of the candidate pass: 0 = automatic (rows grouped by nearest pivot for
      * single embeddings of at least 65,536 cells whose query blocks each scan all references
@@ -73,7 +75,10 @@ typedef struct scrust_hip_knn_stats {
     uint64_t candidates;      /* candidates kept per row by the pass (0 on the exact path) */
     uint64_t pass_flops;      /* 2 * rows * references * d: algorithmic flops of the pass */
     uint32_t path;            /* scrust_hip_precision actually used */
-    uint32_t tile;            /* tile id used */
+    uint32_t tile;            /* candidate buffer slots per row used */
+    uint32_t splits;          /* reference ranges per query block (0 on the exact path) */
+    uint32_t order;           /* row order of the pass: 1 input, 2 grouped by nearest pivot */
+    uint32_t tuned;           /* 1 when the work shape came from the tuning table */
 } scrust_hip_knn_stats;
 
 /* Message for the last failure on the calling thread ("" if none). */
@@ -189,6 +194,59 @@ scrust_hip_status scrust_hip_tsne_squared_distances_device(scrust_hip_context* c
                                                            float* out);
 
 /* ---------------------------------------------------------------------------------------
+ * Autotuning of the candidate pass's work shape. No counterpart in the reference: its
+ * Metal kernel's shape is fixed by threads_per_query (crates/scrust-gpu/src/kernels/knn.rs:
+ * 116-125). The shape of a pass is its candidate buffer (128, 256 or 512 slots per row and
+ * reference range), the reference ranges each 256-row query block's scan is cut into, and
+ * the row order (input or grouped by nearest pivot). Every shape gives the same graph.
+ * The tuner times the shapes on an embedding, checks that all of them return the same
+ * indices and distances, and records the fastest in a process-wide table keyed by bucket:
+ * path, metric, candidates kept (k <= 15: 32, else 64), packed feature chunks (from d) and
+ * n in half-octaves. Single builds and server requests with tile = 0 take the table's shape
+ * for their bucket. The table is shared by every context and server of the process and can
+ * be saved to and loaded from a text file, so tuning runs once per machine. Batches are
+ * tuned as a whole instead (scrust_hip_batch_tune): their work items share one persistent
+ * launch, so a shape that is fastest for a dataset alone is not the fastest inside one.
+ * ------------------------------------------------------------------------------------- */
+
+typedef struct scrust_hip_tune_result {
+    uint32_t tile, splits, order;                         /* shape chosen */
+    uint32_t default_tile, default_splits, default_order; /* shape of the untuned plan */
+    uint32_t shapes;     /* shapes timed (0: the dataset is on the exact path, nothing to tune) */
+    uint32_t reused;     /* 1: SCRUST_HIP_TUNE_SKIP_KNOWN and the bucket was already tuned */
+    double best_ms;      /* best device time of the chosen shape (ms) */
+    double default_ms;   /* best device time of the untuned shape (ms) */
+    double worst_ms;     /* best device time of the slowest shape timed (ms) */
+} scrust_hip_tune_result;
+
+/* Leave a bucket that the table already holds as it is (no device work). */
+#define SCRUST_HIP_TUNE_SKIP_KNOWN 1u
+
+/* Tunes the bucket of an n x d embedding (device pointer on the context's device) for k
+ * neighbours, with options->metric and options->precision (tile and order are what is
+ * searched; NULL options: Euclidean bf16). Each shape runs once to warm up and check its
+ * result, then three times; the best time counts. Synchronous. */
+scrust_hip_status scrust_hip_tune_device(scrust_hip_context* context, const float* embedding,
+                                         size_t n, size_t d, size_t k,
+                                         const scrust_hip_knn_options* options, uint32_t flags,
+                                         scrust_hip_tune_result* result);
+/* As scrust_hip_tune_device with the embedding in host memory. */
+scrust_hip_status scrust_hip_tune(scrust_hip_context* context, const float* embedding, size_t n,
+                                  size_t d, size_t k, const scrust_hip_knn_options* options,
+                                  uint32_t flags, scrust_hip_tune_result* result);
+/* Writes the table as text (one bucket a line); INVALID_PARAMETER if the file cannot be
+ * written. */
+scrust_hip_status scrust_hip_tune_save(const char* path);
+/* Merges a file written by scrust_hip_tune_save into the table (its buckets replace the
+ * table's); INVALID_PARAMETER, with the line, for an unreadable file or a malformed line,
+ * in which case the table is left unchanged. */
+scrust_hip_status scrust_hip_tune_load(const char* path);
+/* Empties the table: automatic shapes return to the plan's defaults. */
+void scrust_hip_tune_clear(void);
+/* Buckets in the table. */
+size_t scrust_hip_tune_size(void);
+
+/* ---------------------------------------------------------------------------------------
  * Batched builds: many embeddings of uneven n, d and k in one persistent MFMA launch,
  * the whole pipeline captured once in a HIP graph and replayed by scrust_hip_batch_run.
  * No counterpart in the reference (its kNN takes one embedding per call).
@@ -222,6 +280,26 @@ scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
                                           scrust_hip_batch** out);
 /* Replays the graph and waits for it. */
 scrust_hip_status scrust_hip_batch_run(scrust_hip_batch* batch, scrust_hip_batch_stats* stats);
+
+typedef struct scrust_hip_batch_tune_result {
+    uint32_t cut, tile;                 /* shape chosen */
+    uint32_t default_cut, default_tile; /* shape the batch was created with */
+    uint32_t shapes;                    /* shapes timed */
+    uint32_t reserved;
+    double best_ms, default_ms;         /* best replay device time of each (ms) */
+    double worst_ms;                    /* best replay device time of the slowest shape (ms) */
+} scrust_hip_batch_tune_result;
+
+/* Batch autotuner (no counterpart in the reference). Lays the batch out and captures it
+ * again under each shape -- the longest reference range of one work item (1/cut of a
+ * workgroup slot's share of the batch, cut in 1..8) and the candidate buffer of every
+ * dataset (128, 256 or 512 slots, at least 4 x the candidates kept) -- replays each once
+ * to warm up and check that every dataset's indices and distances equal the first shape's,
+ * then three times, and keeps the fastest (the creation shape unless another is at least 2%
+ * faster). Runs the batch's graph 4 times per shape, so the outputs are overwritten; the
+ * inputs must hold valid embeddings. On an error the batch keeps the last shape it built
+ * successfully. */
+scrust_hip_status scrust_hip_batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result);
 void scrust_hip_batch_destroy(scrust_hip_batch* batch);
 
 /* ---------------------------------------------------------------------------------------

@@ -33,6 +33,12 @@ __all__ = [
     "neighbors_device",
     "tsne_squared_distances",
     "tsne_squared_distances_device",
+    "tune",
+    "tune_device",
+    "tune_save",
+    "tune_load",
+    "tune_clear",
+    "tune_size",
     "max_supported_k",
     "Batch",
     "Server",
@@ -311,6 +317,61 @@ def tsne_squared_distances_device(points_ptr: int, n: int, d: int, out_ptr: int,
     _native.raise_for(status)
 
 
+def tune(embedding, k: int, *, metric: str = "euclidean", precision: str = "bf16",
+         skip_known: bool = False, context: Context | None = None) -> dict:
+    """Times every work shape of the candidate pass (buffer 128/256/512 slots, reference
+    ranges per query block, row order) on this embedding (host memory), checks that all of
+    them return the same graph, and records the fastest for its (n, d, k) bucket in the
+    process-wide table that later calls with ``tile=0`` use. Returns the tuning result."""
+    x = _embedding(embedding)
+    if isinstance(k, bool) or int(k) != k or k < 0:
+        raise ValueError(f"k must be a non-negative integer, got {k!r}")
+    opts = _options(metric, precision)
+    handle = _handle_or_none(context)
+    n, d = x.shape
+    res = _native.TuneResult()
+    status = _native.load().scrust_hip_tune(
+        handle, x.ctypes.data, n, d, int(k), ctypes.byref(opts),
+        _native.TUNE_SKIP_KNOWN if skip_known else 0, ctypes.byref(res))
+    _native.raise_for(status)
+    return res.as_dict()
+
+
+def tune_device(embedding_ptr: int, n: int, d: int, k: int, *, metric: str = "euclidean",
+                precision: str = "bf16", skip_known: bool = False,
+                context: Context | None = None) -> dict:
+    """``tune`` on a device-resident (n, d) float32 embedding (a raw device pointer)."""
+    opts = _options(metric, precision)
+    ctx = context if context is not None else _thread_context()
+    res = _native.TuneResult()
+    status = _native.load().scrust_hip_tune_device(
+        ctx.handle, ctypes.c_void_p(embedding_ptr), n, d, k, ctypes.byref(opts),
+        _native.TUNE_SKIP_KNOWN if skip_known else 0, ctypes.byref(res))
+    _native.raise_for(status)
+    return res.as_dict()
+
+
+def tune_save(path) -> None:
+    """Writes the tuning table to a text file (one bucket a line)."""
+    _native.raise_for(_native.load().scrust_hip_tune_save(str(path).encode()))
+
+
+def tune_load(path) -> None:
+    """Merges a table written by ``tune_save``; a malformed file raises ValueError and
+    leaves the table unchanged."""
+    _native.raise_for(_native.load().scrust_hip_tune_load(str(path).encode()))
+
+
+def tune_clear() -> None:
+    """Empties the tuning table (automatic shapes return to the plan's defaults)."""
+    _native.load().scrust_hip_tune_clear()
+
+
+def tune_size() -> int:
+    """Buckets in the tuning table."""
+    return int(_native.load().scrust_hip_tune_size())
+
+
 def _graph_array(a, dtype, name):
     """A 2-D array of `dtype` (convert.rs:70-93: a wrong dtype or rank is a ValueError)."""
     if not isinstance(a, np.ndarray):
@@ -349,6 +410,14 @@ class Batch:
         st = _native.BatchStats()
         _native.raise_for(self._lib.scrust_hip_batch_run(self._handle, ctypes.byref(st)))
         return st.as_dict()
+
+    def tune(self) -> dict:
+        """Times the batch under every work shape (longest reference range per work item,
+        candidate buffer), checks that every dataset's result stays the same, and keeps the
+        fastest graph. Overwrites the outputs (the graph is replayed four times per shape)."""
+        res = _native.BatchTuneResult()
+        _native.raise_for(self._lib.scrust_hip_batch_tune(self._handle, ctypes.byref(res)))
+        return res.as_dict()
 
     def close(self) -> None:
         if getattr(self, "_handle", None) is not None:

@@ -44,8 +44,15 @@ EXPORTS = (
     "scrust_hip_neighbors_device",
     "scrust_hip_tsne_squared_distances",
     "scrust_hip_tsne_squared_distances_device",
+    "scrust_hip_tune",
+    "scrust_hip_tune_device",
+    "scrust_hip_tune_save",
+    "scrust_hip_tune_load",
+    "scrust_hip_tune_clear",
+    "scrust_hip_tune_size",
     "scrust_hip_batch_create",
     "scrust_hip_batch_run",
+    "scrust_hip_batch_tune",
     "scrust_hip_batch_destroy",
     "scrust_hip_server_create",
     "scrust_hip_server_submit",
@@ -77,12 +84,37 @@ class KnnStats(ctypes.Structure):
         ("pass_flops", ctypes.c_uint64),
         ("path", ctypes.c_uint32),
         ("tile", ctypes.c_uint32),
+        ("splits", ctypes.c_uint32),
+        ("order", ctypes.c_uint32),
+        ("tuned", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
         out = {name: getattr(self, name) for name, _ in self._fields_}
         out["path"] = PRECISION_NAMES.get(out["path"], out["path"])
         return out
+
+
+TUNE_SKIP_KNOWN = 1
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("tile", ctypes.c_uint32),
+        ("splits", ctypes.c_uint32),
+        ("order", ctypes.c_uint32),
+        ("default_tile", ctypes.c_uint32),
+        ("default_splits", ctypes.c_uint32),
+        ("default_order", ctypes.c_uint32),
+        ("shapes", ctypes.c_uint32),
+        ("reused", ctypes.c_uint32),
+        ("best_ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+        ("worst_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Dataset(ctypes.Structure):
@@ -108,6 +140,23 @@ class BatchStats(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("cut", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32),
+        ("default_cut", ctypes.c_uint32),
+        ("default_tile", ctypes.c_uint32),
+        ("shapes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("best_ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+        ("worst_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class ServerConfig(ctypes.Structure):
@@ -208,10 +257,25 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_neighbors_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, vp, vp, vp, sz,
                                                 ctypes.POINTER(sz)]
     lib.scrust_hip_neighbors_device.restype = ctypes.c_int
+    tune = ctypes.POINTER(TuneResult)
+    lib.scrust_hip_tune.argtypes = [vp, vp, sz, sz, sz, opts, ctypes.c_uint32, tune]
+    lib.scrust_hip_tune.restype = ctypes.c_int
+    lib.scrust_hip_tune_device.argtypes = [vp, vp, sz, sz, sz, opts, ctypes.c_uint32, tune]
+    lib.scrust_hip_tune_device.restype = ctypes.c_int
+    lib.scrust_hip_tune_save.argtypes = [ctypes.c_char_p]
+    lib.scrust_hip_tune_save.restype = ctypes.c_int
+    lib.scrust_hip_tune_load.argtypes = [ctypes.c_char_p]
+    lib.scrust_hip_tune_load.restype = ctypes.c_int
+    lib.scrust_hip_tune_clear.argtypes = []
+    lib.scrust_hip_tune_clear.restype = None
+    lib.scrust_hip_tune_size.argtypes = []
+    lib.scrust_hip_tune_size.restype = sz
     lib.scrust_hip_batch_create.argtypes = [vp, ctypes.POINTER(Dataset), sz, opts, ctypes.POINTER(vp)]
     lib.scrust_hip_batch_create.restype = ctypes.c_int
     lib.scrust_hip_batch_run.argtypes = [vp, ctypes.POINTER(BatchStats)]
     lib.scrust_hip_batch_run.restype = ctypes.c_int
+    lib.scrust_hip_batch_tune.argtypes = [vp, ctypes.POINTER(BatchTuneResult)]
+    lib.scrust_hip_batch_tune.restype = ctypes.c_int
     lib.scrust_hip_batch_destroy.argtypes = [vp]
     lib.scrust_hip_batch_destroy.restype = None
     lib.scrust_hip_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]

@@ -75,7 +75,7 @@ uint32_t partition_even(uint32_t set, uint32_t blocks, uint32_t pairs, uint32_t 
 
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                               const scrust_hip_knn_options* options, Plan& plan,
-                              int compute_units) {
+                              int compute_units, const TuneEntry* force, const BatchKnobs* knobs) {
     plan = Plan{};
     const int metric = options ? static_cast<int>(options->metric) : kMetricEuclidean;
     const int precision = options ? static_cast<int>(options->precision) : SCRUST_HIP_BF16;
@@ -92,12 +92,18 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
     plan.shape.operand = plan.operand;
 
     // Workgroup slots of the pass (two per CU) and, for batches, the longest reference
-    // range of one item: about a third of a slot's share of the whole batch.
+    // range of one item: by default about a third of a slot's share of the whole batch.
     const uint64_t slots = 2ull * static_cast<uint64_t>(compute_units > 0 ? compute_units : 256);
     uint64_t batch_pairs = 0;
     for (const SetSpec& sp : specs)
         batch_pairs += ((sp.n + kPassRows - 1) / kPassRows) * ((sp.n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost));
-    const uint64_t batch_len = std::max<uint64_t>(kMinPiecePairs, (batch_pairs + 3 * slots - 1) / (3 * slots));
+    const uint64_t cut = knobs && knobs->cut ? knobs->cut : 3u;
+    const uint64_t batch_len = std::max<uint64_t>(kMinPiecePairs, (batch_pairs + cut * slots - 1) / (cut * slots));
+    if (!single && knobs && knobs->cap) {
+        if (knobs->cap != 128 && knobs->cap != 256 && knobs->cap != 512)
+            return invalid("tile", "0, 128, 256 or 512", std::to_string(knobs->cap));
+        cap = knobs->cap;
+    }
 
     size_t arena = 0;
     size_t count_words = 0;  // segment counts of every dataset, one region (one clear)
@@ -113,6 +119,8 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         s.row_begin = row;
         s.prep_begin = prep;
         s.cap = cap;
+        TuneEntry tuned{};
+        bool have_tuned = false;
         const bool pass = plan.operand != kOperandNone && sp.k <= static_cast<size_t>(kMaxPassK) &&
                           sp.d <= static_cast<size_t>(kMaxPassD);
         s.on_pass = pass ? 1u : 0u;
@@ -141,6 +149,19 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                 s.norm_rel = 0.0;
             }
             if (s.nch == 0) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: no pass variant for d = " + std::to_string(sp.d));
+            // Work shape: forced (the tuner), from the tuning table (automatic tile), or the
+            // defaults below.
+            if (force && force->cap) {
+                tuned = *force;
+                have_tuned = true;
+            } else if (!force && single && (!options || options->tile == 0)) {
+                have_tuned = tune_lookup(tune_key(precision, metric, s.kp, sp.n, sp.d), &tuned);
+            }
+            if (have_tuned) {
+                if (tuned.cap != 128 && tuned.cap != 256 && tuned.cap != 512) have_tuned = false;
+                else if (tuned.cap >= 4 * s.kp) s.cap = tuned.cap;
+            }
+            plan.tuned |= have_tuned && !force;
             plan.any_pass = true;
             plan.kp32 |= s.kp == 32;
             plan.kp64 |= s.kp == 64;
@@ -178,6 +199,12 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             const double overhead = 65536.0 / pair_cycles;
             if (single) {
                 splits = choose_splits(blocks, pairs, slots, max_splits, overhead);
+                if (have_tuned && tuned.splits >= 1 && tuned.splits <= max_splits) {
+                    // A tuned count is taken when it cuts the scan into exactly that many
+                    // whole-pair ranges.
+                    const uint32_t len = (pairs + tuned.splits - 1) / tuned.splits;
+                    if ((pairs + len - 1) / len == tuned.splits) splits = tuned.splits;
+                }
                 s.chunk_pairs = (pairs + splits - 1) / splits;
             } else {
                 const uint32_t pieces = static_cast<uint32_t>(std::min<uint64_t>(max_splits, (pairs + batch_len - 1) / batch_len));
@@ -190,7 +217,9 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // block scans all references in one range (the scan then starts at the block's
             // own part of the order; a range that starts elsewhere meets the cells in sorted
             // order, which appends more than a random order), or as the options ask.
-            const uint32_t order_knob = options ? options->order : 0u;
+            uint32_t order_knob = options ? options->order : 0u;
+            if (force && force->cap) order_knob = force->order;
+            else if (have_tuned && order_knob == 0u && (tuned.order == 1u || tuned.order == 2u)) order_knob = tuned.order;
             if (single && (order_knob == 2u || (order_knob == 0u && sp.n >= kOrderMinCells && splits == 1))) {
                 plan.ordered = true;
                 plan.order = order_layout(sp.n, sp.d);

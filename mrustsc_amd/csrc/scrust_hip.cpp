@@ -130,13 +130,13 @@ static scrust_hip_status read_status(const Plan& plan, hipStream_t stream, uint3
     return SCRUST_HIP_OK;
 }
 
-// One embedding already on the device.
 scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
                              const scrust_hip_knn_options* options, uint32_t* out_indices,
-                             float* out_distances, scrust_hip_knn_stats* stats) {
+                             float* out_distances, scrust_hip_knn_stats* stats,
+                             const TuneEntry* force) {
     std::vector<SetSpec> specs{SetSpec{x, n, d, k, out_indices, out_distances}};
     Plan plan;
-    scrust_hip_status st = plan_layout(specs, options, plan, ctx->compute_units);
+    scrust_hip_status st = plan_layout(specs, options, plan, ctx->compute_units, force);
     if (st != SCRUST_HIP_OK) return st;
     SCRUST_CHECK(ctx->arena.reserve(plan.arena_bytes), "allocate workspace");
     SCRUST_CHECK(ctx->meta.reserve(plan.meta_bytes), "allocate metadata");
@@ -159,7 +159,153 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
         stats->pass_flops = plan.pass_flops;
         stats->path = plan.any_pass ? static_cast<uint32_t>(plan.precision) : SCRUST_HIP_EXACT;
         stats->tile = plan.sets[0].cap;
+        stats->splits = plan.any_pass ? plan.sets[0].splits : 0;
+        stats->order = plan.ordered ? 2u : 1u;
+        stats->tuned = plan.tuned ? 1u : 0u;
     }
+    return SCRUST_HIP_OK;
+}
+
+}  // namespace scrust_hip
+
+namespace scrust_hip {
+
+// Lays the batch out under `knobs`, uploads its metadata and captures the whole pipeline in
+// a new graph (replacing the previous one).
+static scrust_hip_status batch_build(scrust_hip_batch* batch, const BatchKnobs& knobs) {
+    scrust_hip_context* context = batch->context;
+    hipStream_t s = context->stream;
+    SCRUST_CHECK(hipStreamSynchronize(s), "wait for the batch");
+    Plan plan;
+    scrust_hip_status st = plan_layout(batch->specs, batch->has_options ? &batch->options : nullptr, plan,
+                                       context->compute_units, nullptr, &knobs);
+    if (st != SCRUST_HIP_OK) return st;
+    if (batch->exec) {
+        (void)hipGraphExecDestroy(batch->exec);
+        batch->exec = nullptr;
+    }
+    if (batch->graph) {
+        (void)hipGraphDestroy(batch->graph);
+        batch->graph = nullptr;
+    }
+    SCRUST_CHECK(batch->arena.reserve(plan.arena_bytes), "allocate batch workspace");
+    SCRUST_CHECK(batch->meta.reserve(plan.meta_bytes), "allocate batch workspace");
+    st = plan_place(plan, batch->arena.ptr, batch->meta.ptr, s);
+    if (st != SCRUST_HIP_OK) return st;
+    SCRUST_CHECK(hipStreamSynchronize(s), "upload batch metadata");
+    // Capture the whole pipeline once; runs replay it.
+    SCRUST_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    st = plan_enqueue(plan, false, context->compute_units, s, nullptr, batch->side, batch->fork, batch->join);
+    hipGraph_t graph = nullptr;
+    hipError_t err = hipStreamEndCapture(s, &graph);
+    if (st != SCRUST_HIP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (err != hipSuccess) return hip_failure(err, "end graph capture");
+    err = hipGraphInstantiate(&batch->exec, graph, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        batch->exec = nullptr;
+        return hip_failure(err, "instantiate graph");
+    }
+    batch->graph = graph;
+    batch->plan = std::move(plan);
+    batch->knobs = knobs;
+    return SCRUST_HIP_OK;
+}
+
+// One replay of the batch's graph, waited for: device time and the exact-row count.
+static scrust_hip_status batch_replay(scrust_hip_batch* batch, double* ms, uint32_t* listed) {
+    scrust_hip_context* ctx = batch->context;
+    if (!batch->exec) return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: the batch has no captured graph");
+    hipStream_t s = ctx->stream;
+    SCRUST_CHECK(hipEventRecord(ctx->events[0], s), "record event");
+    SCRUST_CHECK(hipGraphLaunch(batch->exec, s), "launch graph");
+    SCRUST_CHECK(hipEventRecord(ctx->events[1], s), "record event");
+    scrust_hip_status st = read_status(batch->plan, s, listed);
+    if (st != SCRUST_HIP_OK) return st;
+    *ms = event_ms(ctx->events[0], ctx->events[1]);
+    return SCRUST_HIP_OK;
+}
+
+// Copies every dataset's outputs to `ids` / `dist` (concatenated).
+static scrust_hip_status batch_outputs(scrust_hip_batch* batch, std::vector<uint32_t>& ids, std::vector<float>& dist) {
+    size_t total = 0;
+    for (const SetSpec& sp : batch->specs) total += sp.n * sp.k;
+    ids.resize(total);
+    dist.resize(total);
+    hipStream_t s = batch->context->stream;
+    size_t at = 0;
+    for (const SetSpec& sp : batch->specs) {
+        SCRUST_CHECK(hipMemcpyAsync(ids.data() + at, sp.out_indices, sp.n * sp.k * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, s),
+                     "copy batch output");
+        SCRUST_CHECK(hipMemcpyAsync(dist.data() + at, sp.out_distances, sp.n * sp.k * sizeof(float),
+                                    hipMemcpyDeviceToHost, s),
+                     "copy batch output");
+        at += sp.n * sp.k;
+    }
+    SCRUST_CHECK(hipStreamSynchronize(s), "copy batch output");
+    return SCRUST_HIP_OK;
+}
+
+// The batch tuner: every (cut, buffer) shape, each checked against the first shape's
+// outputs and timed over three replays; the fastest is kept (the creation shape unless
+// another is at least 2% faster).
+static scrust_hip_status batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result) {
+    constexpr uint32_t kRepeats = 3;
+    constexpr double kMinGain = 0.98;
+    *result = scrust_hip_batch_tune_result{};
+    const BatchKnobs created = batch->knobs;
+    const uint32_t created_cap = batch->has_options && batch->options.tile ? batch->options.tile : 256u;
+    result->default_cut = created.cut ? created.cut : 3u;
+    result->default_tile = created.cap ? created.cap : created_cap;
+    std::vector<BatchKnobs> shapes;
+    shapes.push_back(BatchKnobs{result->default_cut, result->default_tile});
+    for (uint32_t cut : {1u, 2u, 3u, 4u, 6u, 8u})
+        for (uint32_t cap : {128u, 256u, 512u})
+            if (cut != shapes[0].cut || cap != shapes[0].cap) shapes.push_back(BatchKnobs{cut, cap});
+    std::vector<uint32_t> ref_ids, ids;
+    std::vector<float> ref_dist, dist;
+    BatchKnobs best = shapes[0];
+    double best_ms = 0.0, default_ms = 0.0;
+    for (size_t i = 0; i < shapes.size(); i++) {
+        scrust_hip_status st = batch_build(batch, shapes[i]);
+        if (st != SCRUST_HIP_OK) return st;
+        double ms = 0.0;
+        uint32_t listed = 0;
+        st = batch_replay(batch, &ms, &listed);
+        if (st != SCRUST_HIP_OK) return st;
+        st = batch_outputs(batch, i == 0 ? ref_ids : ids, i == 0 ? ref_dist : dist);
+        if (st != SCRUST_HIP_OK) return st;
+        if (i > 0 && (ids != ref_ids || std::memcmp(dist.data(), ref_dist.data(), dist.size() * sizeof(float)) != 0))
+            return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: batch shape cut " + std::to_string(shapes[i].cut) +
+                                               ", tile " + std::to_string(shapes[i].cap) + " changed a graph");
+        double fastest = 0.0;
+        for (uint32_t r = 0; r < kRepeats; r++) {
+            st = batch_replay(batch, &ms, &listed);
+            if (st != SCRUST_HIP_OK) return st;
+            if (r == 0 || ms < fastest) fastest = ms;
+        }
+        result->shapes++;
+        result->worst_ms = std::max(result->worst_ms, fastest);
+        if (i == 0) default_ms = best_ms = fastest;
+        else if (fastest < best_ms) {
+            best = shapes[i];
+            best_ms = fastest;
+        }
+    }
+    if (!(best_ms < default_ms * kMinGain)) {
+        best = shapes[0];
+        best_ms = default_ms;
+    }
+    scrust_hip_status st = batch_build(batch, best);
+    if (st != SCRUST_HIP_OK) return st;
+    result->cut = best.cut;
+    result->tile = best.cap;
+    result->best_ms = best_ms;
+    result->default_ms = default_ms;
     return SCRUST_HIP_OK;
 }
 
@@ -561,40 +707,22 @@ scrust_hip_status scrust_hip_batch_create(scrust_hip_context* context,
     scrust_hip_batch* batch = new (std::nothrow) scrust_hip_batch();
     if (!batch) return fail(SCRUST_HIP_OUT_OF_MEMORY, "host allocation failed");
     batch->context = context;
-    auto bail = [&](scrust_hip_status st) {
-        delete batch;
-        return st;
-    };
-    scrust_hip_status st = plan_layout(specs, options, batch->plan, context->compute_units);
-    if (st != SCRUST_HIP_OK) return bail(st);
-    Plan& plan = batch->plan;
-    hipError_t err = batch->arena.reserve(plan.arena_bytes);
-    if (err == hipSuccess) err = batch->meta.reserve(plan.meta_bytes);
-    if (err != hipSuccess) return bail(hip_failure(err, "allocate batch workspace"));
-    hipStream_t s = context->stream;
-    st = plan_place(plan, batch->arena.ptr, batch->meta.ptr, s);
-    if (st != SCRUST_HIP_OK) return bail(st);
-    err = hipStreamSynchronize(s);
-    if (err != hipSuccess) return bail(hip_failure(err, "upload batch metadata"));
+    batch->specs = std::move(specs);
+    batch->has_options = options != nullptr;
+    if (options) batch->options = *options;
     // The side stream of the pass's parallel branches (default priority).
-    err = hipStreamCreateWithFlags(&batch->side, hipStreamNonBlocking);
+    hipError_t err = hipStreamCreateWithFlags(&batch->side, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipEventCreateWithFlags(&batch->fork, hipEventDisableTiming);
     if (err == hipSuccess) err = hipEventCreateWithFlags(&batch->join, hipEventDisableTiming);
-    if (err != hipSuccess) return bail(hip_failure(err, "create batch stream"));
-    // Capture the whole pipeline once; runs replay it.
-    err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (err != hipSuccess) return bail(hip_failure(err, "begin graph capture"));
-    st = plan_enqueue(plan, false, context->compute_units, s, nullptr, batch->side, batch->fork, batch->join);
-    hipGraph_t graph = nullptr;
-    err = hipStreamEndCapture(s, &graph);
-    if (st != SCRUST_HIP_OK) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return bail(st);
+    if (err != hipSuccess) {
+        delete batch;
+        return hip_failure(err, "create batch stream");
     }
-    if (err != hipSuccess) return bail(hip_failure(err, "end graph capture"));
-    batch->graph = graph;
-    err = hipGraphInstantiate(&batch->exec, graph, nullptr, nullptr, 0);
-    if (err != hipSuccess) return bail(hip_failure(err, "instantiate graph"));
+    scrust_hip_status st = batch_build(batch, BatchKnobs{0, 0});
+    if (st != SCRUST_HIP_OK) {
+        delete batch;
+        return st;
+    }
     *out = batch;
     return SCRUST_HIP_OK;
 }
@@ -605,16 +733,13 @@ scrust_hip_status scrust_hip_batch_run(scrust_hip_batch* batch, scrust_hip_batch
     scrust_hip_context* ctx = batch->context;
     if (hipSetDevice(ctx->device) != hipSuccess)
         return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(ctx->device));
-    hipStream_t s = ctx->stream;
-    SCRUST_CHECK(hipEventRecord(ctx->events[0], s), "record event");
-    SCRUST_CHECK(hipGraphLaunch(batch->exec, s), "launch graph");
-    SCRUST_CHECK(hipEventRecord(ctx->events[1], s), "record event");
+    double ms = 0.0;
     uint32_t listed = 0;
-    scrust_hip_status st = read_status(batch->plan, s, &listed);
+    scrust_hip_status st = batch_replay(batch, &ms, &listed);
     if (st != SCRUST_HIP_OK) return st;
     if (stats) {
         const Plan& plan = batch->plan;
-        stats->device_ms = event_ms(ctx->events[0], ctx->events[1]);
+        stats->device_ms = ms;
         stats->datasets = plan.sets.size();
         stats->pass_items = plan.items.size();
         stats->exact_rows = listed;
@@ -622,6 +747,16 @@ scrust_hip_status scrust_hip_batch_run(scrust_hip_batch* batch, scrust_hip_batch
         stats->cells = plan.shape.total_rows;
     }
     return SCRUST_HIP_OK;
+}
+
+scrust_hip_status scrust_hip_batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result) {
+    clear_error();
+    if (!batch) return invalid("batch", "a non-null pointer", "null");
+    if (!result) return invalid("result", "a non-null pointer", "null");
+    scrust_hip_context* ctx = batch->context;
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return fail(SCRUST_HIP_NO_GPU, "cannot select HIP device " + std::to_string(ctx->device));
+    return batch_tune(batch, result);
 }
 
 void scrust_hip_batch_destroy(scrust_hip_batch* batch) { delete batch; }

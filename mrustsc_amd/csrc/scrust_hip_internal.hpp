@@ -38,6 +38,35 @@ struct PassGroup {
     uint32_t begin, count;
 };
 
+// Work shape of one dataset's candidate pass. Every shape gives the same graph; the
+// autotuner (knn_tune.cpp) times them and keeps the fastest per bucket.
+struct TuneEntry {
+    uint32_t cap;     // candidate buffer slots per row and reference range: 128, 256 or 512
+    uint32_t splits;  // reference ranges per query block
+    uint32_t order;   // 1 input order, 2 rows grouped by nearest pivot
+    float ms;         // device time of the tuning call that chose it
+};
+
+// Bucket of the tuning table: path, metric, candidates kept, packed feature chunks (from d)
+// and n in half-octaves.
+struct TuneKey {
+    uint32_t precision, metric, kp, nch, nbin;
+    bool operator<(const TuneKey& o) const;
+};
+
+// The bucket of an (n, d) dataset keeping kp candidates per row on the given path.
+TuneKey tune_key(int precision, int metric, uint32_t kp, size_t n, size_t d);
+bool tune_lookup(const TuneKey& key, TuneEntry* out);
+
+// Work shape of a batch as a whole (the batch tuner's search space): `cut` sets the longest
+// reference range of one work item to 1/cut of a workgroup slot's share of the batch (0: 3),
+// `cap` the candidate buffer of every dataset (0: the options' tile, else 256; at least
+// 4 kp either way).
+struct BatchKnobs {
+    uint32_t cut;
+    uint32_t cap;
+};
+
 // Device placement and launch parameters of a set of embeddings.
 struct Plan {
     std::vector<DatasetDesc> sets;  // device pointers filled in
@@ -54,6 +83,7 @@ struct Plan {
     size_t counts_offset = 0;    // their arena offset
     void* arena_base = nullptr;  // set by plan_place
     bool any_pass = false;
+    bool tuned = false;  // a dataset's work shape came from the tuning table
     // Spatial order of a single dataset's pass (knn_order.hip): its workspace's arena offset.
     bool ordered = false;
     OrderLayout order{};
@@ -84,10 +114,14 @@ const char* last_error();
 void clear_error();
 bool device_is_gfx950(int device, hipDeviceProp_t* props_out);
 
-// Lays out a plan for `specs` (no device work) for a device of `compute_units` CUs.
+// Lays out a plan for `specs` (no device work) for a device of `compute_units` CUs. With
+// `force`, a single dataset's pass takes that work shape (cap 0: the untuned defaults,
+// table ignored); otherwise a single dataset with an automatic tile looks its bucket up in
+// the tuning table. Batches take their shape from `knobs` (null: the defaults).
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                               const scrust_hip_knn_options* options, Plan& plan,
-                              int compute_units);
+                              int compute_units, const TuneEntry* force = nullptr,
+                              const BatchKnobs* knobs = nullptr);
 // Points the plan's descriptors into `arena` and `meta` (sized arena_bytes / meta_bytes)
 // and uploads descriptors, items and the exact-row list on `stream`. With `pinned` (host
 // memory of at least meta_bytes that stays untouched until the copy completes) the upload
@@ -101,6 +135,12 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
                                hipStream_t stream, hipEvent_t* events,
                                hipStream_t side = nullptr, hipEvent_t fork = nullptr,
                                hipEvent_t join = nullptr);
+
+// One embedding already on the device, synchronously (the body of scrust_hip_knn_device).
+scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
+                             const scrust_hip_knn_options* options, uint32_t* out_indices,
+                             float* out_distances, scrust_hip_knn_stats* stats,
+                             const TuneEntry* force = nullptr);
 
 }  // namespace scrust_hip
 
@@ -119,6 +159,11 @@ struct scrust_hip_context {
 
 struct scrust_hip_batch {
     scrust_hip_context* context = nullptr;
+    // What the batch was created from, kept so the batch tuner can lay it out again.
+    std::vector<scrust_hip::SetSpec> specs;
+    scrust_hip_knn_options options{};
+    bool has_options = false;
+    scrust_hip::BatchKnobs knobs{0, 0};
     scrust_hip::Plan plan;
     scrust_hip::Buffer arena, meta;
     hipGraph_t graph = nullptr;
