@@ -58,6 +58,7 @@ def parse_args():
     p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
     p.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
+    p.add_argument("--slots", type=int, default=4, help="c5: builds the server runs at once")
     p.add_argument("--gemm-ceiling", action="store_true",
                    help="time the platform's bf16 GEMM (torch / hipBLASLt) as a practical MFMA ceiling")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -530,7 +531,7 @@ def run_serving_config(args):
         t = resident[i]
         return server.submit_device(t.data_ptr(), t.shape[0], t.shape[1], 15, keep_alive=t)
     out = base_line(args, 0.0, 0.0)
-    with mrustsc_amd.Server(slots=4) as server:
+    with mrustsc_amd.Server(slots=args.slots) as server:
         # Saturation: a closed burst of requests, all submitted at once.
         order = rng.integers(0, len(pool), size=64)
         t0 = time.perf_counter()
@@ -572,7 +573,7 @@ def run_serving_config(args):
                 "p99_ms": mid["p99_ms"], "p50_ms": mid["p50_ms"],
                 "saturation_rps": sat, "loads": loads, "server": stats,
                 "config": {"workload": "Poisson arrivals, n log-uniform 1,000-200,000, d=50, "
-                                       "k=15, device-resident embeddings, results to host, 4 slots",
+                                       f"k=15, device-resident embeddings, results to host, {args.slots} slots",
                            "pool": len(pool)}})
     return out
 
