@@ -297,8 +297,8 @@ typedef struct scrust_hip_batch_tune_result {
  * to warm up and check that every dataset's indices and distances equal the first shape's,
  * then three times, and keeps the fastest (the creation shape unless another is at least 2%
  * faster). Runs the batch's graph 4 times per shape, so the outputs are overwritten; the
- * inputs must hold valid embeddings. On an error the batch keeps the last shape it built
- * successfully. */
+ * inputs must hold valid embeddings. On an error the batch is built again with its
+ * creation shape (if that fails too, scrust_hip_batch_run reports KERNEL). */
 scrust_hip_status scrust_hip_batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result);
 void scrust_hip_batch_destroy(scrust_hip_batch* batch);
 

@@ -253,7 +253,7 @@ static scrust_hip_status batch_outputs(scrust_hip_batch* batch, std::vector<uint
 // The batch tuner: every (cut, buffer) shape, each checked against the first shape's
 // outputs and timed over three replays; the fastest is kept (the creation shape unless
 // another is at least 2% faster).
-static scrust_hip_status batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result) {
+static scrust_hip_status batch_tune_shapes(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result) {
     constexpr uint32_t kRepeats = 3;
     constexpr double kMinGain = 0.98;
     *result = scrust_hip_batch_tune_result{};
@@ -307,6 +307,17 @@ static scrust_hip_status batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tu
     result->best_ms = best_ms;
     result->default_ms = default_ms;
     return SCRUST_HIP_OK;
+}
+
+// A failed tuning run leaves the batch with its creation shape again (when that can still be
+// built) and reports the first error.
+static scrust_hip_status batch_tune(scrust_hip_batch* batch, scrust_hip_batch_tune_result* result) {
+    const BatchKnobs created = batch->knobs;
+    scrust_hip_status st = batch_tune_shapes(batch, result);
+    if (st == SCRUST_HIP_OK) return st;
+    const std::string message = last_error();
+    (void)batch_build(batch, created);
+    return fail(st, message);
 }
 
 }  // namespace scrust_hip
