@@ -79,6 +79,8 @@ typedef struct scrust_hip_knn_stats {
     uint32_t splits;          /* reference ranges per query block (0 on the exact path) */
     uint32_t order;           /* row order of the pass: 1 input, 2 grouped by nearest pivot */
     uint32_t tuned;           /* 1 when the work shape came from the tuning table */
+    uint32_t tail_blocks;     /* query blocks of the launch's last round scanned in ranges */
+    uint32_t tail_splits;     /* their reference ranges (0 when tail_blocks is 0) */
 } scrust_hip_knn_stats;
 
 /* Message for the last failure on the calling thread ("" if none). */

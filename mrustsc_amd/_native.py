@@ -87,6 +87,8 @@ class KnnStats(ctypes.Structure):
         ("splits", ctypes.c_uint32),
         ("order", ctypes.c_uint32),
         ("tuned", ctypes.c_uint32),
+        ("tail_blocks", ctypes.c_uint32),
+        ("tail_splits", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:

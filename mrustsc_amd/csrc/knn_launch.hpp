@@ -41,8 +41,8 @@ struct DatasetDesc {
     uint16_t* pack_ref;           // bf16 reference operands, ref_rows x 32 nch
     uint16_t* pack_query;         // bf16 query operands, n x 32 nch
     float* pack_f32;              // fp32 operands, ref_rows x 4 group
-    uint64_t* buf;                // pass candidate buffers, splits x npad x cap
-    uint32_t* seg_counts;         // keys per buffer segment (4 x u8), splits x npad
+    uint64_t* buf;                // pass candidate buffers, range_rows() x cap
+    uint32_t* seg_counts;         // keys per buffer segment (4 x u8), range_rows()
     uint64_t* cand;               // kept candidates, n x kp
     float* theta;                 // kp-th candidate score per row, n
     unsigned* max_norm_bits;      // max squared norm (as bits)
@@ -60,14 +60,32 @@ struct DatasetDesc {
     uint32_t prep_begin;          // first flat prepare row (ref_rows per pass dataset)
     uint32_t on_pass;             // 1: candidate pass + rerank; 0: exact rows only
     uint32_t npad;                // n rounded up to whole query blocks
-    uint32_t splits;              // buffer slots per row (reference ranges of one block)
+    uint32_t splits;              // reference ranges of a block's scan (buffer slots per row)
     uint32_t chunk_pairs;         // tile pairs per reference range (the last may be short)
     uint32_t select_e;            // keys per lane of the final selection (splits x cap / 64)
+    // Single datasets: rows from tail_row0 on (whole query blocks at the end of the launch)
+    // scan in `splits` ranges, the rows before it in one range, whose selection takes
+    // select_e1 keys per lane. 0: every block scans in `splits` ranges.
+    uint32_t tail_row0;
+    uint32_t select_e1;
     // Error model of the pass (see the certificate in rerank_kernel).
     double split_rel;  // operand split error of a.b, relative to |a||b|
     double acc_rel;    // MFMA accumulation error, relative to the sum of |terms|
     double norm_rel;   // error of the |b|^2 feature, relative to |b|^2
 };
+
+// Row offset of reference range j's buffers and segment counts: range 0 holds every row,
+// a range j >= 1 only the rows from tail_row0 on (every row when tail_row0 is 0), so row r
+// of range j sits at range_row_base(s, j) + r.
+__host__ __device__ inline size_t range_row_base(const DatasetDesc& s, uint32_t j) {
+    return j == 0 ? 0
+                  : static_cast<size_t>(s.npad) + static_cast<size_t>(j - 1) * (s.npad - s.tail_row0) - s.tail_row0;
+}
+
+// Rows of buffers and counts over all ranges.
+__host__ __device__ inline size_t range_rows(const DatasetDesc& s) {
+    return static_cast<size_t>(s.npad) + static_cast<size_t>(s.splits - 1) * (s.npad - s.tail_row0);
+}
 
 // One pass work item: a query block against one reference range of its dataset.
 struct PassItem {

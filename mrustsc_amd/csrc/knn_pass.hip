@@ -281,9 +281,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     const uint32_t qbase = item.block * kPassRows;
     const SCRUST_GLOBAL u32x4* ref_rows = global_ptr(reinterpret_cast<const u32x4*>(
         OP == kOperandBf16 ? static_cast<const void*>(s.pack_ref) : static_cast<const void*>(s.pack_f32)));
-    // This range's slot of the row buffers.
-    SCRUST_GLOBAL uint64_t* const buf =
-        global_ptr(s.buf) + static_cast<size_t>(item.split) * s.npad * s.cap;
+    // This range's slot of the row buffers (indexed by the dataset's row).
+    SCRUST_GLOBAL uint64_t* const buf = global_ptr(s.buf) + range_row_base(s, item.split) * s.cap;
 
     // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
     // qb's fragment in lane l carries row qb*16 + l%16.
@@ -322,12 +321,15 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // The reference operand holds whole pairs of tiles (rows n.. are sentinels), so a
     // tile is one contiguous run of tile_u4 16-byte chunks; ranges hold whole pairs.
     const uint32_t t_begin = item.tile_begin, t_end = item.tile_end;
-    // The scan starts item.pad tiles into the range and wraps (a rotation of the order in
-    // which the range's tiles are visited; item.pad is even, so pairs stay pairs).
-    const uint32_t rot = item.pad, span = t_end - t_begin;
+    // With a rotation (item.pad > 0, spatial order) the scan visits tile (t + pad) mod the
+    // dataset's tile count for t in [t_begin, t_end): a block's ranges cut the rotated
+    // sequence, which starts at the block's own part of the order. item.pad and the tile
+    // count are even, so pairs stay pairs. Without one, tile t itself.
+    const uint32_t rot = item.pad;
+    const uint32_t wrap = rot ? 2u * ((n + 2 * kTileRefs - 1) / (2 * kTileRefs)) : t_end;
     auto phys = [&](uint32_t t) __attribute__((always_inline)) {
         const uint32_t u = t + rot;
-        return u >= t_end ? u - span : u;
+        return u >= wrap ? u - wrap : u;
     };
     const uint32_t tile_u4 = kTileRefs * row_u4;
     // Two register sets stage tiles t+1 and t+2 while tile t is consumed from LDS, so a
@@ -570,7 +572,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
 #undef SCRUST_STORE_TILE
     // Segment counts for the final selection (select_kernel): byte g of the row's word.
     SCRUST_GLOBAL uint8_t* const counts = reinterpret_cast<SCRUST_GLOBAL uint8_t*>(
-        global_ptr(s.seg_counts) + static_cast<size_t>(item.split) * s.npad);
+        global_ptr(s.seg_counts) + range_row_base(s, item.split));
 #pragma unroll
     for (int qb = 0; qb < 4; qb++)
         counts[static_cast<size_t>(qbase + w * kWave + qb * 16 + lc) * 4 + g] =
@@ -584,21 +586,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
 void pass_kernel(const DatasetDesc* __restrict__ set) {
     __shared__ PassSmem<OP, NCH> sm;
     const DatasetDesc& s = *set;
-    // Workgroup g scans range j = g / blocks of query block g % blocks: the workgroups
-    // resident together scan the same references (shared through L2).
+    // Workgroups below the tail block scan their block's whole range; from it on, workgroup
+    // tail_block + g scans range j = g / tail_blocks of block tail_block + g % tail_blocks:
+    // the workgroups resident together scan the same references (shared through L2).
     const uint32_t pairs = (s.n + 2 * kTileRefs - 1) / (2 * kTileRefs);
     const uint32_t blocks = s.npad / kPassRows;
+    const uint32_t tail_block = s.tail_row0 / kPassRows;
     PassItem item;
     item.set = 0;
-    item.split = blockIdx.x / blocks;
-    item.block = blockIdx.x - item.split * blocks;
-    item.tile_begin = 2 * item.split * s.chunk_pairs;
-    const uint32_t stop = (item.split + 1) * s.chunk_pairs;
-    item.tile_end = 2 * (stop < pairs ? stop : pairs);
+    if (blockIdx.x < tail_block) {
+        item.split = 0;
+        item.block = blockIdx.x;
+        item.tile_begin = 0;
+        item.tile_end = 2 * pairs;
+    } else {
+        const uint32_t g = blockIdx.x - tail_block, tail_blocks = blocks - tail_block;
+        item.split = g / tail_blocks;
+        item.block = tail_block + (g - item.split * tail_blocks);
+        item.tile_begin = 2 * item.split * s.chunk_pairs;
+        const uint32_t stop = (item.split + 1) * s.chunk_pairs;
+        item.tile_end = 2 * (stop < pairs ? stop : pairs);
+    }
     // In spatial order, a block's scan starts at the tiles of its own group of 16 blocks
     // (4096 rows, about one cell), so its rows meet their neighbours first; the 16 blocks
     // share the start, and its tiles stay in L2 for all of them.
-    item.pad = s.rotate ? (4u * (item.block & ~15u)) % (item.tile_end - item.tile_begin) : 0u;
+    item.pad = s.rotate ? (4u * (item.block & ~15u)) % (2 * pairs) : 0u;
     pass_block<OP, NCH, true>(s, item, sm);
 }
 
@@ -651,9 +663,10 @@ __global__ __launch_bounds__(256) void select_kernel(const DatasetDesc* __restri
     const int lane = threadIdx.x & 63;
     if (flat >= total_rows) return;
     const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
-    if (!s.on_pass || s.select_e != E) return;
     const uint32_t row = flat - s.row_begin;
-    const uint32_t cap = s.cap, seg = cap / 4, npad = s.npad, splits = s.splits;
+    const bool whole = row < s.tail_row0;  // a row whose block scanned in one range
+    if (!s.on_pass || (whole ? s.select_e1 : s.select_e) != E) return;
+    const uint32_t cap = s.cap, seg = cap / 4, splits = whole ? 1u : s.splits;
     const int kp = static_cast<int>(s.kp);
     const SCRUST_GLOBAL uint64_t* buf = global_ptr(s.buf);
     const SCRUST_GLOBAL uint32_t* counts = global_ptr(s.seg_counts);
@@ -665,7 +678,7 @@ __global__ __launch_bounds__(256) void select_kernel(const DatasetDesc* __restri
         const uint32_t j = e / cap, p = e % cap;
         uint64_t k = kNoKey;
         if (j < splits) {
-            const size_t at = static_cast<size_t>(j) * npad + row;
+            const size_t at = range_row_base(s, j) + row;
             const uint32_t c = (counts[at] >> (8 * (p / seg))) & 0xFFu;
             if (p % seg < c) k = buf[at * cap + p];
         }
@@ -743,7 +756,8 @@ int batch_chunks(int operand, int nch) {
 
 template <int OP, int NCH>
 static void run_pass(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream) {
-    const unsigned groups = host.npad / kPassRows * host.splits;
+    const unsigned tail_block = host.tail_row0 / kPassRows;
+    const unsigned groups = tail_block + (host.npad / kPassRows - tail_block) * host.splits;
     hipLaunchKernelGGL((pass_kernel<OP, NCH>), dim3(groups), dim3(256), 0, stream, set);
 }
 

@@ -26,6 +26,7 @@ size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 constexpr uint32_t kMinPiecePairs = 4;   // shortest reference range of a work item (tile pairs)
 constexpr uint32_t kSelectKeysMax = 1024;  // keys per row the final selection takes (16 per lane)
+constexpr uint32_t kTailSplits = 3;        // reference ranges of the blocks of a launch's last round
 
 uint32_t next_pow2(uint32_t v) {
     uint32_t p = 1;
@@ -228,6 +229,24 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                 plan.order_offset = reinterpret_cast<size_t>(take(plan.order.total));
                 s.rotate = splits == 1 ? 1u : 0u;
             }
+            // The launch's last round (single datasets scanned in one range): when it would
+            // fill more than half of the workgroup slots, some CUs run two of its workgroups
+            // and the others one, and the round lasts as long as a full one. The last
+            // blocks % slots blocks then scan in kTailSplits ranges (of the rotated sequence in
+            // spatial order), which spreads that round's work over every slot.
+            const uint64_t rem = blocks % slots;
+            const uint32_t tail_splits = std::min(kTailSplits, max_splits);
+            if (single && splits == 1 && blocks > slots && rem > slots / 2 && tail_splits > 1) {
+                const uint32_t len = (pairs + tail_splits - 1) / tail_splits;
+                if ((pairs + len - 1) / len == tail_splits) {
+                    splits = tail_splits;
+                    s.splits = splits;
+                    s.chunk_pairs = len;
+                    s.tail_row0 = static_cast<uint32_t>((blocks - rem) * kPassRows);
+                    s.select_e1 = next_pow2(std::max<uint32_t>(2, (s.cap + 63) / 64));
+                    plan.select_widths |= s.select_e1;
+                }
+            }
             s.select_e = next_pow2(std::max<uint32_t>(2, (splits * s.cap + 63) / 64));
             plan.select_widths |= s.select_e;
             if (plan.operand == kOperandBf16) {
@@ -237,9 +256,9 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             } else {
                 s.pack_f32 = static_cast<float*>(take(ref_rows * 16ull * s.group));
             }
-            s.buf = static_cast<uint64_t*>(take(static_cast<size_t>(splits) * npad * s.cap * sizeof(uint64_t)));
+            s.buf = static_cast<uint64_t*>(take(range_rows(s) * s.cap * sizeof(uint64_t)));
             s.seg_counts = reinterpret_cast<uint32_t*>(count_words * sizeof(uint32_t));  // region offset
-            count_words += static_cast<size_t>(splits) * npad;
+            count_words += range_rows(s);
             s.cand = static_cast<uint64_t*>(take(sp.n * s.kp * sizeof(uint64_t)));
             s.theta = static_cast<float*>(take(sp.n * sizeof(float)));
         }

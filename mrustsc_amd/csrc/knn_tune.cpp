@@ -120,7 +120,8 @@ scrust_hip_status tune(scrust_hip_context* ctx, const float* x, size_t n, size_t
     if (st != SCRUST_HIP_OK) return st;
     if (!base.any_pass) return SCRUST_HIP_OK;  // exact path: no shape to choose
     const DatasetDesc& s = base.sets[0];
-    const TuneEntry def{s.cap, s.splits, base.ordered ? 2u : 1u, 0.0f};
+    // (A launch's last-round blocks may scan in more ranges; the shape is the other blocks'.)
+    const TuneEntry def{s.cap, s.tail_row0 ? 1u : s.splits, base.ordered ? 2u : 1u, 0.0f};
     result->default_tile = def.cap;
     result->default_splits = def.splits;
     result->default_order = def.order;

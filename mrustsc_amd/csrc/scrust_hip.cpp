@@ -159,7 +159,11 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
         stats->pass_flops = plan.pass_flops;
         stats->path = plan.any_pass ? static_cast<uint32_t>(plan.precision) : SCRUST_HIP_EXACT;
         stats->tile = plan.sets[0].cap;
-        stats->splits = plan.any_pass ? plan.sets[0].splits : 0;
+        const DatasetDesc& s0 = plan.sets[0];
+        const bool tail = plan.any_pass && s0.tail_row0 != 0;
+        stats->splits = plan.any_pass ? (tail ? 1u : s0.splits) : 0u;
+        stats->tail_blocks = tail ? (s0.npad - s0.tail_row0) / kPassRows : 0u;
+        stats->tail_splits = tail ? s0.splits : 0u;
         stats->order = plan.ordered ? 2u : 1u;
         stats->tuned = plan.tuned ? 1u : 0u;
     }

@@ -335,3 +335,40 @@ def test_four_million_cells_match_the_oracle_on_sampled_rows(context):
     np.testing.assert_array_equal(ids[queries], want_ids)
     np.testing.assert_array_equal(dist[queries], want_dist)
     assert stats["fallback_rows"] <= n // 1000
+
+
+@pytest.mark.parametrize(("precision", "n", "blocks", "tail_blocks", "nbin"), [
+    ("bf16", 200_000, 782, 270, 35),     # 782 query blocks on 512 workgroup slots
+    ("fp32", 1_000_000, 3907, 323, 40),  # (fp32 at 200,000 already cuts every block in 3)
+])
+@pytest.mark.parametrize("order", ["input", "grouped"])
+def test_last_round_blocks_scan_in_ranges_and_match_the_oracle(context, tmp_path, precision, n, blocks,
+                                                                tail_blocks, nbin, order):
+    # The launch's last round fills more than half of the 512 workgroup slots, so its
+    # blocks scan in three reference ranges each. Every row equals a plan without that split
+    # (two ranges for every block, forced through the tuning table), and sampled rows, a
+    # third of them in the split blocks, equal the oracle.
+    x = synthetic.clustered(n, 50, seed=41)
+    print(f"[case] n={n} d=50 k=15 {precision} order={order} (last-round split)", flush=True)
+    mrustsc_amd.tune_clear()
+    ids, dist, stats = mrustsc_amd.knn_hip(x, 15, precision=precision, order=order, context=context, stats=True)
+    print(stats, flush=True)
+    assert (stats["splits"], stats["tail_blocks"], stats["tail_splits"]) == (1, tail_blocks, 3)
+    table = tmp_path / "two_ranges.txt"
+    nch = 5 if precision == "bf16" else 13
+    table.write_text(f"{0 if precision == 'bf16' else 1} 0 32 {nch} {nbin} 256 2 {1 if order == 'input' else 2} 1.0\n")
+    try:
+        mrustsc_amd.tune_load(table)
+        ids2, dist2, stats2 = mrustsc_amd.knn_hip(x, 15, precision=precision, context=context, stats=True)
+    finally:
+        mrustsc_amd.tune_clear()
+    assert (stats2["splits"], stats2["tail_blocks"], stats2["tuned"]) == (2, 0, 1)
+    np.testing.assert_array_equal(ids, ids2)
+    np.testing.assert_array_equal(dist, dist2)
+    tail_row0 = (blocks - tail_blocks) * 256
+    rng = np.random.default_rng(5)
+    queries = np.concatenate([rng.choice(n, 40, replace=False), rng.choice(np.arange(tail_row0, n), 20, replace=False),
+                              [0, tail_row0 - 1, tail_row0, n - 1]])
+    want_ids, want_dist = oracle.knn(x, 15, queries=queries)
+    np.testing.assert_array_equal(ids[queries], want_ids)
+    np.testing.assert_array_equal(dist[queries], want_dist)
