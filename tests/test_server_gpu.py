@@ -111,3 +111,33 @@ def test_a_large_request_in_the_mix_equals_its_input_order_build(context):
             ids, dist, _ = server.wait(t)
             np.testing.assert_array_equal(ids, ids_a)
             np.testing.assert_array_equal(dist, dist_a)
+
+
+def test_requests_take_the_tuning_table_and_still_equal_results_alone(context, tmp_path):
+    # A table that moves every request's bucket off the default shape (smallest buffer, two
+    # reference ranges, input order): server builds take it and give the untuned results.
+    reqs = _requests(12, seed=3)
+    alone = [mrustsc_amd.knn_hip(x, k, context=context) for x, k in reqs]
+    lines = []
+    for x, k in reqs:
+        n, d = x.shape
+        kp = 32 if k + 17 <= 32 else 64
+        nch = (3 * d + 3 + 31) // 32
+        nbin = int(round(2 * np.log2(n)))
+        lines.append(f"0 0 {kp} {nch} {nbin} {4 * kp} 2 1 1.0")
+    table = tmp_path / "table.txt"
+    table.write_text("\n".join(lines) + "\n")
+    mrustsc_amd.tune_clear()
+    mrustsc_amd.tune_load(table)
+    try:
+        _, _, st = mrustsc_amd.knn_hip(reqs[0][0], reqs[0][1], context=context, stats=True)
+        assert st["tuned"] == 1 and st["splits"] == 2 and st["order"] == 1
+        print(f"[case] server 12 requests with a tuning table", flush=True)
+        with mrustsc_amd.Server(slots=4) as server:
+            tickets = [server.submit(x, k) for x, k in reqs]
+            for (ids_a, dist_a), t in zip(alone, tickets):
+                ids, dist, _ = server.wait(t)
+                np.testing.assert_array_equal(ids, ids_a)
+                np.testing.assert_array_equal(dist, dist_a)
+    finally:
+        mrustsc_amd.tune_clear()

@@ -202,7 +202,10 @@ def test_batch_tuner_checks_its_arguments():
 
 
 @pytest.mark.gpu
-def test_batch_tuner_keeps_every_graph_and_the_tuned_replays_stay_bit_exact(context, empty_table):
+@pytest.mark.parametrize(("precision", "metric"), [("bf16", "euclidean"), ("fp32", "euclidean"),
+                                                   ("bf16", "cosine")])
+def test_batch_tuner_keeps_every_graph_and_the_tuned_replays_stay_bit_exact(context, empty_table, precision,
+                                                                            metric):
     torch = pytest.importorskip("torch")
     specs = [(3000, 50, 15, 1), (700, 30, 30, 2), (5000, 100, 15, 3), (16, 5, 15, 4),
              (900, 20, 60, 7), (2048, 100, 30, 8)]  # k = 60: exact path inside the batch
@@ -216,8 +219,8 @@ def test_batch_tuner_keeps_every_graph_and_the_tuned_replays_stay_bit_exact(cont
         outs.append((di, dd))
         entries.append((dx.data_ptr(), n, d, k, di.data_ptr(), dd.data_ptr()))
     torch.cuda.synchronize()
-    print(f"[case] batch tune of {len(specs)} datasets", flush=True)
-    with mrustsc_amd.Batch(entries, context=context) as batch:
+    print(f"[case] batch tune of {len(specs)} datasets {precision} {metric}", flush=True)
+    with mrustsc_amd.Batch(entries, precision=precision, metric=metric, context=context) as batch:
         batch.run()
         before = [(di.cpu().numpy().copy(), dd.cpu().numpy().copy()) for di, dd in outs]
         r = batch.tune()
@@ -236,4 +239,4 @@ def test_batch_tuner_keeps_every_graph_and_the_tuned_replays_stay_bit_exact(cont
                 np.testing.assert_array_equal(di.cpu().numpy(), b_ids, err_msg=f"n={n} replay {replay}")
                 np.testing.assert_array_equal(dd.cpu().numpy(), b_dist, err_msg=f"n={n} replay {replay}")
     for (n, d, k, _), (x, _), (b_ids, b_dist) in zip(specs, data, before):
-        _oracle_equal(x, k, b_ids.view(np.uint32), b_dist)
+        _oracle_equal(x, k, b_ids.view(np.uint32), b_dist, metric)
