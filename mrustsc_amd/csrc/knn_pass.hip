@@ -260,12 +260,31 @@ struct PassSmem {
     u32x4 tile[2][kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max)];
 };
 
+// Three tile buffers for LDS-direct loads, each its own __shared__ array: the compiler tells
+// loads into one buffer apart from reads of another only across distinct objects.
+struct DmaTiles {
+    u32x4* b0;
+    u32x4* b1;
+    u32x4* b2;
+};
+
+// Slots (16 bytes) of one LDS tile buffer.
+template <int OP, int NCH>
+constexpr int kTileSlots = kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max);
+
+// The fixed-chunk bf16 pass loads tiles straight into LDS when three tile buffers fit in half
+// of a CU's 160 KB (two workgroups per CU): up to 5 chunks. Measured at 1M x 50 on one GPU,
+// A/B: the bf16 pass 1.5% faster; the fp32 pass 1.3% slower (its loads then have one tile of
+// lead instead of two), so it keeps the register staging.
+template <int OP, int NCH>
+constexpr bool kPassDma = OP == kOperandBf16 && 3 * kTileSlots<OP, NCH> * 16 <= 80 * 1024;
+
 // One work item: query rows [block * 256, block * 256 + 256) of dataset s against the
 // references of tiles [tile_begin, tile_end), into buffer slot `split`. FIXED: the
 // dataset's chunk count equals NCH at compile time.
-template <int OP, int NCH, bool FIXED>
+template <int OP, int NCH, bool FIXED, bool DMA = false>
 __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem& item,
-                                           PassSmem<OP, NCH>& sm) {
+                                           PassSmem<OP, NCH>* sm, DmaTiles dma = {}) {
     typedef Operand<OP, NCH> Op;
     const int nch = FIXED ? NCH : static_cast<int>(s.nch);
     const int row_u4 = FIXED ? Op::kRowU4Max : Op::row_u4(nch);
@@ -332,6 +351,11 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         return u >= wrap ? u - wrap : u;
     };
     const uint32_t tile_u4 = kTileRefs * row_u4;
+    // LDS tile buffer `cur` (a compile-time constant at every use).
+    auto tile_of = [&](int cur) __attribute__((always_inline)) -> u32x4* {
+        if constexpr (DMA) return cur == 0 ? dma.b0 : cur == 1 ? dma.b1 : dma.b2;
+        else return sm->tile[cur];
+    };
     // Two register sets stage tiles t+1 and t+2 while tile t is consumed from LDS, so a
     // tile's loads are issued two tiles before they are stored (HBM/L3 latency exceeds
     // one tile's MFMA time).
@@ -355,12 +379,12 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     }
 #define SCRUST_STORE_TILE(P, B)                                                   \
     _Pragma("unroll") for (int i = 0; i < Op::kLoadsMax; i++) {                   \
-        if (FIXED || i < loads) sm.tile[B][lds_at[i]] = P[i];                     \
+        if (FIXED || i < loads) tile_of(B)[lds_at[i]] = P[i];                     \
     }
     auto mfma_block = [&](f32x4 (&acc)[4], int cur, int cb) __attribute__((always_inline)) {
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) acc[qb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const u32x4* brow = &sm.tile[cur][(cb * 16 + lc) * stride];
+        const u32x4* brow = &tile_of(cur)[(cb * 16 + lc) * stride];
         if constexpr (OP == kOperandBf16) {
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
@@ -436,7 +460,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // other buffer, a barrier, and the compaction of rows whose buffers could overflow.
     // bf16: a block's reference fragments, read from LDS one block ahead of their MFMAs.
     auto load_frags = [&](auto& b, int cur, int cb) __attribute__((always_inline)) {
-        const u32x4* brow = &sm.tile[cur][(cb * 16 + lc) * stride];
+        const u32x4* brow = &tile_of(cur)[(cb * 16 + lc) * stride];
 #pragma unroll
         for (int c = 0; c < NCH; c++)
             if (FIXED || c < nch) b[c] = __builtin_bit_cast(bf16x8, brow[g * gstride + c]);
@@ -548,6 +572,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         else compact_rows(std::integral_constant<int, 8>{}, full);
     };
 
+    if constexpr (!DMA) {
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
     // merge into the loop head, where the compiler must then assume the prefetch just
     // issued is outstanding and waits for it. Loads past the end reload the last tile.
@@ -568,6 +593,62 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         __syncthreads();
         compact();
     }
+    } else {
+        // LDS-direct loads (global_load_lds_dwordx4): one wave-wide load fills 64 consecutive
+        // 16-byte slots of a tile buffer, lane l the l-th. Slot u of a tile is reference row
+        // u / stride, part q = u % stride: lane group q / gstride, chunk q % gstride, or
+        // padding past the group's chunks (it loads the row's first chunk; never read).
+        // Tile t+2 is loaded into the buffer that tile t-1 left while tile t is consumed;
+        // before each barrier a wave waits for all its loads (so the next tile has landed
+        // and its own reads of the tile just consumed are done), so a plain s_barrier
+        // suffices and the compiler has no fence to drain the loads at.
+        constexpr int kSlots = kTileSlots<OP, NCH>;
+        constexpr int kRounds = (kSlots + 4 * kWave - 1) / (4 * kWave);
+        int src_at[kRounds];
+        bool src_on[kRounds];
+#pragma unroll
+        for (int i = 0; i < kRounds; i++) {
+            const int u = (i * 4 + w) * kWave + lane;
+            src_on[i] = u < kSlots;
+            const int v = src_on[i] ? u : 0;
+            const int q = v % stride, gg = q / gstride, c = q % gstride;
+            src_at[i] = (v / stride) * row_u4 + ((gg < 4 && c < group_u4) ? gg * group_u4 + c : 0);
+        }
+        auto dma_tile = [&](uint32_t T, u32x4* to) __attribute__((always_inline)) {
+            const SCRUST_GLOBAL u32x4* from = ref_rows + static_cast<size_t>(phys(T)) * tile_u4;
+#pragma unroll
+            for (int i = 0; i < kRounds; i++) {
+                if (src_on[i])
+                    __builtin_amdgcn_global_load_lds(from + src_at[i],
+                                                     (__attribute__((address_space(3))) void*)(to + (i * 4 + w) * kWave),
+                                                     16, 0, 0);
+            }
+        };
+        auto settle = [&]() __attribute__((always_inline)) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_s_barrier();
+        };
+        dma_tile(t_begin, dma.b0);
+        if (t_begin + 1 < t_end) dma_tile(t_begin + 1, dma.b1);
+        settle();
+        for (uint32_t t = t_begin;;) {
+            if (t + 2 < t_end) dma_tile(t + 2, dma.b2);
+            consume(phys(t), 0);
+            settle();
+            compact();
+            if (++t >= t_end) break;
+            if (t + 2 < t_end) dma_tile(t + 2, dma.b0);
+            consume(phys(t), 1);
+            settle();
+            compact();
+            if (++t >= t_end) break;
+            if (t + 2 < t_end) dma_tile(t + 2, dma.b1);
+            consume(phys(t), 2);
+            settle();
+            compact();
+            if (++t >= t_end) break;
+        }
+    }
 #undef SCRUST_LOAD_TILE
 #undef SCRUST_STORE_TILE
     // Segment counts for the final selection (select_kernel): byte g of the row's word.
@@ -584,7 +665,6 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
 template <int OP, int NCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
 void pass_kernel(const DatasetDesc* __restrict__ set) {
-    __shared__ PassSmem<OP, NCH> sm;
     const DatasetDesc& s = *set;
     // Workgroups below the tail block scan their block's whole range; from it on, workgroup
     // tail_block + g scans range j = g / tail_blocks of block tail_block + g % tail_blocks:
@@ -611,7 +691,15 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
     // (4096 rows, about one cell), so its rows meet their neighbours first; the 16 blocks
     // share the start, and its tiles stay in L2 for all of them.
     item.pad = s.rotate ? (4u * (item.block & ~15u)) % (2 * pairs) : 0u;
-    pass_block<OP, NCH, true>(s, item, sm);
+    if constexpr (kPassDma<OP, NCH>) {
+        __shared__ u32x4 b0[kTileSlots<OP, NCH>];
+        __shared__ u32x4 b1[kTileSlots<OP, NCH>];
+        __shared__ u32x4 b2[kTileSlots<OP, NCH>];
+        pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2});
+    } else {
+        __shared__ PassSmem<OP, NCH> sm;
+        pass_block<OP, NCH, true>(s, item, &sm);
+    }
 }
 
 // Persistent grid over a queue of (dataset, query block, reference range) items, largest
@@ -629,7 +717,7 @@ __device__ __forceinline__ void pass_batch_body(const DatasetDesc* __restrict__ 
         __syncthreads();
         if (item >= n_items) break;
         const PassItem it = items[item];
-        pass_block<OP, NCH_MAX, false>(sets[it.set], it, sm);
+        pass_block<OP, NCH_MAX, false>(sets[it.set], it, &sm);
         __syncthreads();
     }
 }
