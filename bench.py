@@ -582,8 +582,9 @@ def main() -> int:
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(the path shards by query rows; see the design notes)"}), flush=True)
+        print(json.dumps({"error": "scrust_hip drives one GPU; --gpus must be 1 (across GPUs the "
+                          "graph would shard by query rows, every GPU holding all references)"}),
+              flush=True)
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
