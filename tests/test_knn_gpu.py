@@ -372,3 +372,34 @@ def test_last_round_blocks_scan_in_ranges_and_match_the_oracle(context, tmp_path
     want_ids, want_dist = oracle.knn(x, 15, queries=queries)
     np.testing.assert_array_equal(ids[queries], want_ids)
     np.testing.assert_array_equal(dist[queries], want_dist)
+
+
+def test_random_shapes_options_and_data_match_the_oracle(context):
+    # 200 seeded cases across n 20-30,000, d 1-129 (both sides of every chunk boundary and of
+    # the pass's d <= 128 limit), k 1-64 (both sides of KP = 32 / 64 and of the pass's k <= 47),
+    # both metrics, all three paths, every row order and buffer size, on clustered,
+    # integer-lattice (ties) and scaled-normal data; up to 300 rows checked per case.
+    rng = np.random.default_rng(2024)
+    for case in range(200):
+        n = int(np.exp(rng.uniform(np.log(20), np.log(30000))))
+        d = int(rng.choice([1, 2, 3, 5, 8, 13, 20, 30, 31, 32, 41, 42, 50, 52, 53, 64, 100, 128, 129]))
+        k = int(min(n - 1, rng.choice([1, 2, 5, 14, 15, 16, 30, 47, 48, 64])))
+        metric = str(rng.choice(["euclidean", "cosine"]))
+        precision = str(rng.choice(["bf16", "bf16", "fp32", "exact"]))
+        order = str(rng.choice(["auto", "input", "grouped"]))
+        tile = int(rng.choice([0, 0, 128, 256, 512]))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            x = synthetic.clustered(n, d, seed=case, n_clusters=int(rng.integers(2, 12)))
+        elif kind == 1:
+            x = rng.integers(-3, 4, size=(n, d)).astype(np.float32)
+        else:
+            x = rng.normal(size=(n, d)).astype(np.float32) * np.float32(rng.choice([1e-3, 1.0, 1e3]))
+        tag = f"case {case}: n={n} d={d} k={k} {metric} {precision} order={order} tile={tile} data={kind}"
+        print(f"[case] {tag}", flush=True)
+        ids, dist = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, order=order, tile=tile,
+                                        context=context)
+        q = np.arange(n) if n <= 3000 else rng.choice(n, 300, replace=False)
+        want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
+        np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
+        np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)
