@@ -61,6 +61,9 @@ def parse_args():
     p.add_argument("--slots", type=int, default=4, help="c5: builds the server runs at once")
     p.add_argument("--gemm-ceiling", action="store_true",
                    help="time the platform's bf16 GEMM (torch / hipBLASLt) as a practical MFMA ceiling")
+    p.add_argument("--hbm-probe", type=int, default=0, metavar="MB",
+                   help="only time a device fill and a device copy of an MB-sized f32 buffer "
+                        "(HIP events, 10 launches after 3) and exit: the practical HBM rates")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only copy a MIB-sized device buffer three times and exit: the "
                         "known-size copy that calibrates the HBM byte counters")
@@ -116,6 +119,31 @@ def copy_probe(mib: int) -> int:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": src.numel(), "launches": 3}), flush=True)
+    return 0
+
+
+def hbm_probe(mb: int) -> int:
+    """A plain device fill and copy of mb MB: what streaming writes and reads reach on this GPU,
+    beside the 8 TB/s nominal peak (the reference point for the write-bound t-SNE tile kernel,
+    whose output at 20,000 cells is 1,600 MB)."""
+    import torch
+
+    n = mb * 1_000_000 // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty(n, dtype=torch.float32, device="cuda")
+    for name, fn, nbytes in (("fill", lambda: a.fill_(1.0), 4 * n), ("copy", lambda: b.copy_(a), 8 * n)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        print(json.dumps({"op": name, "mb": mb, "ms": ms, "bytes_moved": nbytes,
+                          "tb_per_s": nbytes / ms / 1e9}), flush=True)
     return 0
 
 
@@ -588,6 +616,8 @@ def main() -> int:
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.hbm_probe:
+        return hbm_probe(args.hbm_probe)
     if args.gemm_ceiling:
         return gemm_ceiling()
 
