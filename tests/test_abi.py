@@ -147,3 +147,30 @@ def test_mixtures_are_deterministic_and_shaped():
     assert len(specs) == 64
     assert all(500 <= s["n"] <= 50000 and s["d"] in (30, 50, 100) and s["k"] in (15, 30)
                for s in specs)
+
+
+def test_a_missing_library_fails_loudly_with_no_fallback(tmp_path):
+    # A fresh interpreter whose loader points at a path with no library: every entry point
+    # raises ImportError naming the build step, instead of falling back to any other code.
+    import subprocess
+    import sys
+
+    root = Path(__file__).resolve().parents[1]
+    script = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from pathlib import Path\n"
+        "import mrustsc_amd\n"
+        "from mrustsc_amd import _native\n"
+        "_native.LIB_PATH = Path(%r)\n"
+        "_native._lib = None\n"
+        "import numpy as np\n"
+        "try:\n"
+        "    mrustsc_amd.knn(np.zeros((8, 3), dtype=np.float32), 2)\n"
+        "except ImportError as e:\n"
+        "    print('raised:', e)\n"
+        "    sys.exit(0)\n"
+        "sys.exit(1)\n"
+    ) % (str(root), str(tmp_path / "absent.so"))
+    done = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=300)
+    assert done.returncode == 0, done.stdout + done.stderr
+    assert "is missing" in done.stdout and "build" in done.stdout
