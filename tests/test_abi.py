@@ -174,3 +174,48 @@ def test_a_missing_library_fails_loudly_with_no_fallback(tmp_path):
     done = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=300)
     assert done.returncode == 0, done.stdout + done.stderr
     assert "is missing" in done.stdout and "build" in done.stdout
+
+
+# C struct of include/scrust_hip.h -> its ctypes mirror in mrustsc_amd/_native.py.
+STRUCTS = {
+    "scrust_hip_knn_options": "KnnOptions",
+    "scrust_hip_knn_stats": "KnnStats",
+    "scrust_hip_tune_result": "TuneResult",
+    "scrust_hip_dataset": "Dataset",
+    "scrust_hip_batch_stats": "BatchStats",
+    "scrust_hip_batch_tune_result": "BatchTuneResult",
+    "scrust_hip_server_config": "ServerConfig",
+    "scrust_hip_request_info": "RequestInfo",
+    "scrust_hip_server_stats": "ServerStats",
+}
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    # A C program built against the header prints every struct's size and each field's
+    # offset; the ctypes mirrors must agree field by field (same names, same places).
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler on this machine")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(_native, pyname)
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-o", str(exe), str(src)], check=True, capture_output=True, text=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(_native, pyname)
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, f"{cname}.{field}"
