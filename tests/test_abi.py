@@ -219,3 +219,17 @@ def test_ctypes_structs_match_the_header_layout(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for field, _ in cls._fields_:
             assert got[(cname, field)] == getattr(cls, field).offset, f"{cname}.{field}"
+
+
+@pytest.mark.parametrize("lang", ["c", "c++"])
+def test_header_compiles_as_c_and_cpp(tmp_path, lang):
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc" if lang == "c" else "g++")
+    if cc is None:
+        pytest.skip(f"no {lang} compiler on this machine")
+    std = "-std=c99" if lang == "c" else "-std=c++11"
+    done = subprocess.run([cc, "-fsyntax-only", "-Wall", "-Wextra", "-Werror", std, "-x", lang, str(HEADER)],
+                          capture_output=True, text=True)
+    assert done.returncode == 0, done.stderr
