@@ -233,3 +233,16 @@ def test_header_compiles_as_c_and_cpp(tmp_path, lang):
     done = subprocess.run([cc, "-fsyntax-only", "-Wall", "-Wextra", "-Werror", std, "-x", lang, str(HEADER)],
                           capture_output=True, text=True)
     assert done.returncode == 0, done.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import json
+    import subprocess
+    import sys
+
+    root = Path(__file__).resolve().parents[1]
+    done = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True,
+                          text=True, timeout=300, cwd=str(root))
+    assert done.returncode == 2
+    line = json.loads(done.stdout.strip().splitlines()[-1])
+    assert "one GPU" in line["error"]
