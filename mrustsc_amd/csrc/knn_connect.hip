@@ -71,6 +71,10 @@ __global__ __launch_bounds__(256) void edges_kernel(const uint32_t* __restrict__
     for (uint32_t j = 0; j < k; j++) {
         if (nb[j] >= n) bad |= kConnFlagBadId;
         if (!(d[j] >= 0.0f) || !__builtin_isfinite(d[j])) bad |= kConnFlagBadDistance;
+        // A kNN graph lists each neighbour once per row; the key-run merge below folds at
+        // most one forward and one backward entry per (row, column), so repeats are refused.
+        for (uint32_t i = 0; i < j; i++)
+            if (nb[i] == nb[j]) bad |= kConnFlagRepeatedId;
     }
     const uint64_t invalid = static_cast<uint64_t>(n) << bits;
     uint64_t* out_keys = keys + 2 * static_cast<size_t>(row) * k;

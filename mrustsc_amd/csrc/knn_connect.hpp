@@ -11,6 +11,7 @@ namespace scrust_hip {
 
 constexpr uint32_t kConnFlagBadId = 1u;        // a neighbour id >= n
 constexpr uint32_t kConnFlagBadDistance = 2u;  // a negative or non-finite distance
+constexpr uint32_t kConnFlagRepeatedId = 4u;   // a neighbour id listed twice in one row
 
 // Byte offsets of the pipeline's device workspace.
 struct ConnLayout {

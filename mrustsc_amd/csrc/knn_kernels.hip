@@ -407,11 +407,16 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     // >= theta, so its fp32 squared distance is >= `lower`; if that clears the band's
     // upper edge and every snapping threshold, the kept list holds every reference the
     // oracle could place in the first k (or in the tie band).
-    bool certified = true;
-    if (full) {
+    // A list shorter than KP holds every reference appended over the whole scan; it is
+    // complete only if that is every other cell (scores that overflowed are never appended).
+    // A dataset past the pass's norm limit skipped the pass: every row is listed.
+    const float max_norm = __uint_as_float(*s.max_norm_bits);
+    bool certified = full || n_valid == static_cast<int>(n) - 1;
+    if (!(max_norm <= kPassNormLimit)) certified = false;
+    if (full && certified) {
         const double u = 0x1p-24;
         const double nqd = static_cast<double>(nq);
-        const double nmax = static_cast<double>(__uint_as_float(*s.max_norm_bits));
+        const double nmax = static_cast<double>(max_norm);
         const double gamma = d * u / (1.0 - d * u);
         const double qn = __builtin_sqrt(nqd * (1.0 + gamma));
         const double rn = __builtin_sqrt(nmax * (1.0 + gamma));

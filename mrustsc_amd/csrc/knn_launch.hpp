@@ -28,6 +28,11 @@ constexpr int kMaxK = 256;      // largest k any path serves
 constexpr int kMaxPassK = 47;   // largest k the MFMA candidate pass serves (KP = 64)
 constexpr int kMaxPassD = 128;  // largest d with a compiled MFMA variant
 constexpr int kAccWordsHost = 10;
+// Largest squared norm the MFMA candidate pass serves. Past it (coordinates beyond about
+// 10^15, or a norm that overflowed f32) packed products, their sums or the bf16 rounding of
+// the norm feature can overflow; such a dataset skips the pass and every row takes the
+// exact kernel, whose f32 direct differences are the oracle's arithmetic at any magnitude.
+constexpr float kPassNormLimit = 0x1p100f;
 constexpr int kPassRows = 256;  // query rows per pass work item (4 waves x 64)
 constexpr int kTileRefsHost = 64;  // references per pass tile
 

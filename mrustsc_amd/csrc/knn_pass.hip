@@ -286,6 +286,9 @@ template <int OP, int NCH, bool FIXED, bool DMA = false>
 __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem& item,
                                            PassSmem<OP, NCH>* sm, DmaTiles dma = {}) {
     typedef Operand<OP, NCH> Op;
+    // A dataset with a squared norm past kPassNormLimit is served by the exact kernel (the
+    // rerank lists all its rows); its segment counts stay at the zeros they were cleared to.
+    if (__uint_as_float(*global_ptr(s.max_norm_bits)) > kPassNormLimit) return;
     const int nch = FIXED ? NCH : static_cast<int>(s.nch);
     const int row_u4 = FIXED ? Op::kRowU4Max : Op::row_u4(nch);
     const int group_u4 = FIXED ? Op::kGroupU4Max : Op::group_u4(nch);

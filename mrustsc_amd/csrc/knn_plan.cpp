@@ -76,14 +76,16 @@ uint32_t partition_even(uint32_t set, uint32_t blocks, uint32_t pairs, uint32_t 
 
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                               const scrust_hip_knn_options* options, Plan& plan,
-                              int compute_units, const TuneEntry* force, const BatchKnobs* knobs) {
+                              int compute_units, bool single, const TuneEntry* force,
+                              const BatchKnobs* knobs) {
     plan = Plan{};
+    if (single && specs.size() != 1)
+        return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: a single call lays out exactly one dataset");
     const int metric = options ? static_cast<int>(options->metric) : kMetricEuclidean;
     const int precision = options ? static_cast<int>(options->precision) : SCRUST_HIP_BF16;
     uint32_t cap = options && options->tile ? options->tile : 256u;
     if (cap != 128 && cap != 256 && cap != 512)
         return invalid("tile", "0, 128, 256 or 512", std::to_string(cap));
-    const bool single = specs.size() == 1;
     plan.precision = precision;
     plan.operand = precision == SCRUST_HIP_BF16 ? kOperandBf16
                    : precision == SCRUST_HIP_FP32 ? kOperandF32

@@ -100,7 +100,7 @@ struct Request {
     size_t arena_bytes = 0, meta_bytes = 0;
     uint32_t slot = 0;
     uint64_t fallback_rows = 0;
-    Plan plan;
+    Plan plan;  // laid out at submission, placed into the region at admission
 };
 
 struct Slot {
@@ -204,12 +204,13 @@ bool scrust_hip_server::start(Slot& slot, Request& r) {
     }
     char* pin = static_cast<char*>(slot.pinned);
     if (in_bytes) std::memcpy(pin + l.input, r.x, in_bytes);
-    const float* x = r.device_input ? r.x : reinterpret_cast<const float*>(base + l.input);
-    std::vector<SetSpec> specs{SetSpec{x, r.n, r.d, r.k,
-                                       reinterpret_cast<uint32_t*>(base + l.indices),
-                                       reinterpret_cast<float*>(base + l.distances)}};
-    scrust_hip_status st = plan_layout(specs, &r.options, r.plan, compute_units);
-    if (st != SCRUST_HIP_OK) return failed(st, last_error());
+    // The plan was laid out at submission and sized this region; only the input and output
+    // addresses inside the region are known now (plan_layout copies them, unread).
+    DatasetDesc& desc = r.plan.sets[0];
+    desc.x = r.device_input ? r.x : reinterpret_cast<const float*>(base + l.input);
+    desc.out_indices = reinterpret_cast<uint32_t*>(base + l.indices);
+    desc.out_distances = reinterpret_cast<float*>(base + l.distances);
+    scrust_hip_status st = SCRUST_HIP_OK;
     hipError_t err = hipSuccess;
     if (in_bytes) err = hipMemcpyAsync(base + l.input, pin + l.input, in_bytes, hipMemcpyHostToDevice, slot.stream);
     if (err != hipSuccess) return failed(SCRUST_HIP_KERNEL, hipGetErrorString(err));
@@ -296,6 +297,15 @@ void scrust_hip_server::run() {
             peak_in_flight = std::max(peak_in_flight, in_flight);
             lk.unlock();
             const bool ok = start(*idle, r);
+            if (!ok) {
+                // Copies and kernels start() queued before it failed may still read the
+                // slot's staging and write the region: both are reused only after they end.
+                const hipError_t drained = hipStreamSynchronize(idle->stream);
+                if (drained != hipSuccess && r.status == SCRUST_HIP_OK) {
+                    r.status = SCRUST_HIP_KERNEL;
+                    r.error = std::string("HIP kernel knn: ") + hipGetErrorString(drained);
+                }
+            }
             lk.lock();
             if (!ok) {
                 alloc.release(r.offset, r.bytes);
@@ -379,13 +389,15 @@ scrust_hip_status submit_request(scrust_hip_server* server, const float* embeddi
     if (options) r->options = *options;
     r->out_indices = out_indices;
     r->out_distances = out_distances;
-    // Size the device region now (the layout does not touch the pointers).
-    std::vector<SetSpec> specs{SetSpec{embedding, n, d, k, out_indices, out_distances}};
-    Plan sizing;
-    st = plan_layout(specs, options, sizing, server->compute_units);
+    // Lay the build out once, now: the work shape (a tuning-table lookup for an automatic
+    // tile) and with it the region size are fixed at submission, so later changes to the
+    // tuning table cannot outgrow the region admission hands out. Input and output
+    // addresses are set at admission.
+    std::vector<SetSpec> specs{SetSpec{embedding, n, d, k, nullptr, nullptr}};
+    st = plan_layout(specs, options, r->plan, server->compute_units, true);
     if (st != SCRUST_HIP_OK) return st;
-    r->arena_bytes = sizing.arena_bytes;
-    r->meta_bytes = sizing.meta_bytes;
+    r->arena_bytes = r->plan.arena_bytes;
+    r->meta_bytes = r->plan.meta_bytes;
     r->bytes = region_layout(n, d, k, r->arena_bytes, r->meta_bytes, device_input).total;
     std::lock_guard<std::mutex> lk(server->mu);
     if (r->bytes > server->pool_bytes) {

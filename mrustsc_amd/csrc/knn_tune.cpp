@@ -116,7 +116,7 @@ scrust_hip_status tune(scrust_hip_context* ctx, const float* x, size_t n, size_t
     const TuneEntry defaults{0u, 0u, 0u, 0.0f};
     std::vector<SetSpec> specs{SetSpec{x, n, d, k, nullptr, nullptr}};
     Plan base;
-    scrust_hip_status st = plan_layout(specs, &opt, base, ctx->compute_units, &defaults);
+    scrust_hip_status st = plan_layout(specs, &opt, base, ctx->compute_units, true, &defaults);
     if (st != SCRUST_HIP_OK) return st;
     if (!base.any_pass) return SCRUST_HIP_OK;  // exact path: no shape to choose
     const DatasetDesc& s = base.sets[0];

@@ -136,7 +136,7 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
                              const TuneEntry* force) {
     std::vector<SetSpec> specs{SetSpec{x, n, d, k, out_indices, out_distances}};
     Plan plan;
-    scrust_hip_status st = plan_layout(specs, options, plan, ctx->compute_units, force);
+    scrust_hip_status st = plan_layout(specs, options, plan, ctx->compute_units, true, force);
     if (st != SCRUST_HIP_OK) return st;
     SCRUST_CHECK(ctx->arena.reserve(plan.arena_bytes), "allocate workspace");
     SCRUST_CHECK(ctx->meta.reserve(plan.meta_bytes), "allocate metadata");
@@ -182,7 +182,7 @@ static scrust_hip_status batch_build(scrust_hip_batch* batch, const BatchKnobs& 
     SCRUST_CHECK(hipStreamSynchronize(s), "wait for the batch");
     Plan plan;
     scrust_hip_status st = plan_layout(batch->specs, batch->has_options ? &batch->options : nullptr, plan,
-                                       context->compute_units, nullptr, &knobs);
+                                       context->compute_units, false, nullptr, &knobs);
     if (st != SCRUST_HIP_OK) return st;
     if (batch->exec) {
         (void)hipGraphExecDestroy(batch->exec);
@@ -488,6 +488,8 @@ scrust_hip_status run_connectivities(scrust_hip_context* ctx, const uint32_t* in
         return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected neighbour ids below " + std::to_string(n) +
                                           ", got an out-of-range neighbour id");
     if (flags & kConnFlagBadDistance) return invalid("distances", "finite and >= 0", "a negative or non-finite value");
+    if (flags & kConnFlagRepeatedId)
+        return invalid("indices", "distinct neighbour ids within each row", "a repeated id");
     if (nnz > capacity)
         return fail(SCRUST_HIP_SHAPE, "shape mismatch: expected a capacity of " + std::to_string(nnz) +
                                           " entries, got " + std::to_string(capacity));

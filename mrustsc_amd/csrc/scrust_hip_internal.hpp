@@ -114,13 +114,16 @@ const char* last_error();
 void clear_error();
 bool device_is_gfx950(int device, hipDeviceProp_t* props_out);
 
-// Lays out a plan for `specs` (no device work) for a device of `compute_units` CUs. With
-// `force`, a single dataset's pass takes that work shape (cap 0: the untuned defaults,
-// table ignored); otherwise a single dataset with an automatic tile looks its bucket up in
-// the tuning table. Batches take their shape from `knobs` (null: the defaults).
+// Lays out a plan for `specs` (no device work) for a device of `compute_units` CUs.
+// `single`: the plan is enqueued as one call (plan_enqueue with single = true: exactly one
+// dataset, the compile-time-specialised pass kernel); otherwise as a batch (the persistent
+// pass over work items), whatever the dataset count. With `force`, a single dataset's pass
+// takes that work shape (cap 0: the untuned defaults, table ignored); otherwise a single
+// dataset with an automatic tile looks its bucket up in the tuning table. Batches take
+// their shape from `knobs` (null: the defaults).
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                               const scrust_hip_knn_options* options, Plan& plan,
-                              int compute_units, const TuneEntry* force = nullptr,
+                              int compute_units, bool single, const TuneEntry* force = nullptr,
                               const BatchKnobs* knobs = nullptr);
 // Points the plan's descriptors into `arena` and `meta` (sized arena_bytes / meta_bytes)
 // and uploads descriptors, items and the exact-row list on `stream`. With `pinned` (host

@@ -328,7 +328,9 @@ static int apply_tie_rule(const uint64_t* best, size_t kept, size_t k, size_t n_
     double tau = t * (double)(d + 2) * 0x1p-23;
     if (kept <= k) return 0;
     double next = (double)unordered((uint32_t)(best[k] >> 32));
-    if (next - t > tau) return 0;
+    /* Written as "not within tau": with t = +inf the gap inf - inf is NaN, and a tie of
+     * infinities is left in (d^2, id) order, as on the device. */
+    if (!(next - t <= tau)) return 0;
     size_t lo = k - 1;
     while (lo > 0 && t - (double)unordered((uint32_t)(best[lo - 1] >> 32)) <= tau) lo--;
     size_t hi = k;

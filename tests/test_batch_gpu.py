@@ -102,3 +102,25 @@ def test_batch_rejects_an_invalid_dataset(context):
     with pytest.raises(ValueError, match="dataset 1"):
         mrustsc_amd.Batch([(x.data_ptr(), 100, 8, 15, di.data_ptr(), dd.data_ptr()),
                            (x.data_ptr(), 10, 8, 15, di.data_ptr(), dd.data_ptr())])
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("n", [3000, 70000])
+def test_a_one_dataset_batch_equals_its_single_call_and_the_oracle(context, precision, n):
+    # A batch of one dataset still runs the batched pass (work items and groups); a layout
+    # that inferred a single call from the count launched no pass at all.
+    print(f"[case] one-dataset batch n={n} precision={precision}", flush=True)
+    data, outs, batch = _make([(n, 50, 15, 31)], precision)
+    (x, _), (di, dd) = data[0], outs[0]
+    with batch:
+        stats = batch.run()
+        assert stats["pass_items"] > 0
+        ids = di.cpu().numpy().view(np.uint32)
+        dist = dd.cpu().numpy()
+    want_ids, want_dist = mrustsc_amd.knn_hip(x, 15, precision=precision, context=context)
+    np.testing.assert_array_equal(ids, want_ids)
+    np.testing.assert_array_equal(dist, want_dist)
+    rows = np.linspace(0, n - 1, 200).astype(np.uint32)
+    o_ids, o_dist = oracle.knn(x, 15, queries=rows)
+    np.testing.assert_array_equal(ids[rows], o_ids)
+    np.testing.assert_array_equal(dist[rows], o_dist)

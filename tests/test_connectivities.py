@@ -254,6 +254,17 @@ def test_device_rejects_bad_ids_and_distances(context):
     mrustsc_amd.connectivities(ids, dist, context=context)  # the context still works
 
 
+@pytest.mark.gpu
+def test_device_rejects_a_repeated_neighbour_id(context):
+    # The reference would emit the column once per repeat (neighbors.rs:340-370); a kNN graph
+    # never repeats an id in a row, and the device refuses one rather than fold it.
+    ids = np.array([[1, 2], [0, 2], [0, 1], [1, 1]], dtype=np.uint32)
+    dist = np.ones((4, 2), dtype=np.float32)
+    with pytest.raises(ValueError, match="distinct neighbour ids"):
+        mrustsc_amd.connectivities(ids, dist, context=context)
+    mrustsc_amd.connectivities(ids[:3], dist[:3], context=context)
+
+
 def test_neighbors_rejects_fewer_than_two_neighbours():
     with pytest.raises(ValueError, match="at least 2"):
         mrustsc_amd.neighbors(generic(), n_neighbors=1)

@@ -403,3 +403,22 @@ def test_random_shapes_options_and_data_match_the_oracle(context):
         want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
         np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
         np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("scale", [1e12, 1e15, 1e20])
+def test_huge_coordinates_match_the_oracle(context, precision, scale):
+    # Past a squared norm of 2^100 (coordinates beyond about 1e15) the MFMA pass is skipped
+    # and every row takes the exact kernel; at 1e20 the f32 norms and distances overflow to
+    # +inf, which the oracle's f32 arithmetic does the same way.
+    print(f"[case] huge coordinates scale={scale:g} precision={precision}", flush=True)
+    x = (synthetic.clustered(3000, 20, seed=51, n_clusters=5).astype(np.float64) * scale).astype(np.float32)
+    assert np.isfinite(x).all()
+    ids, dist, st = mrustsc_amd.knn_hip(x, 15, precision=precision, context=context, stats=True)
+    want_ids, want_dist = oracle.knn(x, 15)
+    np.testing.assert_array_equal(ids, want_ids)
+    np.testing.assert_array_equal(dist, want_dist)
+    if scale >= 1e15:
+        assert st["fallback_rows"] == 3000
+    else:
+        assert st["fallback_rows"] == 0

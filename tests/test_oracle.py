@@ -288,3 +288,19 @@ def test_a_band_wider_than_the_cap_keeps_the_plain_order():
     plain, _ = oracle.knn(x, 5, metric="cosine", tie_rule=False)
     np.testing.assert_array_equal(ids[7], plain[7])
     assert ids[7].tolist() == sorted(ids[7].tolist())  # all at d^2 = |y|^2 ~ 1: id order
+
+
+def test_oracle_orders_overflowed_distances_by_id():
+    # Coordinates near 1e21: every f32 squared distance overflows to +inf except between
+    # exact copies (0), so each row lists its copies first, then the smallest other ids, all
+    # at +inf: a tie of infinities is not re-ordered by the fp64 boundary rule.
+    x = (synthetic.clustered(400, 6, seed=5, n_clusters=3, duplicate_fraction=0.02)
+         .astype(np.float64) * 1e20).astype(np.float32)
+    k = 5
+    ids, dist = oracle.knn(x, k)
+    for q in range(x.shape[0]):
+        twins = [r for r in range(x.shape[0]) if r != q and np.array_equal(x[r], x[q])]
+        others = [r for r in range(x.shape[0]) if r != q and r not in twins]
+        want = (twins + others)[:k]
+        assert ids[q].tolist() == want, q
+        assert all(dist[q][j] == (0.0 if want[j] in twins else np.inf) for j in range(k))

@@ -141,3 +141,34 @@ def test_requests_take_the_tuning_table_and_still_equal_results_alone(context, t
                 np.testing.assert_array_equal(dist, dist_a)
     finally:
         mrustsc_amd.tune_clear()
+
+
+def test_the_work_shape_is_fixed_at_submission(context, tmp_path):
+    # Requests queued behind a long build were laid out (and their device regions sized)
+    # with an empty tuning table; retuning their bucket to a larger buffer and more ranges
+    # while they wait must not change the plan admission places into those regions.
+    mrustsc_amd.tune_clear()
+    first = synthetic.clustered(300000, 50, seed=41)
+    small = [synthetic.clustered(6000, 50, seed=42 + i) for i in range(6)]
+    alone = [mrustsc_amd.knn_hip(x, 15, context=context) for x in small]
+    nbin = int(round(2 * np.log2(6000)))
+    table = tmp_path / "table.txt"
+    table.write_text(f"0 0 32 5 {nbin} 512 2 1 1.0\n")
+    print("[case] server: tuning table changed between submission and admission", flush=True)
+    try:
+        with mrustsc_amd.Server(slots=1) as server:
+            t_first = server.submit(first, 15)
+            tickets = [server.submit(x, 15) for x in small]
+            mrustsc_amd.tune_load(table)
+            server.wait(t_first)
+            for (ids_a, dist_a), t in zip(alone, tickets):
+                ids, dist, _ = server.wait(t)
+                np.testing.assert_array_equal(ids, ids_a)
+                np.testing.assert_array_equal(dist, dist_a)
+        # The retuned shape itself gives the same graph.
+        ids, dist, st = mrustsc_amd.knn_hip(small[0], 15, context=context, stats=True)
+        assert st["tuned"] == 1 and st["tile"] == 512
+        np.testing.assert_array_equal(ids, alone[0][0])
+        np.testing.assert_array_equal(dist, alone[0][1])
+    finally:
+        mrustsc_amd.tune_clear()
