@@ -124,6 +124,25 @@ scrust_hip_status scrust_hip_knn_device(scrust_hip_context* context, const float
                                         uint32_t* out_indices, float* out_distances,
                                         scrust_hip_knn_stats* stats);
 
+/* Bytes of the device scratch array scrust_hip_knn_device_noalloc works in (256 MiB). */
+#define SCRUST_HIP_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). As
+ * scrust_hip_knn_device (same arguments, checks, results and errors), but the call's
+ * workspace is a file-scope device array of SCRUST_HIP_NOALLOC_SCRATCH_BYTES that each
+ * device holds until the process ends, so the call allocates nothing. When the default work
+ * shape does not fit, the build takes the compact one (128-slot candidate buffer, one
+ * reference range; the same graph); a build that fits in neither (past about 120,000 cells
+ * at d = 50, k = 15) fails with INVALID_PARAMETER. Every call on a device shares that
+ * device's array, which keeps the last call's contents: calls that overlap on one device
+ * race. */
+scrust_hip_status scrust_hip_knn_device_noalloc(scrust_hip_context* context,
+                                                const float* embedding, size_t n, size_t d,
+                                                size_t k, const scrust_hip_knn_options* options,
+                                                uint32_t* out_indices, float* out_distances,
+                                                scrust_hip_knn_stats* stats);
+
 /* ---------------------------------------------------------------------------------------
  * Connectivities: UMAP's fuzzy simplicial set of a kNN graph, the weighted graph scanpy
  * stores in obsp["connectivities"]. Replaces scrust_core::neighbors::connectivities

@@ -27,6 +27,8 @@ __all__ = [
     "knn",
     "knn_hip",
     "knn_device",
+    "knn_device_noalloc",
+    "NOALLOC_SCRATCH_BYTES",
     "connectivities",
     "connectivities_device",
     "neighbors",
@@ -185,6 +187,25 @@ def knn_device(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int, dis
     ctx = context if context is not None else _thread_context()
     st = KnnStats()
     status = _native.load().scrust_hip_knn_device(
+        ctx.handle, ctypes.c_void_p(embedding_ptr), n, d, k, ctypes.byref(opts),
+        ctypes.c_void_p(indices_ptr), ctypes.c_void_p(distances_ptr), ctypes.byref(st))
+    _native.raise_for(status)
+    return st.as_dict()
+
+
+NOALLOC_SCRATCH_BYTES = 256 << 20  # SCRUST_HIP_NOALLOC_SCRATCH_BYTES
+
+
+def knn_device_noalloc(embedding_ptr: int, n: int, d: int, k: int, indices_ptr: int,
+                       distances_ptr: int, *, metric: str = "euclidean", precision: str = "bf16",
+                       context: Context | None = None, tile: int = 0, order: str = "auto") -> dict:
+    """As ``knn_device``, but the call allocates no device memory: its workspace is the
+    library's per-device scratch array of ``NOALLOC_SCRATCH_BYTES`` (a build that does not
+    fit raises ValueError). Calls that overlap on one device race. Returns the stats."""
+    opts = _options(metric, precision, tile, order)
+    ctx = context if context is not None else _thread_context()
+    st = KnnStats()
+    status = _native.load().scrust_hip_knn_device_noalloc(
         ctx.handle, ctypes.c_void_p(embedding_ptr), n, d, k, ctypes.byref(opts),
         ctypes.c_void_p(indices_ptr), ctypes.c_void_p(distances_ptr), ctypes.byref(st))
     _native.raise_for(status)

@@ -38,6 +38,7 @@ EXPORTS = (
     "scrust_hip_knn",
     "scrust_hip_knn_ex",
     "scrust_hip_knn_device",
+    "scrust_hip_knn_device_noalloc",
     "scrust_hip_connectivities",
     "scrust_hip_connectivities_device",
     "scrust_hip_neighbors",
@@ -246,6 +247,8 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_knn_ex.restype = ctypes.c_int
     lib.scrust_hip_knn_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
     lib.scrust_hip_knn_device.restype = ctypes.c_int
+    lib.scrust_hip_knn_device_noalloc.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
+    lib.scrust_hip_knn_device_noalloc.restype = ctypes.c_int
     lib.scrust_hip_connectivities.argtypes = [vp, vp, vp, sz, sz, vp, vp, vp, sz, ctypes.POINTER(sz)]
     lib.scrust_hip_connectivities.restype = ctypes.c_int
     lib.scrust_hip_connectivities_device.argtypes = [vp, vp, vp, sz, sz, vp, vp, vp, sz, ctypes.POINTER(sz)]

@@ -133,14 +133,38 @@ static scrust_hip_status read_status(const Plan& plan, hipStream_t stream, uint3
 scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
                              const scrust_hip_knn_options* options, uint32_t* out_indices,
                              float* out_distances, scrust_hip_knn_stats* stats,
-                             const TuneEntry* force) {
+                             const TuneEntry* force, const Workspace* workspace) {
     std::vector<SetSpec> specs{SetSpec{x, n, d, k, out_indices, out_distances}};
     Plan plan;
     scrust_hip_status st = plan_layout(specs, options, plan, ctx->compute_units, true, force);
     if (st != SCRUST_HIP_OK) return st;
-    SCRUST_CHECK(ctx->arena.reserve(plan.arena_bytes), "allocate workspace");
-    SCRUST_CHECK(ctx->meta.reserve(plan.meta_bytes), "allocate metadata");
-    st = plan_place(plan, ctx->arena.ptr, ctx->meta.ptr, ctx->stream);
+    void* arena = nullptr;
+    void* meta = nullptr;
+    if (workspace) {
+        // Arena first, metadata after it at a 256-byte boundary.
+        auto need_of = [](const Plan& p) { return (p.arena_bytes + 255) / 256 * 256 + p.meta_bytes; };
+        if (need_of(plan) > workspace->bytes && !force) {
+            // The compact work shape (the smallest candidate buffer, one reference range,
+            // automatic row order): the same graph in the least memory.
+            const TuneEntry compact{128u, 1u, 0u, 0.0f};
+            st = plan_layout(specs, options, plan, ctx->compute_units, true, &compact);
+            if (st != SCRUST_HIP_OK) return st;
+        }
+        const size_t meta_at = (plan.arena_bytes + 255) / 256 * 256;
+        const size_t need = meta_at + plan.meta_bytes;
+        if (need > workspace->bytes)
+            return invalid("embedding", ("a build whose workspace fits the " + std::to_string(workspace->bytes) +
+                                         "-byte scratch array (this one needs " + std::to_string(need) + " bytes)").c_str(),
+                           std::to_string(n) + " x " + std::to_string(d));
+        arena = workspace->ptr;
+        meta = static_cast<char*>(workspace->ptr) + meta_at;
+    } else {
+        SCRUST_CHECK(ctx->arena.reserve(plan.arena_bytes), "allocate workspace");
+        SCRUST_CHECK(ctx->meta.reserve(plan.meta_bytes), "allocate metadata");
+        arena = ctx->arena.ptr;
+        meta = ctx->meta.ptr;
+    }
+    st = plan_place(plan, arena, meta, ctx->stream);
     if (st != SCRUST_HIP_OK) return st;
     st = plan_enqueue(plan, true, ctx->compute_units, ctx->stream, ctx->events);
     if (st != SCRUST_HIP_OK) return st;
@@ -387,6 +411,9 @@ scrust_hip_status scrust_hip_context_create(int device, scrust_hip_context** out
         delete ctx;
         return hip_failure(err, "create context");
     }
+    // The allocation-free entry point's scratch address on this device (once per device; a
+    // failure is reported by that entry point, the others do not need it).
+    noalloc_setup(device);
     *out = ctx;
     return SCRUST_HIP_OK;
 }

@@ -139,11 +139,23 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
                                hipStream_t side = nullptr, hipEvent_t fork = nullptr,
                                hipEvent_t join = nullptr);
 
+// Caller-provided device memory for a single call's arena and launch metadata.
+struct Workspace {
+    void* ptr;
+    size_t bytes;
+};
+
 // One embedding already on the device, synchronously (the body of scrust_hip_knn_device).
+// With `workspace`, the plan is placed there (no allocation; a plan that does not fit is an
+// INVALID_PARAMETER error); otherwise in the context's grow-only buffers.
 scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, size_t d, size_t k,
                              const scrust_hip_knn_options* options, uint32_t* out_indices,
                              float* out_distances, scrust_hip_knn_stats* stats,
-                             const TuneEntry* force = nullptr);
+                             const TuneEntry* force = nullptr, const Workspace* workspace = nullptr);
+
+// Takes the device address of the allocation-free entry point's scratch array on `device`
+// (the current device), once per device; called at context creation.
+void noalloc_setup(int device);
 
 }  // namespace scrust_hip
 
