@@ -7,12 +7,13 @@ mixture with 0.5% duplicate rows); its headline `value` is the BF16+rerank path,
 plain FP32 kernel, the cosine metric and the host-to-host rate are measured beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
-                    [--n N] [--no-cpu] [--no-fp32]
+                    [--n N] [--no-cpu] [--no-fp32] [--no-extra]
 
 Every configuration prints ONE JSON line:
     c1  2,638 x 50 (8 clusters, 20 duplicate rows), full CPU baseline (best of 3)
     c2  100,000 x 50 mixture, full CPU baseline
-    c3  1,000,000 x 50 mixture (default): BF16 vs FP32, cosine, distance errors
+    c3  1,000,000 x 50 mixture (default): BF16 vs FP32, cosine, distance errors; the
+        default run also carries blocks for c1, c2, c4 and c5 (p99 latency) beside it
     c4  64 datasets (n 500-50,000, d 30/50/100, k 15/30) in one graph launch
     c5  serving: Poisson arrivals, n 1,000-200,000, d 50, k 15, three offered loads
 """
@@ -59,6 +60,10 @@ def parse_args():
     p.add_argument("--parity-rows", type=int, default=256)
     p.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
     p.add_argument("--slots", type=int, default=4, help="c5: builds the server runs at once")
+    p.add_argument("--block-requests", type=int, default=150,
+                   help="requests per offered load of the serving block in the default run")
+    p.add_argument("--no-extra", action="store_true",
+                   help="default run: skip the C1, C2, C4 and C5 blocks beside the headline")
     p.add_argument("--gemm-ceiling", action="store_true",
                    help="time the platform's bf16 GEMM (torch / hipBLASLt) as a practical MFMA ceiling")
     p.add_argument("--hbm-probe", type=int, default=0, metavar="MB",
@@ -72,6 +77,38 @@ def parse_args():
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def host_cores() -> int:
+    """CPU cores this process may run on (the CPU baseline uses all of them)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def fixture_check(name: str, x, ids, dist, index: int | None = None) -> dict:
+    """The whole graph against a committed oracle fixture (digests of the oracle's ids and
+    distances over every row, written by tests/golden/make_config_fixtures.py)."""
+    import hashlib
+
+    import numpy as np
+
+    path = GOLDEN / f"{name}_oracle_fixture.json"
+    if not path.exists():
+        return {"fixture": path.name, "checked": False}
+    fix = json.loads(path.read_text())
+    if index is not None:
+        fix = fix["datasets"][index]
+    h = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    return {"fixture": path.name, "checked": True,
+            "embedding_equal": h(x) == fix["x_sha256"],
+            "ids_equal": h(ids.astype(np.uint32)) == fix["ids_sha256"],
+            "distances_equal": h(dist.astype(np.float32)) == fix["dist_sha256"],
+            "rows": int(ids.shape[0])}
 
 
 def measured_traffic(n: int, d: int, k: int):
@@ -210,7 +247,7 @@ def parity_block(x, k, got_ids, got_dist, rows, metric="euclidean"):
 
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()
     t0 = time.time()
     w_ids, w_dist = oracle.knn(x, k, metric=metric, queries=rows, threads=threads)
     f_ids, f_dist = oracle.knn_fp64(x, k, metric=metric, queries=rows, threads=threads)
@@ -309,7 +346,7 @@ def tsne_block(ctx, with_oracle: bool, n: int = 20000, d: int = 50) -> dict:
     if with_oracle:
         import oracle
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cores()
         rows = (0, 512)
         t0 = time.perf_counter()
         want = oracle.tsne_squared_distances(x, rows=rows, threads=threads)
@@ -324,7 +361,7 @@ def tsne_block(ctx, with_oracle: bool, n: int = 20000, d: int = 50) -> dict:
                            "bitwise_fraction": float(np.mean(got == want)),
                            "bar": "|device - oracle| <= (2 d + 8) 2^-24 (|a|^2 + |b|^2)"}
         block["cpu_baseline"] = {"value": (rows[1] - rows[0]) * n / sec, "unit": "entries/s",
-                                 "cores": threads, "kind": "port",
+                                 "cores": threads, "host_cpus": os.cpu_count(), "kind": "port",
                                  "sample": f"rows {rows[0]}-{rows[1]} of the {n} x {n} matrix "
                                            "(tsne.rs:173-188 restated in C)",
                                  "device_entries_per_s": n * n / best}
@@ -338,7 +375,7 @@ def cpu_baseline(x, k, rows=None, repeats=1):
 
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()
     n = x.shape[0]
     q = np.arange(n if rows is None else min(n, rows), dtype=np.uint32)
     best = None
@@ -349,7 +386,8 @@ def cpu_baseline(x, k, rows=None, repeats=1):
         best = sec if best is None else min(best, sec)
     sample = (f"all {n} rows" if rows is None or q.size == n else
               f"{q.size} query rows x all {n} references, scaled: cells/s = rows / seconds")
-    return {"value": q.size / best, "unit": "cells/s", "cores": threads, "kind": "port",
+    return {"value": q.size / best, "unit": "cells/s", "cores": threads,
+            "host_cpus": os.cpu_count(), "kind": "port",
             "sample": sample + (f", best of {repeats}" if repeats > 1 else "") +
             " (expansion path of neighbors.rs restated in C)"}
 
@@ -387,6 +425,9 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
                        "executed_tflops": 2.0 * n * ((n + 255) // 256 * 256) * 32 * ((3 * d + 3 + 31) // 32)
                                           / (dev["pass_ms"] * 1e-3) / 1e12}
     got_ids, got_dist = single.result()
+    fixture = name if args.n == 0 and args.d == 50 and args.k == 15 else None
+    if fixture:
+        out["whole_graph"] = fixture_check(fixture, x, got_ids, got_dist)
     out["connectivities"] = connectivities_block(single, ctx, not args.no_cpu)
 
     if not args.no_fp32:
@@ -413,6 +454,8 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
         out["cosine"] = {"cells_per_s": n / (sum(wc) / len(wc)),
                          "ms_per_step": sum(wc) / len(wc) * 1e3,
                          "fallback_rows": int(stc[-1]["fallback_rows"])}
+        if fixture == "c3":
+            out["cosine"]["whole_graph"] = fixture_check("c3_cosine", x, c_ids, c_dist)
         if args.parity_rows > 0:
             out["cosine"]["parity"] = parity_block(x, k, c_ids, c_dist, rows, "cosine")
 
@@ -442,6 +485,42 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
     torch.cuda.synchronize()
     ctx.close()
     return out
+
+
+def config_block(args, name, x, workload, ctx, cpu_repeats=1):
+    """One single-embedding workload beside the headline: bf16 and fp32 builds of the
+    device-resident embedding (best of `args.steps` after warm-ups), the whole graph against
+    its oracle fixture, the pass's roofline and the CPU baseline on every row."""
+    import numpy as np
+
+    n, d = x.shape
+    k = 15
+    single = Single(x, k, ctx)
+    walls, st = single.timed(args.steps, args.warmup, "bf16")
+    best = min(walls)
+    dev = stage_means(st)
+    flops = 2.0 * n * n * d
+    achieved = flops / (dev["pass_ms"] * 1e-3) / 1e12
+    ids, dist = single.result()
+    block = {"workload": workload, "cells_per_s": n / (sum(walls) / len(walls)),
+             "cells_per_s_best": n / best, "ms_per_step": sum(walls) / len(walls) * 1e3,
+             "stages_ms": dev, "fallback_rows": int(st[-1]["fallback_rows"]),
+             "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
+                          "kernel": "pass_kernel (bf16)", "algorithmic_flops": flops},
+             "whole_graph": fixture_check(name, x, ids, dist)}
+    if not args.no_fp32:
+        w32, st32 = single.timed(max(2, args.fp32_steps), 1, "fp32")
+        ids32, dist32 = single.result()
+        pass32 = stage_means(st32)["pass_ms"]
+        block["fp32"] = {"cells_per_s": n / (sum(w32) / len(w32)), "pass_ms": pass32,
+                         "roofline_frac": flops / (pass32 * 1e-3) / 1e12 / FP32_MATRIX_PEAK_TFLOPS,
+                         "identical_to_bf16": bool(np.array_equal(ids32, ids) and np.array_equal(dist32, dist))}
+        block["bf16_over_fp32"] = block["cells_per_s"] / block["fp32"]["cells_per_s"]
+    if not args.no_cpu:
+        block["cpu_baseline"] = cpu_baseline(x, k, None, cpu_repeats)
+    del single
+    return block
 
 
 def run_batch_config(args):
@@ -496,13 +575,15 @@ def run_batch_config(args):
         out = {}
         sec, devs = time_batch(batch, "tuned")
         # Each dataset against its own standalone run and the untuned batch.
-        equal = equal_untuned = 0
-        for sp, (x, dx), (di, dd), b_ids, b_dist in zip(specs, data, outs, base_ids, base_dist):
+        equal = equal_untuned = equal_oracle = 0
+        for i, (sp, (x, dx), (di, dd), b_ids, b_dist) in enumerate(zip(specs, data, outs, base_ids, base_dist)):
             ids = di.cpu().numpy().view(np.uint32)
             dist = dd.cpu().numpy()
             w_ids, w_dist = mrustsc_amd.knn_hip(x, sp["k"], context=ctx)
             equal += int(np.array_equal(ids, w_ids) and np.array_equal(dist, w_dist))
             equal_untuned += int(np.array_equal(ids, b_ids) and np.array_equal(dist, b_dist))
+            fc = fixture_check("c4", x, ids, dist, index=i)
+            equal_oracle += int(fc.get("ids_equal", False) and fc.get("distances_equal", False))
         # The same 64 builds as separate device-resident calls, for comparison.
         seq = []
         for _ in range(2):
@@ -521,6 +602,7 @@ def run_batch_config(args):
         out["device_ms_best"] = min(devs)
         out["datasets_equal_to_standalone"] = equal
         out["datasets_equal_to_untuned"] = equal_untuned
+        out["datasets_equal_to_oracle_fixture"] = equal_oracle
         out["tuning"] = {**tuned, "seconds": tune_sec, "untuned_ms": untuned_sec * 1e3,
                          "untuned_device_ms_best": min(untuned_devs),
                          "tuned_over_untuned": untuned_sec / sec,
@@ -532,11 +614,24 @@ def run_batch_config(args):
                            "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
                            "traffic": None, "kernel": "whole batch (device time)",
                            "algorithmic_flops": flops}
+    if not args.no_cpu:
+        import oracle
+
+        threads = host_cores()
+        t0 = time.perf_counter()
+        for sp, (x, _) in zip(specs, data):
+            oracle.baseline_knn(x, sp["k"], threads=threads)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cells / sec, "unit": "cells/s", "cores": threads,
+                               "host_cpus": os.cpu_count(), "kind": "port",
+                               "sample": "all 64 datasets, one after another (expansion path of "
+                                         "neighbors.rs restated in C), summed"}
+    del data, outs
     ctx.close()
     return out
 
 
-def run_serving_config(args):
+def run_serving_config(args, requests: int | None = None):
     """Poisson arrivals at three offered loads; latency from arrival to host result."""
     import numpy as np
 
@@ -571,7 +666,7 @@ def run_serving_config(args):
         loads = []
         for frac in (0.5, 0.8, 1.1):
             rate = frac * sat
-            count = args.requests
+            count = requests or args.requests
             gaps = rng.exponential(1.0 / rate, size=count)
             picks = rng.integers(0, len(pool), size=count)
             # Arrivals and the server's timestamps share CLOCK_MONOTONIC (steady_clock).
@@ -595,6 +690,7 @@ def run_serving_config(args):
                           "p99_ms": float(np.percentile(lat, 99))})
             log(f"[bench] load {frac}: {loads[-1]}")
         stats = server.stats()
+    del resident
     mid = loads[1]
     out.update({"metric": METRIC + " (serving)", "unit": "requests/s",
                 "value": mid["throughput_rps"], "ms_per_step": mid["p99_ms"],
@@ -604,6 +700,40 @@ def run_serving_config(args):
                                        f"k=15, device-resident embeddings, results to host, {args.slots} slots",
                            "pool": len(pool)}})
     return out
+
+
+def extra_blocks(args, out: dict) -> None:
+    """The other workloads of the metric beside the 1M headline: C1 and C2 single builds,
+    the 64-dataset batch (C4) and the serving loop's latency at three offered loads (C5)."""
+    import mrustsc_amd
+    from mrustsc_amd import synthetic
+
+    ctx = mrustsc_amd.Context(0)
+    t0 = time.time()
+    out["c1"] = config_block(args, "c1", synthetic.pbmc_like(),
+                             "2638 x 50, 8 clusters (sd 5) + N(0, I), 20 duplicate rows, k=15", ctx,
+                             cpu_repeats=3)
+    log(f"[bench] c1 block in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    out["c2"] = config_block(args, "c2", synthetic.clustered(100_000, 50, seed=1),
+                             "clustered mixture 100000 x 50, k=15, euclidean", ctx)
+    log(f"[bench] c2 block in {time.time() - t0:.1f}s")
+    ctx.close()
+    t0 = time.time()
+    b = run_batch_config(args)
+    out["batch_c4"] = {key: b[key] for key in b if key not in (
+        "metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "data")}
+    log(f"[bench] c4 block in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    c5 = run_serving_config(args, requests=args.block_requests)
+    out["serving_c5"] = {key: c5[key] for key in ("saturation_rps", "loads", "server", "config", "p50_ms", "p99_ms")}
+    mid = c5["loads"][1]
+    out["p99_request_latency_ms"] = {
+        "value": mid["p99_ms"], "p50_ms": mid["p50_ms"], "offered_rps": mid["offered_rps"],
+        "fraction_of_saturation": mid["fraction_of_saturation"], "throughput_rps": mid["throughput_rps"],
+        "note": "serving loop (C5), Poisson arrivals at 0.8 of the measured saturation; "
+                "latency from arrival to the result in host memory; all three loads under serving_c5"}
+    log(f"[bench] c5 block in {time.time() - t0:.1f}s")
 
 
 def main() -> int:
@@ -646,6 +776,8 @@ def main() -> int:
         out = run_single_config(args, args.config, x, workload, cpu_rows, repeats)
         out["config"].update({"clusters": 8 if args.config == "c1" else 20,
                               "duplicate_fraction": 0.0076 if args.config == "c1" else 0.005})
+        if args.config == "c3" and args.n == 0 and not args.no_extra:
+            extra_blocks(args, out)
     print(json.dumps(out), flush=True)
     return 0
 

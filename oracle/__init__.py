@@ -49,6 +49,9 @@ def _load():
     lib.oracle_knn.argtypes = [f32p, sz, sz, sz, ctypes.c_int, ctypes.c_int, u32p, sz,
                                ctypes.c_int, u32p, f32p]
     lib.oracle_knn.restype = ctypes.c_int
+    lib.oracle_knn_fast.argtypes = [f32p, sz, sz, sz, ctypes.c_int, u32p, sz, ctypes.c_int, u32p,
+                                    f32p]
+    lib.oracle_knn_fast.restype = ctypes.c_int
     lib.oracle_knn_fp64.argtypes = [f32p, sz, sz, sz, ctypes.c_int, u32p, sz, ctypes.c_int,
                                     u32p, f64p]
     lib.oracle_knn_fp64.restype = ctypes.c_int
@@ -126,6 +129,23 @@ def knn(x, k: int, *, metric: str = "euclidean", tie_rule: bool = True, queries=
     rc = _load().oracle_knn(_ptr(x, ctypes.c_float), n, d, k, METRICS[metric], int(tie_rule),
                             None if q is None else _ptr(q, ctypes.c_uint32), nq, threads,
                             _ptr(idx, ctypes.c_uint32), _ptr(dist, ctypes.c_float))
+    if rc != 0:
+        raise ValueError(f"invalid arguments: n={n} d={d} k={k}")
+    return idx, dist
+
+
+def knn_fast(x, k: int, *, metric: str = "euclidean", queries=None,
+             threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """knn (tie rule on) vectorised across references: the same output bit for bit, fast
+    enough for whole graphs of 10^6 cells."""
+    x = _f32(x)
+    n, d = x.shape
+    q, nq = _queries(queries, n)
+    idx = np.empty((nq, k), dtype=np.uint32)
+    dist = np.empty((nq, k), dtype=np.float32)
+    rc = _load().oracle_knn_fast(_ptr(x, ctypes.c_float), n, d, k, METRICS[metric],
+                                 None if q is None else _ptr(q, ctypes.c_uint32), nq, threads,
+                                 _ptr(idx, ctypes.c_uint32), _ptr(dist, ctypes.c_float))
     if rc != 0:
         raise ValueError(f"invalid arguments: n={n} d={d} k={k}")
     return idx, dist

@@ -409,6 +409,162 @@ int oracle_knn(const float* x, size_t n, size_t d, size_t k, int metric, int tie
 }
 
 /* ------------------------------------------------------------------------ */
+/* The same oracle, vectorised across references (whole graphs at 10^6).    */
+/*                                                                          */
+/* oracle_knn_fast computes exactly what oracle_knn (tie rule on) computes: */
+/* each pair's squared distance is the same f32 difference and f32 fma      */
+/* chain in dimension order and the same snap (direct_square above, i.e.    */
+/* knn.rs:229-240); only the pairs run side by side, 16 references to an    */
+/* AVX-512 vector and 16 query rows per register block, over references      */
+/* stored dimension-major in blocks of 16. Per row it keeps the k + 32      */
+/* smallest keys exactly as smallest_keys does (references arrive in id     */
+/* order, so a key whose d^2 is not below the kept maximum is never kept)   */
+/* and finishes through the same tie rule; a row whose tie band needs more  */
+/* keys than were kept is recomputed by oracle_row. Without AVX-512 it is   */
+/* oracle_knn.                                                              */
+/* ------------------------------------------------------------------------ */
+
+#if defined(__x86_64__) && defined(__GNUC__)
+#include <immintrin.h>
+#define FAST_W 16
+#define FAST_Q 16
+
+typedef struct {
+    const float* coords;
+    const float* norms;
+    const float* tblock; /* n_blocks x d x 16, references dimension-major per block */
+    const float* tnorm;  /* n_blocks x 16, +inf past n */
+    size_t n, d, k, n_queries, n_blocks;
+    int metric;
+    const uint32_t* queries;
+    uint32_t* out_indices;
+    float* out_distances;
+} fast_ctx;
+
+/* Insert `key` into the ascending list best[0..*count) of at most `keep` keys. */
+static inline void keep_key(uint64_t* best, size_t* count, size_t keep, uint64_t key) {
+    if (*count == keep && key >= best[keep - 1]) return;
+    size_t slot = *count < keep ? (*count)++ : keep - 1;
+    while (slot > 0 && best[slot - 1] > key) {
+        best[slot] = best[slot - 1];
+        slot--;
+    }
+    best[slot] = key;
+}
+
+__attribute__((target("avx512f"))) static void fast_rows(void* vctx, size_t item, void* scratch) {
+    fast_ctx* c = (fast_ctx*)vctx;
+    const size_t d = c->d, k = c->k, n = c->n;
+    const size_t keep = k + 32 < n - 1 ? k + 32 : n - 1;
+    const size_t first = item * FAST_Q;
+    size_t rows = c->n_queries - first < FAST_Q ? c->n_queries - first : FAST_Q;
+    size_t q[FAST_Q];
+    float qn[FAST_Q];
+    for (size_t i = 0; i < FAST_Q; i++) {
+        size_t at = first + (i < rows ? i : 0);
+        q[i] = c->queries ? c->queries[at] : at;
+        qn[i] = c->norms[q[i]];
+    }
+    uint64_t* best = (uint64_t*)scratch; /* FAST_Q x keep */
+    size_t count[FAST_Q] = {0};
+    float worst[FAST_Q];
+    for (size_t i = 0; i < FAST_Q; i++) worst[i] = INFINITY;
+    const float scale = ((float)d + 2.0f) * FLT_EPSILON; /* exact: an integer times 2^-23 */
+    const __m512 vscale = _mm512_set1_ps(scale);
+    const __m512i lane = _mm512_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+    for (size_t b = 0; b < c->n_blocks; b++) {
+        const float* t = c->tblock + b * d * FAST_W;
+        __m512 acc[FAST_Q];
+        for (size_t i = 0; i < FAST_Q; i++) acc[i] = _mm512_setzero_ps();
+        for (size_t j = 0; j < d; j++) {
+            const __m512 r = _mm512_loadu_ps(t + j * FAST_W);
+            for (size_t i = 0; i < FAST_Q; i++) {
+                const __m512 delta = _mm512_sub_ps(_mm512_set1_ps(c->coords[q[i] * d + j]), r);
+                acc[i] = _mm512_fmadd_ps(delta, delta, acc[i]);
+            }
+        }
+        const __m512 rn = _mm512_loadu_ps(c->tnorm + b * FAST_W);
+        const __m512i ids = _mm512_add_epi32(_mm512_set1_epi32((int)(b * FAST_W)), lane);
+        for (size_t i = 0; i < rows; i++) {
+            /* snap: s < ((d + 2) eps) * (|a|^2 + |b|^2) -> 0, left to right in f32 */
+            const __m512 th = _mm512_mul_ps(vscale, _mm512_add_ps(_mm512_set1_ps(qn[i]), rn));
+            const __mmask16 snap = _mm512_cmp_ps_mask(acc[i], th, _CMP_LT_OQ);
+            const __m512 v = _mm512_mask_blend_ps(snap, acc[i], _mm512_setzero_ps());
+            __mmask16 m = count[i] < keep ? (__mmask16)0xFFFF
+                                          : _mm512_cmp_ps_mask(v, _mm512_set1_ps(worst[i]), _CMP_LT_OQ);
+            m &= (__mmask16)(b * FAST_W + FAST_W <= n ? 0xFFFF : ((1u << (n - b * FAST_W)) - 1u));
+            m &= (__mmask16)~_mm512_cmpeq_epi32_mask(ids, _mm512_set1_epi32((int)q[i]));
+            if (!m) continue;
+            float vals[FAST_W];
+            _mm512_storeu_ps(vals, v);
+            while (m) {
+                const int l = __builtin_ctz(m);
+                m &= (__mmask16)(m - 1);
+                keep_key(best + i * keep, &count[i], keep, make_key(vals[l], (uint32_t)(b * FAST_W + l)));
+            }
+            if (count[i] == keep) worst[i] = unordered((uint32_t)(best[i * keep + keep - 1] >> 32));
+        }
+    }
+    for (size_t i = 0; i < rows; i++) {
+        const size_t out = first + i;
+        const uint64_t* mine = best + i * keep;
+        uint32_t ids[4096];
+        float d2s[4096];
+        for (size_t j = 0; j < k; j++) {
+            ids[j] = (uint32_t)mine[j];
+            d2s[j] = unordered((uint32_t)(mine[j] >> 32));
+        }
+        if (apply_tie_rule(mine, count[i], k, n - 1, c->coords, d, c->coords + q[i] * d, ids, d2s)) {
+            /* The band may run past the kept keys: the scalar oracle for this row. */
+            oracle_ctx oc = {c->coords, c->norms, n, d, k, c->metric, 1, c->queries,
+                             c->out_indices, c->out_distances};
+            uint64_t* keys = (uint64_t*)malloc(n * sizeof(uint64_t));
+            oracle_row(&oc, out, keys);
+            free(keys);
+            continue;
+        }
+        for (size_t j = 0; j < k; j++) {
+            c->out_indices[out * k + j] = ids[j];
+            c->out_distances[out * k + j] = c->metric == ORACLE_COSINE ? 0.5f * d2s[j] : sqrtf(d2s[j]);
+        }
+    }
+}
+#endif
+
+int oracle_knn_fast(const float* x, size_t n, size_t d, size_t k, int metric,
+                    const uint32_t* queries, size_t n_queries, int threads,
+                    uint32_t* out_indices, float* out_distances) {
+#if defined(__x86_64__) && defined(__GNUC__)
+    if (n == 0 || d == 0 || k == 0 || k >= n || k > 4096) return -1;
+    if (!__builtin_cpu_supports("avx512f"))
+        return oracle_knn(x, n, d, k, metric, 1, queries, n_queries, threads, out_indices, out_distances);
+    if (!queries) n_queries = n;
+    float* coords = (float*)malloc(n * d * sizeof(float));
+    float* norms = (float*)malloc(n * sizeof(float));
+    oracle_prepare(x, n, d, metric, coords, norms);
+    const size_t n_blocks = (n + FAST_W - 1) / FAST_W;
+    float* tblock = (float*)calloc(n_blocks * d * FAST_W, sizeof(float));
+    float* tnorm = (float*)malloc(n_blocks * FAST_W * sizeof(float));
+    for (size_t r = 0; r < n_blocks * FAST_W; r++) {
+        tnorm[r] = r < n ? norms[r] : INFINITY;
+        if (r < n)
+            for (size_t j = 0; j < d; j++) tblock[(r / FAST_W) * d * FAST_W + j * FAST_W + r % FAST_W] = coords[r * d + j];
+    }
+    fast_ctx c = {coords, norms, tblock, tnorm, n, d, k, n_queries, n_blocks, metric,
+                  queries, out_indices, out_distances};
+    const size_t keep = k + 32 < n - 1 ? k + 32 : n - 1;
+    parallel_for(fast_rows, &c, (n_queries + FAST_Q - 1) / FAST_Q, FAST_Q * keep * sizeof(uint64_t), threads);
+    free(tblock);
+    free(tnorm);
+    free(coords);
+    free(norms);
+    return 0;
+#else
+    return oracle_knn(x, n, d, k, metric, 1, queries, n_queries, threads, out_indices, out_distances);
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
 /* fp64 ground truth (tests/test_neighbors_audit.py:52-61).                 */
 /* ------------------------------------------------------------------------ */
 

@@ -46,6 +46,13 @@ int oracle_knn(const float* x, size_t n, size_t d, size_t k, int metric, int tie
                const uint32_t* queries, size_t n_queries, int threads, uint32_t* out_indices,
                float* out_distances);
 
+/* oracle_knn with the tie rule, vectorised across references (AVX-512; oracle_knn itself
+ * on CPUs without it): the same arithmetic per pair and the same selection, so the same
+ * output bit for bit, fast enough for whole graphs of 10^6 cells. */
+int oracle_knn_fast(const float* x, size_t n, size_t d, size_t k, int metric,
+                    const uint32_t* queries, size_t n_queries, int threads,
+                    uint32_t* out_indices, float* out_distances);
+
 /* Ground truth in f64 on the raw input (no centring): Euclidean distance, or
  * cosine distance 1 - x.y/(|x||y|). Ties by the smaller id. */
 int oracle_knn_fp64(const float* x, size_t n, size_t d, size_t k, int metric,

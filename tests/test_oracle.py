@@ -304,3 +304,66 @@ def test_oracle_orders_overflowed_distances_by_id():
         want = (twins + others)[:k]
         assert ids[q].tolist() == want, q
         assert all(dist[q][j] == (0.0 if want[j] in twins else np.inf) for j in range(k))
+
+
+@pytest.mark.parametrize("case", ["clustered", "cosine", "lattice", "duplicates", "tiny", "k_large"])
+def test_vectorised_oracle_equals_the_scalar_oracle(case):
+    # oracle.knn_fast runs 16 references per vector and 16 rows per block; each pair's f32
+    # difference / fma chain / snap and the kept-key selection are the scalar oracle's, so
+    # every id and distance bit must agree, ties and bands included.
+    metric, k = "euclidean", 15
+    if case == "clustered":
+        x = synthetic.clustered(3000, 50, seed=3)
+    elif case == "cosine":
+        x, metric, k = synthetic.clustered(2000, 7, seed=4), "cosine", 30
+    elif case == "lattice":  # heavy exact ties and tie bands
+        x = np.floor(np.random.default_rng(1).uniform(0, 4, (517, 3))).astype(np.float32)
+        k = 5
+    elif case == "duplicates":
+        x = synthetic.clustered(1500, 20, seed=5, duplicate_fraction=0.2)
+    elif case == "tiny":  # n = k + 1, fewer rows than a block, a ragged last block
+        x, k = synthetic.clustered(21, 4, seed=6), 20
+    else:
+        x, k = synthetic.clustered(700, 9, seed=7), 100
+    want = oracle.knn(x, k, metric=metric)
+    got = oracle.knn_fast(x, k, metric=metric)
+    np.testing.assert_array_equal(got[0], want[0])
+    np.testing.assert_array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
+    rows = np.array([0, 3, len(x) - 1], dtype=np.uint32)
+    sub = oracle.knn_fast(x, k, metric=metric, queries=rows)
+    np.testing.assert_array_equal(sub[0], want[0][rows])
+    np.testing.assert_array_equal(sub[1], want[1][rows])
+
+
+@pytest.mark.parametrize("name,n,metric", [("c2", 100_000, "euclidean"), ("c3", 1_000_000, "euclidean"),
+                                           ("c3_cosine", 1_000_000, "cosine")])
+def test_workload_fixtures_hold_the_scalar_oracles_rows(name, n, metric):
+    # The whole-graph fixtures were written with the vectorised oracle; their rows in full
+    # must be the scalar oracle's, on the embedding the generator reproduces.
+    import hashlib
+
+    fix = json.loads((Path(__file__).parent / "golden" / f"{name}_oracle_fixture.json").read_text())
+    x = synthetic.clustered(n, 50, seed=1)
+    assert hashlib.sha256(x.tobytes()).hexdigest() == fix["x_sha256"]
+    rows = np.array(sorted(int(r) for r in fix["rows"]), dtype=np.uint32)
+    ids, dist = oracle.knn(x, fix["k"], metric=metric, queries=rows)
+    for i, r in enumerate(rows):
+        row = fix["rows"][str(r)]
+        assert ids[i].tolist() == row["ids"]
+        assert [f"{b:08x}" for b in dist[i].view(np.uint32)] == row["dist_bits"]
+
+
+def test_batch_fixture_holds_the_scalar_oracles_rows():
+    import hashlib
+
+    fix = json.loads((Path(__file__).parent / "golden" / "c4_oracle_fixture.json").read_text())
+    specs = synthetic.batch_specs(64, seed=7)
+    assert [rec["spec"] for rec in fix["datasets"]] == specs
+    for spec, rec in zip(specs[::8], fix["datasets"][::8]):
+        x = synthetic.batch_dataset(spec)
+        assert hashlib.sha256(x.tobytes()).hexdigest() == rec["x_sha256"]
+        rows = np.array(sorted(int(r) for r in rec["rows"]), dtype=np.uint32)
+        ids, dist = oracle.knn(x, spec["k"], queries=rows)
+        for i, r in enumerate(rows):
+            assert ids[i].tolist() == rec["rows"][str(r)]["ids"]
+            assert [f"{b:08x}" for b in dist[i].view(np.uint32)] == rec["rows"][str(r)]["dist_bits"]
