@@ -45,7 +45,13 @@ typedef enum scrust_hip_precision {
     /* The same pipeline with an fp32 MFMA candidate pass: the plain FP32 kernel. */
     SCRUST_HIP_FP32 = 1,
     /* fp32 direct differences over every pair (no MFMA); the path for k > 47 or d > 128. */
-    SCRUST_HIP_EXACT = 2
+    SCRUST_HIP_EXACT = 2,
+    /* fp16 single products of locally centred coordinates on MFMA (one origin per 512 query
+     * rows of the spatial row order, fp32 accumulate, a rigorous per-pair error term folded
+     * into the product) select the candidates; the same rerank and certificate follow. Single
+     * builds whose rows are grouped (from 65,536 cells, or order = 2), d <= 125; other shapes
+     * take the bf16 path. No counterpart in the reference. */
+    SCRUST_HIP_FP16 = 3
 } scrust_hip_precision;
 
 typedef struct scrust_hip_context scrust_hip_context;

@@ -25,7 +25,8 @@ METRICS = {"euclidean": EUCLIDEAN, "cosine": COSINE}
 BF16 = 0
 FP32 = 1
 EXACT = 2
-PRECISIONS = {"bf16": BF16, "fp32": FP32, "exact": EXACT}
+FP16 = 3
+PRECISIONS = {"bf16": BF16, "fp32": FP32, "exact": EXACT, "fp16": FP16}
 PRECISION_NAMES = {v: k for k, v in PRECISIONS.items()}
 
 # Every entry point include/scrust_hip.h declares.

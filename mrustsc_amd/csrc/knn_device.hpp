@@ -126,6 +126,32 @@ __device__ __forceinline__ float bf16_value(uint16_t h) {
 }
 
 // ---------------------------------------------------------------------------
+// Local-origin fp16 operands (the fp16 candidate pass): per-dataset scale.
+// ---------------------------------------------------------------------------
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// sigma = 2^s with sqrt(max_norm) * sigma in [2^12, 2^13): every shifted, scaled coordinate
+// |c - o| * sigma (o a mean of rows, so |o| <= sqrt(max_norm)) stays below 2^14, far inside
+// fp16's range, and its squared norm below 2^28. Clamped to [2^-100, 2^100]; 1 when every
+// norm is 0. A power of two, so scaling is exact.
+__device__ __forceinline__ float local_scale(float max_norm) {
+    if (!(max_norm > 0.0f)) return 1.0f;
+    int e = 0;
+    (void)__builtin_frexpf(__builtin_sqrtf(max_norm), &e);  // sqrt(max_norm) < 2^e
+    int s = 13 - e;
+    s = s < -100 ? -100 : (s > 100 ? 100 : s);
+    return __builtin_ldexpf(1.0f, s);
+}
+
+// The smallest fp16 at or above v (v >= 0 finite, below 65504).
+__device__ __forceinline__ _Float16 f16_up(float v) {
+    _Float16 h = static_cast<_Float16>(v);
+    if (static_cast<float>(h) < v) h = __builtin_bit_cast(_Float16, static_cast<uint16_t>(__builtin_bit_cast(uint16_t, h) + 1u));
+    return h;
+}
+
+// ---------------------------------------------------------------------------
 // Exact sums of f32 values: each f32 is an integer multiple of 2^-149; the sum is
 // kept as a two's-complement integer in 32-bit digits held in int64 words and is
 // rounded once to f64 at the end, so any summation order gives the same value.
@@ -269,6 +295,70 @@ __device__ __forceinline__ uint64_t segment_sort(uint64_t key, int lane) {
         }
     }
     return key;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level selection of the smallest keys (ballot radix select).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Place of the j-th kept key: out[j], or with quarter > 0 back in the four segments
+// (quarter keys at the start of each segment of `seg` slots).
+__device__ __forceinline__ uint32_t kept_slot(uint32_t j, uint32_t quarter, uint32_t seg) {
+    return quarter ? (j / quarter) * seg + j % quarter : j;
+}
+
+// Keeps the `keep` smallest keys (at least `keep` are valid), written unordered through
+// kept_slot; returns the keep-th smallest score. Ballot radix select: a binary search over
+// the ordered score bits, then over ids among keys that tie on the boundary score.
+template <int E, class Out>
+__device__ float select_keys(const uint64_t (&key)[E], int keep, int lane,
+                             Out out, uint32_t quarter, uint32_t seg) {
+    uint32_t lo = 0, hi = 0xFFFFFFFFu;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        int c = 0;
+#pragma unroll
+        for (int i = 0; i < E; i++) c += __popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) <= mid));
+        if (c >= keep) hi = mid; else lo = mid + 1;
+    }
+    const uint32_t v = lo;
+    int below = 0, equal = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
+        below += __popcll(__ballot(sc < v));
+        equal += __popcll(__ballot(sc == v));
+    }
+    const int need = keep - below;
+    uint32_t idcut = 0xFFFFFFFFu;
+    if (equal > need) {
+        uint32_t l2 = 0, h2 = 0xFFFFFFFFu;
+        while (l2 < h2) {
+            const uint32_t mid = l2 + (h2 - l2) / 2;
+            int c = 0;
+#pragma unroll
+            for (int i = 0; i < E; i++)
+                c += __popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) == v &&
+                                       static_cast<uint32_t>(key[i]) <= mid));
+            if (c >= need) h2 = mid; else l2 = mid + 1;
+        }
+        idcut = l2;
+    }
+    uint32_t base = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
+        const bool kept = sc < v || (sc == v && static_cast<uint32_t>(key[i]) <= idcut);
+        const uint64_t m = __ballot(kept);
+        if (kept) out[kept_slot(base + lanes_below(m), quarter, seg)] = key[i];
+        base += static_cast<uint32_t>(__popcll(m));
+    }
+    return unordered(v);
 }
 
 }  // namespace scrust_hip

@@ -107,8 +107,8 @@ __global__ void prepare_kernel(const DatasetDesc* __restrict__ sets, uint32_t co
         const size_t row = flat - s.prep_begin;
         const int d = static_cast<int>(s.d);
         const int nch = static_cast<int>(s.nch), group = static_cast<int>(s.group);
-        const int op = s.on_pass ? operand : kOperandNone;
-        if (op == kOperandBf16) {
+        const int op = s.on_pass ? (s.local ? kOperandF16Local : operand) : kOperandNone;
+        if (op == kOperandBf16 || op == kOperandF16Local) {
             // prepare_rows_kernel's (coalesced, 64 rows per block).
         } else if (row < s.n) {
             has_norm = true;
@@ -251,10 +251,25 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
     SCRUST_GLOBAL float* coords = global_ptr(s.coords) + static_cast<size_t>(r0) * d;
     for (uint32_t i = tid; i < rows * d; i += 256) coords[i] = xs[i];
     if (static_cast<uint32_t>(tid) < rows) global_ptr(s.norms)[r0 + tid] = nrm_s[tid];
+    const uint32_t prep_rows = ref_rows - r0 < kPrepRows ? ref_rows - r0 : kPrepRows;
+    if (s.local) {
+        // Local-origin pass: the coordinates again, padded with zeros to 32 nch floats per
+        // row (sentinel rows n .. ref_rows - 1 all zero); the pass converts them tile by tile.
+        const int width = 32 * nch;
+        SCRUST_GLOBAL f32x4* src = reinterpret_cast<SCRUST_GLOBAL f32x4*>(global_ptr(s.pack_src) + static_cast<size_t>(r0) * width);
+        for (uint32_t u = tid; u < prep_rows * (width / 4); u += 256) {
+            const uint32_t row = u / (width / 4);
+            const int f0 = static_cast<int>(u - row * (width / 4)) * 4;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) v[e] = (row < rows && f0 + e < d) ? xs[row * d + f0 + e] : 0.0f;
+            src[u] = f32x4{v[0], v[1], v[2], v[3]};
+        }
+        return;
+    }
     // Packed operands (see knn_pass.hip): reference [h | l | h | nh nl nl2 | 0], query
     // [h | h | l | -1/2 x3 | 0], 32 nch features per row, eight per 16-byte chunk.
     const int chunks = 4 * nch;
-    const uint32_t prep_rows = ref_rows - r0 < kPrepRows ? ref_rows - r0 : kPrepRows;
     SCRUST_GLOBAL u32x4* rp = global_ptr(reinterpret_cast<u32x4*>(s.pack_ref)) + static_cast<size_t>(r0) * chunks;
     SCRUST_GLOBAL u32x4* qp = global_ptr(reinterpret_cast<u32x4*>(s.pack_query)) + static_cast<size_t>(r0) * chunks;
     for (uint32_t u = tid; u < prep_rows * chunks; u += 256) {
@@ -305,6 +320,64 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
     }
 }
 
+// Local-origin pass: one workgroup per kLocalRows query rows. The origin o is the mean of
+// the superblock's rows (any f32 value serves: it only shifts both operands of every
+// product); then each row's a'' = (c - o) sigma, computed as fma(c, sigma, -o sigma), which
+// equals sigma fl(c - o) exactly (sigma is a power of two). Query operand, 32 nch fp16
+// features: [a'' | 0 | -2^15 -2^15 q_e] with q_e >= local_c |a''| + local_c2 rounded up;
+// the reference side puts |b''|^2 2^-16 (hi, lo) and an upper bound of |b''| in the same
+// three places (knn_pass.hip). anorm holds a lower bound of |a''|^2.
+__global__ __launch_bounds__(256) void local_queries_kernel(const DatasetDesc* __restrict__ sets) {
+    __shared__ float o_s[kMaxPassD + 32];
+    const DatasetDesc& s = sets[blockIdx.y];
+    if (!s.on_pass || !s.local) return;
+    const uint32_t n = s.n, r0 = blockIdx.x * kLocalRows;
+    if (r0 >= s.npad) return;
+    const int d = static_cast<int>(s.d), width = 32 * static_cast<int>(s.nch);
+    const uint32_t rows = r0 < n ? (n - r0 < kLocalRows ? n - r0 : kLocalRows) : 0u;
+    const int tid = threadIdx.x;
+    const SCRUST_GLOBAL float* coords = global_ptr(s.coords);
+    if (tid < width) {
+        float sum = 0.0f;
+        if (tid < d)
+            for (uint32_t r = 0; r < rows; r++) sum += coords[static_cast<size_t>(r0 + r) * d + tid];
+        const float o = (tid < d && rows > 0) ? sum / static_cast<float>(rows) : 0.0f;
+        o_s[tid] = o;
+        global_ptr(s.origins)[static_cast<size_t>(blockIdx.x) * width + tid] = o;
+    }
+    __syncthreads();
+    const float sigma = local_scale(__uint_as_float(*s.max_norm_bits));
+    for (uint32_t r = tid; r < rows; r += 256) {
+        const uint32_t row = r0 + r;
+        const SCRUST_GLOBAL float* c = coords + static_cast<size_t>(row) * d;
+        float nrm = 0.0f;
+        for (int j = 0; j < d; j++) {
+            const float a = __builtin_fmaf(c[j], sigma, -(o_s[j] * sigma));
+            nrm = __builtin_fmaf(a, a, nrm);
+        }
+        // |a''|^2 within (1 +- 2^-16) of nrm (d fma roundings), so these bound it.
+        const float lower = nrm * (1.0f - 0x1p-15f);
+        const float len_up = __builtin_sqrtf(nrm * (1.0f + 0x1p-15f)) * (1.0f + 0x1p-20f);
+        global_ptr(s.anorm)[row] = lower;
+        const _Float16 qe = f16_up((s.local_c * len_up + s.local_c2) * (1.0f + 0x1p-20f));
+        SCRUST_GLOBAL u32x4* dst = reinterpret_cast<SCRUST_GLOBAL u32x4*>(global_ptr(s.pack_query)) +
+                                   static_cast<size_t>(row) * (width / 8);
+        for (int f0 = 0; f0 < width; f0 += 8) {
+            f16x8 h;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int f = f0 + e;
+                float v = 0.0f;
+                if (f < d) v = __builtin_fmaf(c[f], sigma, -(o_s[f] * sigma));
+                h[e] = static_cast<_Float16>(v);
+                if (f == width - 3 || f == width - 2) h[e] = static_cast<_Float16>(-32768.0f);
+                if (f == width - 1) h[e] = qe;
+            }
+            dst[f0 / 8] = __builtin_bit_cast(u32x4, h);
+        }
+    }
+}
+
 void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsigned* flags,
                         hipStream_t stream) {
     hipLaunchKernelGGL(clear_sums_kernel, dim3(shape.count), dim3(256), 0, stream, sets);
@@ -317,9 +390,12 @@ void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsign
     const unsigned blocks = (shape.total_prep + 255) / 256;
     hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
                        shape.total_prep, shape.metric, shape.operand, flags);
-    if (shape.operand == kOperandBf16 && shape.max_prep_blocks > 0)
+    if ((shape.operand == kOperandBf16 || shape.operand == kOperandF16Local) && shape.max_prep_blocks > 0)
         hipLaunchKernelGGL(prepare_rows_kernel, dim3(shape.max_prep_blocks, 1, shape.count), dim3(256), 0,
                            stream, sets, shape.metric, flags);
+    if (shape.operand == kOperandF16Local && shape.max_local_blocks > 0)
+        hipLaunchKernelGGL(local_queries_kernel, dim3(shape.max_local_blocks, shape.count), dim3(256), 0,
+                           stream, sets);
 }
 
 // ===========================================================================
@@ -368,7 +444,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     key = segment_sort<KP>(key, lane);
 
     // Segment-local bookkeeping (every lane of a segment computes the same values).
-    const uint64_t seg_mask = (KP == 64) ? ~0ull : (((1ull << KP) - 1) << seg);
+    const uint64_t seg_mask = (KP == 64) ? ~0ull : (((1ull << (KP & 63)) - 1) << seg);
     const int n_valid = __popcll(__ballot(key != kNoKey) & seg_mask);
     const bool full = (__ballot(present) & seg_mask) == seg_mask;
     const float my_d = key_value(key);
@@ -426,7 +502,22 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
                                    2.0 * s.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
                                    s.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
         const double cd = (d + 3) * u * 1.01;
-        const double lower = (static_cast<double>(s.theta[p]) + nqd - err) * (1.0 - cd);
+        double lower = (static_cast<double>(s.theta[p]) + nqd - err) * (1.0 - cd);
+        if (s.local) {
+            // Local-origin pass, in units of sigma (local_scale): a reference not kept has
+            // pass score >= theta, and the score is a lower bound of |b''|^2 - 2 a''.b'' up
+            // to the per-row term 2 kq (fp16 rounding below the normal range), so
+            // |a'' - b''|^2 >= theta + |a''|^2 - 2 kq. Then a' = fl(a - o) differs from a - o
+            // by at most u |a - o| per coordinate: |a - b| >= |a' - b'| (1 - 2.02 u) - 2.02 u |a'|.
+            const double sigma = static_cast<double>(local_scale(max_norm));
+            const double an2 = static_cast<double>(s.anorm[p]);
+            const double an_up = __builtin_sqrt(an2 / (1.0 - 0x1p-15) * (1.0 + 0x1p-16)) * (1.0 + 0x1p-40);
+            const double kq = static_cast<double>(s.local_c2) * an_up + d * 0x1p-50;
+            const double base = static_cast<double>(s.theta[p]) + an2 - 2.0 * kq;
+            double len = base > 0.0 ? __builtin_sqrt(base) / sigma : 0.0;
+            len = len * (1.0 - 2.02 * u) - 2.02 * u * (an_up / sigma);
+            lower = len > 0.0 ? len * len * (1.0 - cd) : 0.0;
+        }
         const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);
         const double need = (t + tau) > snap_max ? (t + tau) : snap_max;
         certified = lower > need;
@@ -470,6 +561,59 @@ void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp
 constexpr int kExactBuf = 1024;
 constexpr int kExactKeep = 512;
 
+// One thread: the first k of `m` keys sorted by (d^2, id) into out_id / out_d, then the fp64
+// boundary rule (bd / bid / bf: kBandCap entries of scratch). `dropped`: keys past the m kept
+// ones were discarded, so a band reaching the last kept key may be incomplete (flagged).
+__device__ void band_order(const DatasetDesc& s, const uint64_t* keys, int m, int k, bool dropped,
+                           const float* xq, uint32_t* out_id, float* out_d, double* bd, uint32_t* bid,
+                           float* bf, unsigned* flags) {
+    const int d = static_cast<int>(s.d);
+    for (int j = 0; j < k; j++) {
+        out_id[j] = key_id(keys[j]);
+        out_d[j] = key_value(keys[j]);
+    }
+    const double t = static_cast<double>(key_value(keys[k - 1]));
+    const double tau = t * static_cast<double>(d + 2) * 0x1p-23;
+    // With t == 0 every band member was snapped to zero and keeps (0, id) order.
+    if (t > 0.0 && m > k && static_cast<double>(key_value(keys[k])) - t <= tau) {
+        int lo = k - 1;
+        while (lo > 0 && t - static_cast<double>(key_value(keys[lo - 1])) <= tau) lo--;
+        int hi = k;
+        while (hi + 1 < m && hi - lo + 1 <= kBandCap && static_cast<double>(key_value(keys[hi + 1])) - t <= tau)
+            hi++;
+        // A band wider than kBandCap keeps the (d^2, id) order. With keys dropped, m is at
+        // least k + kBandCap (kExactKeep, or kSplitKeep with k <= 47), so a band that reaches
+        // the end of the kept keys is always wider than the cap.
+        const int len = hi - lo + 1;
+        if (len <= kBandCap && hi + 1 == m && dropped) {
+            atomicOr(flags, kFlagBandOverflow);
+        } else if (len <= kBandCap) {
+            // Insertion sort of the band by (fp64 d^2, id); the band is small.
+            for (int i = 0; i < len; i++) {
+                const uint64_t key = keys[lo + i];
+                const float v = key_value(key);
+                const uint32_t id = key_id(key);
+                const uint32_t at = s.inv_perm ? s.inv_perm[id] : id;
+                const double v64 = v == 0.0f ? 0.0 : direct_square_f64(xq, s.coords + static_cast<size_t>(at) * d, d);
+                int slot = i;
+                while (slot > 0 && (bd[slot - 1] > v64 || (bd[slot - 1] == v64 && bid[slot - 1] > id))) {
+                    bd[slot] = bd[slot - 1];
+                    bid[slot] = bid[slot - 1];
+                    bf[slot] = bf[slot - 1];
+                    slot--;
+                }
+                bd[slot] = v64;
+                bid[slot] = id;
+                bf[slot] = v;
+            }
+            for (int p = lo; p < k; p++) {
+                out_id[p] = bid[p - lo];
+                out_d[p] = bf[p - lo];
+            }
+        }
+    }
+}
+
 __device__ void lds_sort(uint64_t* keys, int count) {
     // Bitonic sort of kExactBuf keys by the workgroup (entries past count are kNoKey).
     for (int i = threadIdx.x; i < kExactBuf; i += blockDim.x)
@@ -496,7 +640,7 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
                                                     uint32_t count, uint32_t total_rows,
                                                     int metric, const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ list_count,
-                                                    unsigned* __restrict__ flags) {
+                                                    unsigned* __restrict__ flags, bool split_on) {
     extern __shared__ float xq[];
     __shared__ uint64_t keys[kExactBuf];
     __shared__ uint32_t kept;
@@ -509,9 +653,11 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
     __shared__ float bf[kBandCap];
     const int tid = threadIdx.x;
     const uint32_t total = list ? *list_count : total_rows;
+    const bool split = split_on && total <= kSplitRowsMax;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t flat = list ? list[item] : item;
         const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+        if (split && s.on_pass) continue;  // exact_split_kernel's
         const uint32_t q = flat - s.row_begin;
         const int d = static_cast<int>(s.d), k = static_cast<int>(s.k);
         const uint32_t n = s.n;
@@ -549,54 +695,7 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
         }
         const int m = static_cast<int>(kept);
         lds_sort(keys, m);
-        if (tid == 0) {
-            // (d^2, id) order, then the fp64 boundary rule on the sorted keys.
-            for (int j = 0; j < k; j++) {
-                out_id[j] = key_id(keys[j]);
-                out_d[j] = key_value(keys[j]);
-            }
-            const double t = static_cast<double>(key_value(keys[k - 1]));
-            const double tau = t * static_cast<double>(d + 2) * 0x1p-23;
-            // With t == 0 every band member was snapped to zero and keeps (0, id) order.
-            if (t > 0.0 && m > k && static_cast<double>(key_value(keys[k])) - t <= tau) {
-                int lo = k - 1;
-                while (lo > 0 && t - static_cast<double>(key_value(keys[lo - 1])) <= tau) lo--;
-                int hi = k;
-                while (hi + 1 < m && hi - lo + 1 <= kBandCap &&
-                       static_cast<double>(key_value(keys[hi + 1])) - t <= tau)
-                    hi++;
-                // A band wider than kBandCap keeps the (d^2, id) order. After a compaction
-                // m >= kExactKeep and lo < k <= kMaxK, so a band that reaches the end of
-                // the kept keys is always wider than the cap.
-                const int len = hi - lo + 1;
-                if (len <= kBandCap && hi + 1 == m && dropped) {
-                    atomicOr(flags, kFlagBandOverflow);
-                } else if (len <= kBandCap) {
-                    // Insertion sort of the band by (fp64 d^2, id); the band is small.
-                    for (int i = 0; i < len; i++) {
-                        const uint64_t key = keys[lo + i];
-                        const float v = key_value(key);
-                        const uint32_t id = key_id(key);
-                        const uint32_t at = s.inv_perm ? s.inv_perm[id] : id;
-                        const double v64 = v == 0.0f ? 0.0 : direct_square_f64(xq, s.coords + static_cast<size_t>(at) * d, d);
-                        int slot = i;
-                        while (slot > 0 && (bd[slot - 1] > v64 || (bd[slot - 1] == v64 && bid[slot - 1] > id))) {
-                            bd[slot] = bd[slot - 1];
-                            bid[slot] = bid[slot - 1];
-                            bf[slot] = bf[slot - 1];
-                            slot--;
-                        }
-                        bd[slot] = v64;
-                        bid[slot] = id;
-                        bf[slot] = v;
-                    }
-                    for (int p = lo; p < k; p++) {
-                        out_id[p] = bid[p - lo];
-                        out_d[p] = bf[p - lo];
-                    }
-                }
-            }
-        }
+        if (tid == 0) band_order(s, keys, m, k, dropped != 0, xq, out_id, out_d, bd, bid, bf, flags);
         __syncthreads();
         for (int j = tid; j < k; j += blockDim.x) {
             s.out_indices[static_cast<size_t>(q) * k + j] = out_id[j];
@@ -607,12 +706,157 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
     }
 }
 
+// Split scan of a short exact list (pass datasets' rows): wave item = (list entry i, range j)
+// over kSplitRange references, keeping its kSplitKeep smallest keys in `partial`. A wave
+// holds its kept keys two per lane and takes new keys fourteen per lane at a time; when more
+// than kSplitKeep are held it keeps the kSplitKeep smallest (select_keys, through LDS) and
+// drops later keys scoring above the kept maximum. Keys are (f32 direct-difference d^2,
+// input id), as in exact_kernel.
+constexpr int kSplitE = 16;
+
+__device__ void split_keep(uint64_t (&key)[kSplitE], float& cut, int lane, uint64_t* stage) {
+    uint32_t total = 0;
+#pragma unroll
+    for (int e = 0; e < kSplitE; e++) total += static_cast<uint32_t>(__popcll(__ballot(key[e] != kNoKey)));
+    if (total > kSplitKeep) {
+        cut = select_keys<kSplitE>(key, static_cast<int>(kSplitKeep), lane, stage, 0u, 0u);
+    } else {
+        uint32_t base = 0;
+#pragma unroll
+        for (int e = 0; e < kSplitE; e++) {
+            const uint64_t m = __ballot(key[e] != kNoKey);
+            if (key[e] != kNoKey) stage[base + lanes_below(m)] = key[e];
+            base += static_cast<uint32_t>(__popcll(m));
+        }
+        for (uint32_t p = base + static_cast<uint32_t>(lane); p < kSplitKeep; p += kWave) stage[p] = kNoKey;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    key[0] = stage[lane];
+    key[1] = stage[kWave + lane];
+#pragma unroll
+    for (int e = 2; e < kSplitE; e++) key[e] = kNoKey;
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void exact_split_kernel(const DatasetDesc* __restrict__ sets, uint32_t count,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ list_count,
+                                                          uint32_t ranges_max, uint64_t* __restrict__ partial) {
+    __shared__ uint64_t stage_all[4][kSplitKeep];
+    const uint32_t total = *list_count;
+    if (total > kSplitRowsMax) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t* stage = stage_all[w];
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t item = blockIdx.x * 4 + w; item < total * ranges_max; item += waves) {
+        const uint32_t i = item / ranges_max, j = item - i * ranges_max;
+        const uint32_t flat = list[i];
+        const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+        if (!s.on_pass) continue;
+        const uint32_t n = s.n, q = flat - s.row_begin;
+        const uint32_t pq = s.inv_perm ? s.inv_perm[q] : q;
+        const int d = static_cast<int>(s.d);
+        const float* xq = s.coords + static_cast<size_t>(pq) * d;
+        const float nq = s.norms[pq];
+        uint64_t key[kSplitE];
+#pragma unroll
+        for (int e = 0; e < kSplitE; e++) key[e] = kNoKey;
+        float cut = kInf;
+        const uint32_t r0 = j * kSplitRange;
+        const uint32_t r1 = r0 < n ? (n - r0 < kSplitRange ? n : r0 + kSplitRange) : r0;
+        for (uint32_t b = r0; b < r1; b += (kSplitE - 2) * kWave) {
+#pragma unroll
+            for (int e = 2; e < kSplitE; e++) {
+                const uint32_t r = b + static_cast<uint32_t>(e - 2) * kWave + static_cast<uint32_t>(lane);
+                uint64_t kk = kNoKey;
+                if (r < r1 && r != pq) {
+                    const float v = direct_square(xq, s.coords + static_cast<size_t>(r) * d, d, nq, s.norms[r]);
+                    if (!(v > cut)) kk = make_key(v, s.perm ? s.perm[r] : r);
+                }
+                key[e] = kk;
+            }
+            split_keep(key, cut, lane, stage);
+        }
+        uint64_t* out = partial + static_cast<size_t>(item) * kSplitKeep;
+        out[lane] = key[0];
+        out[kWave + lane] = key[1];
+    }
+}
+
+// One wave per listed row: the kSplitKeep smallest keys over the row's ranges, sorted, then
+// the first k and the fp64 boundary rule (band_order) into the outputs.
+__global__ __launch_bounds__(256) void exact_merge_kernel(const DatasetDesc* __restrict__ sets, uint32_t count,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ list_count,
+                                                          uint32_t ranges_max, const uint64_t* __restrict__ partial,
+                                                          int metric, unsigned* __restrict__ flags) {
+    __shared__ uint64_t stage_all[4][kSplitKeep];
+    __shared__ double bd_all[4][kBandCap];
+    __shared__ uint32_t bid_all[4][kBandCap];
+    __shared__ float bf_all[4][kBandCap];
+    __shared__ uint32_t id_all[4][kMaxPassK];
+    __shared__ float dist_all[4][kMaxPassK];
+    const uint32_t total = *list_count;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 4 + w;
+    if (total > kSplitRowsMax || i >= total) return;
+    const uint32_t flat = list[i];
+    const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+    if (!s.on_pass) return;
+    uint64_t* stage = stage_all[w];
+    const uint32_t n = s.n, q = flat - s.row_begin, k = s.k;
+    const uint32_t pq = s.inv_perm ? s.inv_perm[q] : q;
+    const uint32_t ranges = (n + kSplitRange - 1) / kSplitRange;
+    const uint32_t keys_in = ranges * kSplitKeep;
+    const uint64_t* src = partial + static_cast<size_t>(i) * ranges_max * kSplitKeep;
+    uint64_t key[kSplitE];
+#pragma unroll
+    for (int e = 0; e < kSplitE; e++) key[e] = kNoKey;
+    float cut = kInf;
+    for (uint32_t b = 0; b < keys_in; b += (kSplitE - 2) * kWave) {
+#pragma unroll
+        for (int e = 2; e < kSplitE; e++) {
+            const uint32_t p = b + static_cast<uint32_t>(e - 2) * kWave + static_cast<uint32_t>(lane);
+            const uint64_t kk = p < keys_in ? src[p] : kNoKey;
+            key[e] = (kk != kNoKey && !(key_value(kk) > cut)) ? kk : kNoKey;
+        }
+        split_keep(key, cut, lane, stage);
+    }
+    // Sort the kept keys (two per lane) and finish on lane 0.
+    uint64_t two[2] = {key[0], key[1]};
+    wave_sort<2>(two, lane);
+    stage[lane] = two[0];
+    stage[kWave + lane] = two[1];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        int m = 0;
+        while (m < static_cast<int>(kSplitKeep) && stage[m] != kNoKey) m++;
+        const float* xq = s.coords + static_cast<size_t>(pq) * s.d;
+        band_order(s, stage, m, static_cast<int>(k), n - 1 > kSplitKeep, xq, id_all[w], dist_all[w], bd_all[w],
+                   bid_all[w], bf_all[w], flags);
+        for (uint32_t j = 0; j < k; j++) {
+            s.out_indices[static_cast<size_t>(q) * k + j] = id_all[w][j];
+            s.out_distances[static_cast<size_t>(q) * k + j] =
+                metric == kMetricCosine ? 0.5f * dist_all[w][j] : __builtin_sqrtf(dist_all[w][j]);
+        }
+    }
+}
+
 void launch_exact(const DatasetDesc* sets, const BatchShape& shape, const uint32_t* list,
                   const uint32_t* list_count, unsigned blocks, unsigned* flags,
-                  hipStream_t stream) {
+                  uint64_t* partial, uint32_t ranges_max, hipStream_t stream) {
     if (blocks == 0) return;
+    const bool split = partial != nullptr && ranges_max > 0;
+    if (split) {
+        hipLaunchKernelGGL(exact_split_kernel, dim3(1024), dim3(256), 0, stream, sets, shape.count, list, list_count,
+                           ranges_max, partial);
+        hipLaunchKernelGGL(exact_merge_kernel, dim3(kSplitRowsMax / 4), dim3(256), 0, stream, sets, shape.count, list,
+                           list_count, ranges_max, static_cast<const uint64_t*>(partial), shape.metric, flags);
+    }
     hipLaunchKernelGGL(exact_kernel, dim3(blocks), dim3(256), shape.max_d * sizeof(float), stream,
-                       sets, shape.count, shape.total_rows, shape.metric, list, list_count, flags);
+                       sets, shape.count, shape.total_rows, shape.metric, list, list_count, flags, split);
 }
 
 }  // namespace scrust_hip

@@ -15,6 +15,22 @@ constexpr int kMetricCosine = 1;
 constexpr int kOperandNone = 0;  // exact path: coordinates only
 constexpr int kOperandBf16 = 1;
 constexpr int kOperandF32 = 2;
+// fp16 single products of local-origin coordinates (one origin per 512 query rows),
+// converted from the padded fp32 coordinates tile by tile inside the pass.
+constexpr int kOperandF16Local = 3;
+constexpr int kLocalRows = 512;  // query rows per local-origin work item (8 waves x 64)
+// Error model of the local-origin pass (see the certificate in rerank_kernel): fp16 rounding
+// (unit 2^-11) of both operands, the MFMA's fp32 accumulation over 32 nch + 1 terms.
+__host__ __device__ inline double local_product_coeff(int nch) {
+    const double u = 0x1p-11, g = (32.0 * nch + 2.0) * 0x1p-24 * 1.01;
+    return (2.0 * u + u * u + g * (1.0 + u) * (1.0 + u)) / (1.0 - g) * (1.0 + 0x1p-10);
+}
+// Absolute fp16 rounding error below the normal range (2^-25 per coordinate), over d terms.
+__host__ __device__ inline double local_abs_coeff(int d) {
+    double r = 1.0;
+    while (r * r < d) r += 1.0;  // an integer >= sqrt(d)
+    return 0x1p-25 * r * 1.01;
+}
 
 constexpr unsigned kFlagNonFinite = 1u;
 constexpr unsigned kFlagBandOverflow = 2u;
@@ -35,6 +51,12 @@ constexpr int kAccWordsHost = 10;
 constexpr float kPassNormLimit = 0x1p100f;
 constexpr int kPassRows = 256;  // query rows per pass work item (4 waves x 64)
 constexpr int kTileRefsHost = 64;  // references per pass tile
+// Split scan of a short list of exact rows: at most kSplitRowsMax rows, each cut into ranges
+// of kSplitRange references, each range keeping its kSplitKeep smallest keys (k + kBandCap
+// <= 47 + 64 fit, so the merged keys hold the first k and any tie band the cap allows).
+constexpr uint32_t kSplitRowsMax = 256;
+constexpr uint32_t kSplitRange = 8192;
+constexpr uint32_t kSplitKeep = 128;
 
 // Device-visible description of one embedding and its workspace.
 struct DatasetDesc {
@@ -74,6 +96,15 @@ struct DatasetDesc {
     uint32_t tail_row0;
     uint32_t select_e1;
     // Error model of the pass (see the certificate in rerank_kernel).
+    // Local-origin fp16 pass (operand kOperandF16Local): padded fp32 coordinates (ref_rows x
+    // 32 nch, pass order, zero past d), one origin per kLocalRows query rows (32 nch floats
+    // each), and per query row a lower bound of |a''|^2 (a'' = (a - o) sigma, local_scale).
+    float* pack_src;
+    float* origins;
+    float* anorm;
+    uint32_t local;    // 1: this dataset's pass is the local-origin fp16 pass
+    float local_c;     // local_product_coeff(nch), rounded up to f32
+    float local_c2;    // local_abs_coeff(d), rounded up to f32
     double split_rel;  // operand split error of a.b, relative to |a||b|
     double acc_rel;    // MFMA accumulation error, relative to the sum of |terms|
     double norm_rel;   // error of the |b|^2 feature, relative to |b|^2
@@ -110,6 +141,7 @@ struct BatchShape {
     uint32_t max_col_groups;  // max ceil(d / 64)
     uint32_t max_d;           // largest d
     uint32_t max_prep_blocks; // max ceil(ref_rows / 64) over datasets on the bf16 pass
+    uint32_t max_local_blocks; // max npad / kLocalRows over datasets on the local-origin pass
     int metric;
     int operand;
 };
@@ -153,15 +185,20 @@ void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned wi
 void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp32,
                    bool has_kp64, uint32_t* fail_list, uint32_t* fail_count,
                    hipStream_t stream);
-// Exact rows listed in `list` (flat rows, count on the device).
+// Exact rows listed in `list` (flat rows, count on the device). With `partial` (the split
+// scan's workspace, kSplitRowsMax x ranges_max x kSplitKeep keys), a list of at most
+// kSplitRowsMax rows has its pass datasets' rows scanned by many waves per row instead.
 void launch_exact(const DatasetDesc* sets, const BatchShape& shape, const uint32_t* list,
                   const uint32_t* list_count, unsigned blocks, unsigned* flags,
-                  hipStream_t stream);
+                  uint64_t* partial, uint32_t ranges_max, hipStream_t stream);
 
 // MFMA chunk counts of the compiled pass variants for a dimension d (0: none).
 int bf16_chunks(int d);
 int fp32_chunks(int d);
 int fp32_group(int nch);
+// Chunks of the local-origin fp16 pass for d (d + 3 features, a compiled count 1, 2 or 4;
+// 0: none, d > 125).
+int local_chunks(int d);
 // Chunk count of the persistent batched kernel able to serve nch (0: none).
 int batch_chunks(int operand, int nch);
 

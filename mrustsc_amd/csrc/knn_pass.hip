@@ -51,11 +51,6 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
-
 // Row buffers: a row's `cap` slots are four segments of cap/4, segment g appended to by
 // lane group g (lanes 16g..16g+15) with a per-lane counter; c[g] keys of segment g are
 // valid.
@@ -73,61 +68,6 @@ __device__ __forceinline__ void gather_row(const SCRUST_GLOBAL uint64_t* rowbuf,
         const uint32_t cs = sg == 0 ? c[0] : sg == 1 ? c[1] : sg == 2 ? c[2] : c[3];
         key[i] = static_cast<uint32_t>(o) < cs ? rowbuf[e] : kNoKey;
     }
-}
-
-// Place of the j-th kept key: out[j], or with quarter > 0 back in the four segments
-// (quarter keys at the start of each segment of `seg` slots).
-__device__ __forceinline__ uint32_t kept_slot(uint32_t j, uint32_t quarter, uint32_t seg) {
-    return quarter ? (j / quarter) * seg + j % quarter : j;
-}
-
-// Keeps the `keep` smallest keys (at least `keep` are valid), written unordered through
-// kept_slot; returns the keep-th smallest score. Ballot radix select: a binary search over
-// the ordered score bits, then over ids among keys that tie on the boundary score.
-template <int E>
-__device__ float select_keys(const uint64_t (&key)[E], int keep, int lane,
-                             SCRUST_GLOBAL uint64_t* out, uint32_t quarter, uint32_t seg) {
-    uint32_t lo = 0, hi = 0xFFFFFFFFu;
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        int c = 0;
-#pragma unroll
-        for (int i = 0; i < E; i++) c += __popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) <= mid));
-        if (c >= keep) hi = mid; else lo = mid + 1;
-    }
-    const uint32_t v = lo;
-    int below = 0, equal = 0;
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-        const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
-        below += __popcll(__ballot(sc < v));
-        equal += __popcll(__ballot(sc == v));
-    }
-    const int need = keep - below;
-    uint32_t idcut = 0xFFFFFFFFu;
-    if (equal > need) {
-        uint32_t l2 = 0, h2 = 0xFFFFFFFFu;
-        while (l2 < h2) {
-            const uint32_t mid = l2 + (h2 - l2) / 2;
-            int c = 0;
-#pragma unroll
-            for (int i = 0; i < E; i++)
-                c += __popcll(__ballot(static_cast<uint32_t>(key[i] >> 32) == v &&
-                                       static_cast<uint32_t>(key[i]) <= mid));
-            if (c >= need) h2 = mid; else l2 = mid + 1;
-        }
-        idcut = l2;
-    }
-    uint32_t base = 0;
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-        const uint32_t sc = static_cast<uint32_t>(key[i] >> 32);
-        const bool kept = sc < v || (sc == v && static_cast<uint32_t>(key[i]) <= idcut);
-        const uint64_t m = __ballot(kept);
-        if (kept) out[kept_slot(base + lanes_below(m), quarter, seg)] = key[i];
-        base += static_cast<uint32_t>(__popcll(m));
-    }
-    return unordered(v);
 }
 
 // Wave-wide min / max on the VALU (DPP): row shifts by 1, 2, 4 and 8 reduce each row of
@@ -245,6 +185,19 @@ struct Operand<kOperandBf16, NCH> {
     static constexpr int kLoadsMax = NCH;  // 64 rows x 4 nch chunks / 256 threads
 };
 
+// Local-origin fp16 operands: the same LDS layout as bf16 (4 nch 16-byte slots per row,
+// lane group g's chunk c at g * gstride + c, holding features 8 (g nch + c) .. + 7), filled
+// by converting the padded fp32 coordinates inside the pass.
+template <int NCH>
+struct Operand<kOperandF16Local, NCH> {
+    typedef f16x8 Frag;
+    static __device__ __forceinline__ int row_u4(int nch) { return 4 * nch; }
+    static __device__ __forceinline__ int group_u4(int nch) { return nch; }
+    static constexpr int kRowU4Max = 4 * NCH;
+    static constexpr int kGroupU4Max = NCH;
+    static constexpr int kLoadsMax = 1;
+};
+
 template <int NCH>
 struct Operand<kOperandF32, NCH> {
     typedef float Frag;
@@ -254,6 +207,14 @@ struct Operand<kOperandF32, NCH> {
     static constexpr int kGroupU4Max = (NCH + 3) / 4;
     static constexpr int kLoadsMax = kRowU4Max / 4;
 };
+
+// One 16x16x32 product of the 16-bit operands into acc.
+__device__ __forceinline__ f32x4 mfma_k32(bf16x8 b, bf16x8 a, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, acc, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma_k32(f16x8 b, f16x8 a, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(b, a, acc, 0, 0, 0);
+}
 
 template <int OP, int NCH>
 struct PassSmem {
@@ -313,11 +274,12 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     for (int qb = 0; qb < 4; qb++) {
         uint32_t row = qbase + w * kWave + qb * 16 + lc;
         if (row >= n) row = n - 1;
-        if constexpr (OP == kOperandBf16) {
+        if constexpr (OP == kOperandBf16 || OP == kOperandF16Local) {
+            typedef typename Op::Frag Frag;
             const SCRUST_GLOBAL u32x4* src = global_ptr(reinterpret_cast<const u32x4*>(s.pack_query)) + static_cast<size_t>(row) * row_u4 + g * nch;
 #pragma unroll
             for (int c = 0; c < NCH; c++)
-                a[qb][c] = (FIXED || c < nch) ? __builtin_bit_cast(bf16x8, src[c]) : bf16x8{};
+                a[qb][c] = (FIXED || c < nch) ? __builtin_bit_cast(Frag, src[c]) : Frag{};
         } else {
             const SCRUST_GLOBAL float* src = global_ptr(s.pack_f32) + static_cast<size_t>(row) * 4 * group + g * group;
 #pragma unroll
@@ -388,14 +350,13 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) acc[qb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         const u32x4* brow = &tile_of(cur)[(cb * 16 + lc) * stride];
-        if constexpr (OP == kOperandBf16) {
+        if constexpr (OP == kOperandBf16 || OP == kOperandF16Local) {
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 if (FIXED || c < nch) {
-                    const bf16x8 b = __builtin_bit_cast(bf16x8, brow[g * gstride + c]);
+                    const typename Op::Frag b = __builtin_bit_cast(typename Op::Frag, brow[g * gstride + c]);
 #pragma unroll
-                    for (int qb = 0; qb < 4; qb++)
-                        acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[qb][c], acc[qb], 0, 0, 0);
+                    for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(b, a[qb][c], acc[qb]);
                 }
             }
         } else {
@@ -466,7 +427,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         const u32x4* brow = &tile_of(cur)[(cb * 16 + lc) * stride];
 #pragma unroll
         for (int c = 0; c < NCH; c++)
-            if (FIXED || c < nch) b[c] = __builtin_bit_cast(bf16x8, brow[g * gstride + c]);
+            if (FIXED || c < nch) b[c] = __builtin_bit_cast(std::remove_reference_t<decltype(b[0])>, brow[g * gstride + c]);
     };
     auto mfma_frags = [&](f32x4 (&acc)[4], const auto& b) __attribute__((always_inline)) {
 #pragma unroll
@@ -475,18 +436,17 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         for (int c = 0; c < NCH; c++) {
             if (FIXED || c < nch) {
 #pragma unroll
-                for (int qb = 0; qb < 4; qb++)
-                    acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[c], a[qb][c], acc[qb], 0, 0, 0);
+                for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(b[c], a[qb][c], acc[qb]);
             }
         }
     };
     auto consume = [&](uint32_t t, int cur) __attribute__((always_inline)) {
         f32x4 acc0[4], acc1[4];
-        if constexpr (OP == kOperandBf16 && FIXED) {
+        if constexpr ((OP == kOperandBf16 || OP == kOperandF16Local) && FIXED) {
             // Software pipeline: block cb+1's fragments are read while block cb's MFMAs
             // run, and block cb-1 is filtered while they run.
             // Scheduling barriers keep the compiler from sinking each read to its use.
-            bf16x8 b0[NCH], b1[NCH];
+            typename Op::Frag b0[NCH], b1[NCH];
             load_frags(b0, cur, 0);
             load_frags(b1, cur, 1);
             __builtin_amdgcn_sched_barrier(0);
@@ -575,7 +535,98 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         else compact_rows(std::integral_constant<int, 8>{}, full);
     };
 
-    if constexpr (!DMA) {
+    if constexpr (OP == kOperandF16Local) {
+        // 512 threads convert a 64-row tile: slot u = tid + 512 i is 16 bytes of LDS, row
+        // u / (4 nch), part pp = u % (4 nch): features 8 pp .. 8 pp + 7 of the row, from 32
+        // bytes of padded fp32 coordinates. 4 nch divides 512 (nch 1, 2 or 4), so a thread's
+        // parts, and with them its origin values, are the same on every tile.
+        static_assert(512 % (4 * NCH) == 0, "local-origin pass: nch must divide 128");
+        constexpr int kRowSlots = 4 * NCH;
+        constexpr int kSlots = kTileRefs * kRowSlots;
+        constexpr int kPer = (kSlots + 511) / 512;
+        const float sigma = local_scale(__uint_as_float(*global_ptr(s.max_norm_bits)));
+        const SCRUST_GLOBAL float4* src4 = reinterpret_cast<const SCRUST_GLOBAL float4*>(global_ptr(s.pack_src));
+        const SCRUST_GLOBAL float* origin = global_ptr(s.origins) + static_cast<size_t>(item.block / 2) * (32 * NCH);
+        float os[kPer][8];  // -o sigma for this thread's parts
+        int at[kPer];       // LDS slot of part i in a tile buffer
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int u = tid + 512 * i;
+            const int pp = u % kRowSlots;
+#pragma unroll
+            for (int e = 0; e < 8; e++) os[i][e] = -(origin[8 * pp + e] * sigma);
+            at[i] = (u / kRowSlots) * stride + (pp / NCH) * gstride + pp % NCH;
+        }
+        const float limit_lo = 1.0f - 0x1p-15f, limit_up = 1.0f + 0x1p-15f;
+        u32x4 pa[2 * kPer], pb[2 * kPer];
+        auto load = [&](u32x4 (&P)[2 * kPer], uint32_t T) __attribute__((always_inline)) {
+#pragma unroll
+            for (int i = 0; i < kPer; i++) {
+                const int u = tid + 512 * i;
+                if (u < kSlots) {
+                    const size_t f4 = (static_cast<size_t>(phys(T)) * kTileRefs) * (8 * NCH) + 2 * static_cast<size_t>(u);
+                    P[2 * i] = __builtin_bit_cast(u32x4, src4[f4]);
+                    P[2 * i + 1] = __builtin_bit_cast(u32x4, src4[f4 + 1]);
+                }
+            }
+        };
+        // Registers of tile T -> fp16 operand rows in LDS buffer B. The row's last three
+        // features: |b''|^2 2^-16 as fp16 hi and lo (scaled down by 2^-15 first, so their sum
+        // is below |b''|^2 2^-16 despite the norm's own rounding and the MFMA's accumulation)
+        // and |b''| rounded up; a sentinel row (past n) gets +inf, 0, 0 and scores -inf.
+        auto store = [&](const u32x4 (&P)[2 * kPer], int B, uint32_t T) __attribute__((always_inline)) {
+#pragma unroll
+            for (int i = 0; i < kPer; i++) {
+                const int u = tid + 512 * i;
+                if (kSlots % 512 == 0 || u < kSlots) {
+                    float x[8];
+                    const float4 v0 = __builtin_bit_cast(float4, P[2 * i]), v1 = __builtin_bit_cast(float4, P[2 * i + 1]);
+                    const float raw[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                    float part = 0.0f;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        x[e] = __builtin_fmaf(raw[e], sigma, os[i][e]);
+                        part = __builtin_fmaf(x[e], x[e], part);
+                    }
+                    // Sum over the row's kRowSlots consecutive lanes.
+#pragma unroll
+                    for (int m = 1; m < kRowSlots; m <<= 1) part += __shfl_xor(part, m, kWave);
+                    f16x8 h;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) h[e] = static_cast<_Float16>(x[e]);
+                    if (u % kRowSlots == kRowSlots - 1) {
+                        const uint32_t ref = phys(T) * kTileRefs + static_cast<uint32_t>(u / kRowSlots);
+                        const float nb = part * limit_lo * 0x1p-16f;
+                        const _Float16 hi = static_cast<_Float16>(nb);
+                        const _Float16 lo = static_cast<_Float16>(nb - static_cast<float>(hi));
+                        const _Float16 up = f16_up(__builtin_sqrtf(part * limit_up) * (1.0f + 0x1p-20f));
+                        const bool live = ref < n;
+                        h[5] = live ? hi : static_cast<_Float16>(kInf);
+                        h[6] = live ? lo : static_cast<_Float16>(0.0f);
+                        h[7] = live ? up : static_cast<_Float16>(0.0f);
+                    }
+                    tile_of(B)[at[i]] = __builtin_bit_cast(u32x4, h);
+                }
+            }
+        };
+        const uint32_t last_load = t_end - 1;
+        load(pa, t_begin);
+        load(pb, t_begin + 1);
+        store(pa, 0, t_begin);
+        __syncthreads();
+        for (uint32_t t = t_begin; t < t_end; t += 2) {
+            load(pa, t + 2 < last_load ? t + 2 : last_load);
+            consume(phys(t), 0);
+            store(pb, 1, t + 1);
+            __syncthreads();
+            compact();
+            load(pb, t + 3 < last_load ? t + 3 : last_load);
+            consume(phys(t + 1), 1);
+            store(pa, 0, t + 2 < last_load ? t + 2 : last_load);
+            __syncthreads();
+            compact();
+        }
+    } else if constexpr (!DMA) {
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
     // merge into the loop head, where the compiler must then assume the prefetch just
     // issued is outstanding and waits for it. Loads past the end reload the last tile.
@@ -703,6 +754,25 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
         __shared__ PassSmem<OP, NCH> sm;
         pass_block<OP, NCH, true>(s, item, &sm);
     }
+}
+
+// Local-origin fp16 pass of one dataset: workgroup i scans every reference for the 512
+// query rows of blocks 2i and 2i + 1 (npad is a multiple of 512), whose origin is
+// origins[i]; in spatial order the scan starts at the tiles of the rows' own group.
+template <int NCH>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void pass_local_kernel(const DatasetDesc* __restrict__ set) {
+    const DatasetDesc& s = *set;
+    const uint32_t pairs = (s.n + 2 * kTileRefs - 1) / (2 * kTileRefs);
+    PassItem item;
+    item.set = 0;
+    item.split = 0;
+    item.block = 2 * blockIdx.x;
+    item.tile_begin = 0;
+    item.tile_end = 2 * pairs;
+    item.pad = s.rotate ? (4u * (item.block & ~15u)) % (2 * pairs) : 0u;
+    __shared__ PassSmem<kOperandF16Local, NCH> sm;
+    pass_block<kOperandF16Local, NCH, true>(s, item, &sm);
 }
 
 // Persistent grid over a queue of (dataset, query block, reference range) items, largest
@@ -857,8 +927,22 @@ static void run_pass(const DatasetDesc* set, const DatasetDesc& host, hipStream_
         run_pass<OP, N>(set, host, stream); \
         return true;
 
+int local_chunks(int d) {
+    const int need = (d + 3 + 31) / 32;
+    return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 0;
+}
+
 bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,
                  hipStream_t stream) {
+    if (operand == kOperandF16Local) {
+        const dim3 grid(host.npad / kLocalRows), block(kLocalRows);
+        switch (host.nch) {
+            case 1: hipLaunchKernelGGL(pass_local_kernel<1>, grid, block, 0, stream, set); return true;
+            case 2: hipLaunchKernelGGL(pass_local_kernel<2>, grid, block, 0, stream, set); return true;
+            case 4: hipLaunchKernelGGL(pass_local_kernel<4>, grid, block, 0, stream, set); return true;
+            default: return false;
+        }
+    }
     if (operand == kOperandBf16) {
         switch (host.nch) {
             SCRUST_PASS_CASE(kOperandBf16, 1) SCRUST_PASS_CASE(kOperandBf16, 2)

@@ -90,6 +90,18 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
     plan.operand = precision == SCRUST_HIP_BF16 ? kOperandBf16
                    : precision == SCRUST_HIP_FP32 ? kOperandF32
                                                   : kOperandNone;
+    if (precision == SCRUST_HIP_FP16) {
+        // The local-origin fp16 pass needs the rows grouped (each 512-row work item's origin
+        // is the mean of nearby rows): single datasets in spatial order, automatic from
+        // kOrderMinCells cells or asked for. Anything else takes the bf16 split pass, which
+        // gives the same graph.
+        const uint32_t order_knob = force && force->cap ? force->order : (options ? options->order : 0u);
+        const bool local = single && specs[0].k <= static_cast<size_t>(kMaxPassK) &&
+                           local_chunks(static_cast<int>(specs[0].d)) > 0 &&
+                           (order_knob == 2u || (order_knob == 0u && specs[0].n >= kOrderMinCells));
+        plan.operand = local ? kOperandF16Local : kOperandBf16;
+        plan.precision = local ? SCRUST_HIP_FP16 : SCRUST_HIP_BF16;
+    }
     plan.shape.count = static_cast<uint32_t>(specs.size());
     plan.shape.metric = metric;
     plan.shape.operand = plan.operand;
@@ -128,11 +140,19 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                           sp.d <= static_cast<size_t>(kMaxPassD);
         s.on_pass = pass ? 1u : 0u;
         if (pass) {
-            s.kp = sp.k + 17 <= 32 ? 32u : 64u;
+            // The local-origin pass's bound is looser for rows far from their work item's
+            // origin: it keeps 64 candidates so those rows still certify.
+            s.kp = sp.k + 17 <= 32 && plan.operand != kOperandF16Local ? 32u : 64u;
             // A row's buffer is four lane segments of cap/4 slots; a compaction leaves kp/4
             // keys in each and a tile appends at most 16 per segment.
             if (s.cap < 4 * s.kp) s.cap = 4 * s.kp;
-            if (plan.operand == kOperandBf16) {
+            if (plan.operand == kOperandF16Local) {
+                s.nch = static_cast<uint32_t>(local_chunks(static_cast<int>(sp.d)));
+                s.group = 0;
+                s.local = 1;
+                s.local_c = static_cast<float>(local_product_coeff(static_cast<int>(s.nch)) * (1.0 + 0x1p-20));
+                s.local_c2 = static_cast<float>(local_abs_coeff(static_cast<int>(sp.d)) * (1.0 + 0x1p-20));
+            } else if (plan.operand == kOperandBf16) {
                 // Single calls use the compiled chunk counts; the batched kernel takes any.
                 s.nch = single ? bf16_chunks(static_cast<int>(sp.d))
                                : static_cast<uint32_t>((3 * sp.d + 3 + 31) / 32);
@@ -188,7 +208,9 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         // rows being sentinels (norm feature +inf).
         const size_t ref_rows = (sp.n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
         if (pass) {
-            const size_t npad = (sp.n + kPassRows - 1) / kPassRows * kPassRows;
+            // The local-origin pass takes kLocalRows query rows per work item.
+            const size_t unit = s.local ? kLocalRows : kPassRows;
+            const size_t npad = (sp.n + unit - 1) / unit * unit;
             const uint32_t blocks = static_cast<uint32_t>(npad / kPassRows);
             const uint32_t pairs = static_cast<uint32_t>(ref_rows / (2 * kTileRefsHost));
             const uint32_t set = static_cast<uint32_t>(plan.sets.size());
@@ -200,7 +222,10 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // MFMAs of 16 (bf16) or 32 (fp32) cycles.
             const double pair_cycles = 32.0 * s.nch * (plan.operand == kOperandBf16 ? 16.0 : 32.0);
             const double overhead = 65536.0 / pair_cycles;
-            if (single) {
+            if (s.local) {
+                splits = 1;  // every work item scans all references, from its group's tiles
+                s.chunk_pairs = pairs;
+            } else if (single) {
                 splits = choose_splits(blocks, pairs, slots, max_splits, overhead);
                 if (have_tuned && tuned.splits >= 1 && tuned.splits <= max_splits) {
                     // A tuned count is taken when it cuts the scan into exactly that many
@@ -223,7 +248,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             uint32_t order_knob = options ? options->order : 0u;
             if (force && force->cap) order_knob = force->order;
             else if (have_tuned && order_knob == 0u && (tuned.order == 1u || tuned.order == 2u)) order_knob = tuned.order;
-            if (single && (order_knob == 2u || (order_knob == 0u && sp.n >= kOrderMinCells && splits == 1))) {
+            if (s.local || (single && (order_knob == 2u || (order_knob == 0u && sp.n >= kOrderMinCells && splits == 1)))) {
                 plan.ordered = true;
                 plan.order = order_layout(sp.n, sp.d);
                 s.perm = static_cast<const uint32_t*>(take(sp.n * sizeof(uint32_t)));
@@ -238,7 +263,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // spatial order), which spreads that round's work over every slot.
             const uint64_t rem = blocks % slots;
             const uint32_t tail_splits = std::min(kTailSplits, max_splits);
-            if (single && splits == 1 && blocks > slots && rem > slots / 2 && tail_splits > 1) {
+            if (single && !s.local && splits == 1 && blocks > slots && rem > slots / 2 && tail_splits > 1) {
                 const uint32_t len = (pairs + tail_splits - 1) / tail_splits;
                 if ((pairs + len - 1) / len == tail_splits) {
                     splits = tail_splits;
@@ -251,7 +276,12 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             }
             s.select_e = next_pow2(std::max<uint32_t>(2, (splits * s.cap + 63) / 64));
             plan.select_widths |= s.select_e;
-            if (plan.operand == kOperandBf16) {
+            if (s.local) {
+                s.pack_src = static_cast<float*>(take(ref_rows * 128ull * s.nch));
+                s.pack_query = static_cast<uint16_t*>(take(sp.n * 64ull * s.nch));
+                s.origins = static_cast<float*>(take(npad / kLocalRows * 128ull * s.nch));
+                s.anorm = static_cast<float*>(take(sp.n * sizeof(float)));
+            } else if (plan.operand == kOperandBf16) {
                 const size_t rowb = 64ull * s.nch;
                 s.pack_ref = static_cast<uint16_t*>(take(ref_rows * rowb));
                 s.pack_query = static_cast<uint16_t*>(take(sp.n * rowb));
@@ -268,14 +298,23 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         plan.shape.max_row_chunks = std::max<uint32_t>(plan.shape.max_row_chunks, static_cast<uint32_t>((sp.n + colsum_rows(s.n) - 1) / colsum_rows(s.n)));
         plan.shape.max_col_groups = std::max<uint32_t>(plan.shape.max_col_groups, static_cast<uint32_t>((sp.d + 63) / 64));
         plan.shape.max_d = std::max<uint32_t>(plan.shape.max_d, static_cast<uint32_t>(sp.d));
-        if (pass && plan.operand == kOperandBf16)
+        if (pass && (plan.operand == kOperandBf16 || plan.operand == kOperandF16Local))
             plan.shape.max_prep_blocks = std::max<uint32_t>(plan.shape.max_prep_blocks,
                                                             static_cast<uint32_t>((ref_rows + 63) / 64));
+        if (pass && s.local)
+            plan.shape.max_local_blocks = std::max<uint32_t>(plan.shape.max_local_blocks, s.npad / kLocalRows);
         row += s.n;
         prep += pass ? static_cast<uint32_t>(ref_rows) : s.n;
     }
     plan.shape.total_rows = row;
     plan.shape.total_prep = prep;
+    // Workspace of the split scan that recomputes a short list of uncertified rows.
+    for (const DatasetDesc& s : plan.sets)
+        if (s.on_pass) plan.split_ranges = std::max<uint32_t>(plan.split_ranges, (s.n + kSplitRange - 1) / kSplitRange);
+    if (plan.split_ranges) {
+        plan.split_offset = arena;
+        arena += align_up(static_cast<size_t>(kSplitRowsMax) * plan.split_ranges * kSplitKeep * sizeof(uint64_t));
+    }
     // The segment counts of all datasets: one region, cleared before every pass.
     plan.count_words = count_words;
     plan.counts_offset = arena;
@@ -330,7 +369,12 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
         fix(s.acc);
         fix(s.max_norm_bits);
         if (s.on_pass) {
-            if (plan.operand == kOperandBf16) {
+            if (s.local) {
+                fix(s.pack_src);
+                fix(s.pack_query);
+                fix(s.origins);
+                fix(s.anorm);
+            } else if (plan.operand == kOperandBf16) {
                 fix(s.pack_ref);
                 fix(s.pack_query);
             } else {
@@ -439,7 +483,9 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     }
     if ((err = mark(3)) != hipSuccess) return hip_failure(err, "record event");
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>(plan.shape.total_rows, 4096));
-    launch_exact(plan.d_sets, plan.shape, plan.d_list, plan.d_scalars + kScalarListCount, blocks, flags,
+    uint64_t* split = plan.split_ranges ? reinterpret_cast<uint64_t*>(static_cast<char*>(plan.arena_base) + plan.split_offset)
+                                        : nullptr;
+    launch_exact(plan.d_sets, plan.shape, plan.d_list, plan.d_scalars + kScalarListCount, blocks, flags, split, plan.split_ranges,
                  stream);
     if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch exact rows");
     if ((err = mark(4)) != hipSuccess) return hip_failure(err, "record event");

@@ -66,8 +66,8 @@ scrust_hip_status validate(const void* embedding, size_t n, size_t d, size_t k,
     if (options) {
         if (options->metric > SCRUST_HIP_COSINE)
             return invalid("metric", "euclidean or cosine", std::to_string(options->metric));
-        if (options->precision > SCRUST_HIP_EXACT)
-            return invalid("precision", "bf16, fp32 or exact", std::to_string(options->precision));
+        if (options->precision > SCRUST_HIP_FP16)
+            return invalid("precision", "bf16, fp32, exact or fp16", std::to_string(options->precision));
         if (options->tile != 0 && options->tile != 128 && options->tile != 256 && options->tile != 512)
             return invalid("tile", "0, 128, 256 or 512", std::to_string(options->tile));
         if (options->order > 2)

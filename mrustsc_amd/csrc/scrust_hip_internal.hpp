@@ -88,6 +88,10 @@ struct Plan {
     bool ordered = false;
     OrderLayout order{};
     size_t order_offset = 0;
+    // Split scan of a short list of uncertified rows (exact_split_kernel): ranges per row of
+    // the largest pass dataset, and its key workspace's arena offset (0 ranges: none).
+    uint32_t split_ranges = 0;
+    size_t split_offset = 0;
     uint64_t pass_flops = 0;
     size_t arena_bytes = 0;
     size_t meta_bytes = 0;
