@@ -40,6 +40,7 @@ METRIC = "kNN-graph cells/s at 1M x 50, k=15 (BF16+rerank vs FP32); p99 request 
 # WRITE_SIZE counters (each scaled by the reading of a known-size copy); the summary file
 # holds the commands and the raw readings.
 TRAFFIC_SUMMARY = ROOT / "profiles" / "pass_kernel_bf16_1Mx50_k15_traffic.md"
+FP16_TRAFFIC_SUMMARY = ROOT / "profiles" / "pass_local_kernel_fp16_1Mx50_k15_traffic.md"
 
 
 def parse_args():
@@ -55,6 +56,7 @@ def parse_args():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cosine", action="store_true")
+    p.add_argument("--no-fp16", action="store_true", help="skip the fp16 local-origin block (c3)")
     p.add_argument("--no-tsne", action="store_true", help="skip the t-SNE input distances block")
     p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
@@ -111,10 +113,10 @@ def fixture_check(name: str, x, ids, dist, index: int | None = None) -> dict:
             "rows": int(ids.shape[0])}
 
 
-def measured_traffic(n: int, d: int, k: int):
-    if (n, d, k) != (1_000_000, 50, 15) or not TRAFFIC_SUMMARY.exists():
+def measured_traffic(n: int, d: int, k: int, summary: Path = TRAFFIC_SUMMARY):
+    if (n, d, k) != (1_000_000, 50, 15) or not summary.exists():
         return None
-    for line in TRAFFIC_SUMMARY.read_text().splitlines():
+    for line in summary.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
@@ -442,6 +444,38 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
                        "identical_to_bf16": bool(np.array_equal(ids32, got_ids) and
                                                  np.array_equal(dist32, got_dist))}
         out["bf16_over_fp32"] = (n / sec) / (n / sec32)
+
+    if not args.no_fp16 and name == "c3":
+        # The local-origin fp16 candidate pass beside the bf16 headline: the same graph from
+        # one fp16 product per feature (64 packed features per pair at d = 50).
+        w16, st16 = single.timed(max(2, args.steps // 2), 1, "fp16")
+        ids16, dist16 = single.result()
+        dev16 = stage_means(st16)
+        sec16 = sum(w16) / len(w16)
+        ach16 = flops / (dev16["pass_ms"] * 1e-3) / 1e12
+        nch16 = (d + 3 + 31) // 32
+        nch16 = 4 if nch16 > 2 else nch16
+        out["fp16"] = {"cells_per_s": n / sec16, "ms_per_step": sec16 * 1e3,
+                       "device_cells_per_s_best": n / (min(x["device_ms"] for x in st16) * 1e-3),
+                       "stages_ms": dev16, "fallback_rows": int(st16[-1]["fallback_rows"]),
+                       "path": st16[-1]["path"], "candidates": int(st16[-1]["candidates"]),
+                       "roofline": {"bound": "mfma", "achieved": ach16, "peak": BF16_DENSE_PEAK_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": ach16 / BF16_DENSE_PEAK_TFLOPS,
+                                    "traffic": measured_traffic(n, d, k, FP16_TRAFFIC_SUMMARY),
+                                    "kernel": f"pass_local_kernel<{nch16}> (fp16 local-origin operands)",
+                                    "algorithmic_flops": flops,
+                                    "executed_tflops": 2.0 * n * ((n + 511) // 512 * 512) * 32 * nch16
+                                                       / (dev16["pass_ms"] * 1e-3) / 1e12},
+                       "identical_to_bf16": bool(np.array_equal(ids16, got_ids) and np.array_equal(dist16, got_dist)),
+                       "fp16_over_bf16": (n / sec16) / (n / sec)}
+        if fixture:
+            out["fp16"]["whole_graph"] = fixture_check(fixture, x, ids16, dist16)
+        if "fp32" in out:
+            out["fp16"]["fp16_over_fp32"] = (n / sec16) / out["fp32"]["cells_per_s"]
+        if not args.no_cosine:
+            single.step("fp16", "cosine")
+            ci16, cd16 = single.result()
+            out["fp16"]["cosine_whole_graph"] = fixture_check("c3_cosine", x, ci16, cd16) if fixture else None
 
     rng = np.random.default_rng(0)
     rows = np.sort(rng.choice(n, min(n, args.parity_rows), replace=False)).astype(np.uint32)
