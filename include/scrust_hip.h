@@ -130,6 +130,42 @@ scrust_hip_status scrust_hip_knn_device(scrust_hip_context* context, const float
                                         uint32_t* out_indices, float* out_distances,
                                         scrust_hip_knn_stats* stats);
 
+/* ---------------------------------------------------------------------------------------
+ * PCA: the producer of the embedding the graph is built on. Replaces
+ * scrust_core::pca::pca (crates/scrust-core/src/pca.rs:74-215; bound as _scrust.pca,
+ * crates/scrust-py/src/preprocess.rs:154-185): truncated PCA of an n_rows x n_cols CSR matrix
+ * (cells x genes; indptr has n_rows + 1 entries, indices and values indptr[n_rows]) by
+ * randomised SVD with the reference's steps (sketch of n_components + 24 columns, 7 or 4 power
+ * iterations, whitening orthonormalisation, final Gram eigensolve, sign convention, both
+ * explained-variance statistics). Outputs: embedding n_rows x n_components (obsm["X_pca"]),
+ * components n_components x n_cols, explained variance and its ratio (n_components each).
+ * Errors: INVALID_PARAMETER for n_components 0 or above min(n_rows, n_cols), fewer than 2
+ * rows (as the reference), n_components above 104 (this device path's sketch holds at most
+ * 128 columns) and non-finite values; SHAPE for an inconsistent CSR matrix; KERNEL when a
+ * Jacobi eigensolve does not converge. The sketch's normal draws are this library's own
+ * (splitmix64 with Box-Muller), not the rand crate's: results agree with the reference to the
+ * accuracy of a randomised SVD, not bit for bit.
+ * ------------------------------------------------------------------------------------- */
+
+/* Host CSR in, host outputs. */
+scrust_hip_status scrust_hip_pca(scrust_hip_context* context, const uint32_t* indptr,
+                                 const uint32_t* indices, const float* values, size_t n_rows,
+                                 size_t n_cols, size_t n_components, int zero_center,
+                                 uint64_t seed, float* out_embedding, float* out_components,
+                                 float* out_explained_variance,
+                                 float* out_explained_variance_ratio);
+
+/* Device CSR in (device pointers on the context's device) and the embedding left in device
+ * memory (out_embedding a device pointer: the input of scrust_hip_knn_device and
+ * scrust_hip_neighbors_device, nothing crossing PCIe); components and variances in host
+ * memory. No counterpart in the reference (its PCA returns host arrays). */
+scrust_hip_status scrust_hip_pca_device(scrust_hip_context* context, const uint32_t* indptr,
+                                        const uint32_t* indices, const float* values,
+                                        size_t n_rows, size_t n_cols, size_t n_components,
+                                        int zero_center, uint64_t seed, float* out_embedding,
+                                        float* out_components, float* out_explained_variance,
+                                        float* out_explained_variance_ratio);
+
 /* Bytes of the device scratch array scrust_hip_knn_device_noalloc works in (256 MiB). */
 #define SCRUST_HIP_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
 

@@ -35,6 +35,8 @@ __all__ = [
     "neighbors_device",
     "tsne_squared_distances",
     "tsne_squared_distances_device",
+    "pca",
+    "pca_device",
     "tune",
     "tune_device",
     "tune_save",
@@ -336,6 +338,59 @@ def tsne_squared_distances_device(points_ptr: int, n: int, d: int, out_ptr: int,
     status = _native.load().scrust_hip_tsne_squared_distances_device(
         ctx.handle, ctypes.c_void_p(points_ptr), n, d, ctypes.c_void_p(out_ptr))
     _native.raise_for(status)
+
+
+def _csr_array(a, dtype, name):
+    if not isinstance(a, np.ndarray) or a.ndim != 1:
+        raise ValueError(f"{name} must be a 1-D numpy array")
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def pca(indptr, indices, values, n_cols: int, n_components: int, zero_center: bool = True,
+        seed: int = 0, device: str = "auto", *, context: Context | None = None) -> dict:
+    """Mirror of ``_scrust.pca`` (crates/scrust-py/src/preprocess.rs:154-185): truncated PCA
+    of a CSR matrix (cells x genes) by randomised SVD on the device. Returns the dict the
+    reference returns: ``embedding`` (n x n_components), ``components`` (n_components x
+    n_cols), ``explained_variance``, ``explained_variance_ratio``."""
+    if device not in DEVICES:
+        raise ValueError(f"device must be one of auto, gpu, hip, got {device}")
+    ip = _csr_array(indptr, np.uint32, "indptr")
+    ix = _csr_array(indices, np.uint32, "indices")
+    vs = _csr_array(values, np.float32, "values")
+    n = ip.size - 1
+    if n < 0 or ix.size != vs.size or (n >= 0 and ip.size and int(ip[-1]) != ix.size):
+        raise ValueError(f"shape mismatch: expected indptr[-1] == len(indices) == len(values), got "
+                         f"{int(ip[-1]) if ip.size else None}, {ix.size}, {vs.size}")
+    k = int(n_components)
+    emb = np.empty((max(n, 0), k), dtype=np.float32)
+    comp = np.empty((k, int(n_cols)), dtype=np.float32)
+    ev = np.empty(k, dtype=np.float32)
+    evr = np.empty(k, dtype=np.float32)
+    handle = _handle_or_none(context)
+    status = _native.load().scrust_hip_pca(
+        handle, ip.ctypes.data, ix.ctypes.data, vs.ctypes.data, max(n, 0), int(n_cols), k, int(bool(zero_center)),
+        int(seed), emb.ctypes.data, comp.ctypes.data, ev.ctypes.data, evr.ctypes.data)
+    _native.raise_for(status)
+    return {"embedding": emb, "components": comp, "explained_variance": ev, "explained_variance_ratio": evr}
+
+
+def pca_device(indptr_ptr: int, indices_ptr: int, values_ptr: int, n_rows: int, n_cols: int,
+               n_components: int, out_embedding_ptr: int, *, zero_center: bool = True, seed: int = 0,
+               context: Context | None = None) -> dict:
+    """PCA of a device-resident CSR matrix; the embedding (n_rows x n_components float32) is
+    written to device memory at ``out_embedding_ptr``, ready for ``knn_device`` or
+    ``neighbors_device``. Returns the components and variances (host arrays)."""
+    ctx = context if context is not None else _thread_context()
+    k = int(n_components)
+    comp = np.empty((k, int(n_cols)), dtype=np.float32)
+    ev = np.empty(k, dtype=np.float32)
+    evr = np.empty(k, dtype=np.float32)
+    status = _native.load().scrust_hip_pca_device(
+        ctx.handle, ctypes.c_void_p(indptr_ptr), ctypes.c_void_p(indices_ptr), ctypes.c_void_p(values_ptr),
+        int(n_rows), int(n_cols), k, int(bool(zero_center)), int(seed), ctypes.c_void_p(out_embedding_ptr),
+        comp.ctypes.data, ev.ctypes.data, evr.ctypes.data)
+    _native.raise_for(status)
+    return {"components": comp, "explained_variance": ev, "explained_variance_ratio": evr}
 
 
 def tune(embedding, k: int, *, metric: str = "euclidean", precision: str = "bf16",

@@ -44,6 +44,8 @@ EXPORTS = (
     "scrust_hip_connectivities_device",
     "scrust_hip_neighbors",
     "scrust_hip_neighbors_device",
+    "scrust_hip_pca",
+    "scrust_hip_pca_device",
     "scrust_hip_tsne_squared_distances",
     "scrust_hip_tsne_squared_distances_device",
     "scrust_hip_tune",
@@ -248,6 +250,10 @@ def load() -> ctypes.CDLL:
     lib.scrust_hip_knn_ex.restype = ctypes.c_int
     lib.scrust_hip_knn_device.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
     lib.scrust_hip_knn_device.restype = ctypes.c_int
+    lib.scrust_hip_pca.argtypes = [vp, vp, vp, vp, sz, sz, sz, ctypes.c_int, ctypes.c_uint64, vp, vp, vp, vp]
+    lib.scrust_hip_pca.restype = ctypes.c_int
+    lib.scrust_hip_pca_device.argtypes = [vp, vp, vp, vp, sz, sz, sz, ctypes.c_int, ctypes.c_uint64, vp, vp, vp, vp]
+    lib.scrust_hip_pca_device.restype = ctypes.c_int
     lib.scrust_hip_knn_device_noalloc.argtypes = [vp, vp, sz, sz, sz, opts, vp, vp, stats]
     lib.scrust_hip_knn_device_noalloc.restype = ctypes.c_int
     lib.scrust_hip_connectivities.argtypes = [vp, vp, vp, sz, sz, vp, vp, vp, sz, ctypes.POINTER(sz)]

@@ -173,6 +173,7 @@ struct scrust_hip_context {
     scrust_hip::Buffer conn;                              // connectivities workspace
     scrust_hip::Buffer conn_in, conn_out;                 // its host-input staging
     scrust_hip::Buffer tsne, tsne_out;                    // t-SNE distances workspace, output staging
+    scrust_hip::Buffer pca, pca_io;                       // PCA workspace, its host-input staging
     ~scrust_hip_context();
 };
 

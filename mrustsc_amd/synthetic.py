@@ -131,3 +131,30 @@ def batch_specs(count: int = 64, seed: int = 7) -> list[dict]:
 def batch_dataset(spec: dict) -> np.ndarray:
     return anisotropic_mixture(spec["n"], spec["d"], spec["clusters"], spec["seed"],
                                duplicate_fraction=0.005)
+
+
+def expression_csr(n_cells: int, n_genes: int, n_clusters: int = 12, density: float = 0.1,
+                   seed: int = 0):
+    """A cells x genes expression matrix in CSR form (indptr, indices, values as uint32,
+    uint32, float32): each cluster has its own mean profile over the genes; a cell keeps
+    each gene with probability `density` (raised for the cluster's marker genes) at
+    log1p(Poisson(profile)) + 1. The input of the PCA that produces the kNN embedding."""
+    rng = np.random.default_rng(seed)
+    profiles = np.exp(rng.normal(0.0, 1.0, size=(n_clusters, n_genes)))
+    markers = rng.random((n_clusters, n_genes)) < 0.05
+    z = rng.integers(0, n_clusters, size=n_cells)
+    indptr = np.zeros(n_cells + 1, dtype=np.uint64)
+    cols, vals = [], []
+    block = 8192
+    for r0 in range(0, n_cells, block):
+        zz = z[r0:r0 + block]
+        keep = rng.random((zz.size, n_genes)) < np.where(markers[zz], 0.8, density)
+        rows, cc = np.nonzero(keep)
+        lam = profiles[zz[rows], cc] * (1.0 + 4.0 * markers[zz[rows], cc])
+        cols.append(cc.astype(np.uint32))
+        vals.append(np.log1p(rng.poisson(lam) + 1).astype(np.float32))
+        indptr[r0 + 1:r0 + zz.size + 1] = np.cumsum(keep.sum(axis=1))
+    offsets = np.cumsum([0] + [c.size for c in cols]).astype(np.uint64)
+    for i, r0 in enumerate(range(0, n_cells, block)):
+        indptr[r0 + 1:min(n_cells, r0 + block) + 1] += offsets[i]
+    return indptr.astype(np.uint32), np.concatenate(cols), np.concatenate(vals)

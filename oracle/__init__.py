@@ -52,6 +52,13 @@ def _load():
     lib.oracle_knn_fast.argtypes = [f32p, sz, sz, sz, ctypes.c_int, u32p, sz, ctypes.c_int, u32p,
                                     f32p]
     lib.oracle_knn_fast.restype = ctypes.c_int
+    lib.oracle_pca.argtypes = [u32p, u32p, f32p, sz, sz, sz, ctypes.c_int, ctypes.c_uint64, f32p, f32p, f32p, f32p]
+    lib.oracle_pca.restype = ctypes.c_int
+    lib.oracle_pca_sketch.argtypes = [sz, sz, ctypes.c_uint64, f32p]
+    lib.oracle_pca_sketch.restype = None
+    lib.oracle_jacobi_eigen.argtypes = [ctypes.POINTER(ctypes.c_double), sz, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_jacobi_eigen.restype = ctypes.c_int
     lib.oracle_knn_fp64.argtypes = [f32p, sz, sz, sz, ctypes.c_int, u32p, sz, ctypes.c_int,
                                     u32p, f64p]
     lib.oracle_knn_fp64.restype = ctypes.c_int
@@ -230,3 +237,46 @@ def tsne_squared_distances(points, *, rows=None, threads: int = 0) -> np.ndarray
 def threads_env_default() -> int:
     value = os.environ.get("OMP_NUM_THREADS")
     return int(value) if value and value.isdigit() else 0
+
+
+def pca(indptr, indices, values, n_cols: int, n_components: int, *, zero_center: bool = True,
+        seed: int = 0) -> dict:
+    """Randomised-SVD PCA of a CSR matrix (pca.rs:74-215 restated with f64 algebra): the
+    dict the reference's _scrust.pca returns (embedding, components, explained_variance,
+    explained_variance_ratio)."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.uint32)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    values = np.ascontiguousarray(values, dtype=np.float32)
+    n = indptr.size - 1
+    emb = np.empty((n, n_components), dtype=np.float32)
+    comp = np.empty((n_components, n_cols), dtype=np.float32)
+    ev = np.empty(n_components, dtype=np.float32)
+    evr = np.empty(n_components, dtype=np.float32)
+    rc = _load().oracle_pca(_ptr(indptr, ctypes.c_uint32), _ptr(indices, ctypes.c_uint32),
+                            _ptr(values, ctypes.c_float), n, n_cols, n_components, int(zero_center),
+                            seed, _ptr(emb, ctypes.c_float), _ptr(comp, ctypes.c_float),
+                            _ptr(ev, ctypes.c_float), _ptr(evr, ctypes.c_float))
+    if rc == -1:
+        raise ValueError(f"invalid arguments: n={n} n_cols={n_cols} n_components={n_components}")
+    if rc != 0:
+        raise RuntimeError("Jacobi eigenvalue iteration did not converge")
+    return {"embedding": emb, "components": comp, "explained_variance": ev,
+            "explained_variance_ratio": evr}
+
+
+def pca_sketch(rows: int, cols: int, seed: int) -> np.ndarray:
+    out = np.empty((rows, cols), dtype=np.float32)
+    _load().oracle_pca_sketch(rows, cols, seed, _ptr(out, ctypes.c_float))
+    return out
+
+
+def jacobi_eigen(a: np.ndarray):
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    n = a.shape[0]
+    vals = np.empty(n)
+    vecs = np.empty((n, n))
+    p = ctypes.POINTER(ctypes.c_double)
+    rc = _load().oracle_jacobi_eigen(a.ctypes.data_as(p), n, vals.ctypes.data_as(p), vecs.ctypes.data_as(p))
+    if rc != 0:
+        raise RuntimeError("Jacobi eigenvalue iteration did not converge")
+    return vals, vecs

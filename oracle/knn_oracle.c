@@ -993,3 +993,268 @@ int oracle_tsne_squared_distances(const float* x, size_t n, size_t d, size_t row
     free(mean);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* PCA by randomised SVD (crates/scrust-core/src/pca.rs:74-215), in f64.   */
+/*                                                                          */
+/* The steps of pca.rs: per-gene f64 sums in row order (ColumnStats, :226-  */
+/* 255), a Gaussian sketch of width min(n_components + 24, min(n, g))       */
+/* (:21, :103), orthonormalisation by two whitening passes with the Gram    */
+/* eigenvalues clamped at 1e-7 of the largest (:352-386), 7 or 4 power      */
+/* iterations by the 0.1 rule (:111-119), the final Gram eigensolve of      */
+/* B^T (:148-166), embedding and loadings (:168-190), the sign convention   */
+/* (:471-490) and both explained-variance statistics (:192-208). Dense      */
+/* algebra is f64 here (the reference runs it in f32 on candle); the        */
+/* sketch's normal draws come from pca_sketch below, not from the rand      */
+/* crate's StdRng (rand 0.8, not under the reference tree).                 */
+/* ------------------------------------------------------------------------ */
+
+/* splitmix64 stream from `seed`; pairs of 53-bit uniforms through Box-Muller in f64,
+ * rounded to f32, row-major rows x cols. The device library draws the same values. */
+void oracle_pca_sketch(size_t rows, size_t cols, uint64_t seed, float* out) {
+    uint64_t state = seed;
+    size_t count = rows * cols, i = 0;
+    while (i < count) {
+        double u[2];
+        for (int t = 0; t < 2; t++) {
+            state += 0x9E3779B97F4A7C15ull;
+            uint64_t z = state;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            u[t] = (double)(z >> 11) * 0x1p-53;
+        }
+        double uniform = u[0] > DBL_MIN ? u[0] : DBL_MIN;
+        double radius = sqrt(-2.0 * log(uniform));
+        double angle = 6.283185307179586 * u[1];
+        out[i++] = (float)(radius * cos(angle));
+        if (i < count) out[i++] = (float)(radius * sin(angle));
+    }
+}
+
+/* Cyclic Jacobi rotations of a symmetric n x n row-major matrix (pca.rs:390-445): returns 0
+ * and the eigenvalues (diagonal) and eigenvectors (columns), or -1 when 60 sweeps do not
+ * bring the off-diagonal mass to eps^2 of the total. */
+int oracle_jacobi_eigen(double* a, size_t n, double* values, double* vectors) {
+    for (size_t i = 0; i < n * n; i++) vectors[i] = 0.0;
+    for (size_t i = 0; i < n; i++) vectors[i * n + i] = 1.0;
+    double scale = 0.0;
+    for (size_t i = 0; i < n * n; i++) scale += a[i] * a[i];
+    const double tolerance = DBL_EPSILON * DBL_EPSILON * (scale > DBL_MIN ? scale : DBL_MIN);
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0;
+        for (size_t r = 0; r < n; r++)
+            for (size_t c = 0; c < n; c++)
+                if (r != c) off += a[r * n + c] * a[r * n + c];
+        if (off <= tolerance) {
+            for (size_t i = 0; i < n; i++) values[i] = a[i * n + i];
+            return 0;
+        }
+        for (size_t p = 0; p < n; p++) {
+            for (size_t q = p + 1; q < n; q++) {
+                double pivot = a[p * n + q];
+                if (pivot == 0.0) continue;
+                double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * pivot);
+                double sgn = theta > 0.0 ? 1.0 : (theta < 0.0 ? -1.0 : (signbit(theta) ? -1.0 : 1.0));
+                double den = fabs(theta) + sqrt(theta * theta + 1.0);
+                double t = sgn / (den > DBL_EPSILON ? den : DBL_EPSILON);
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (size_t k = 0; k < n; k++) {
+                    double l = a[k * n + p], r = a[k * n + q];
+                    a[k * n + p] = c * l - s * r;
+                    a[k * n + q] = s * l + c * r;
+                }
+                for (size_t k = 0; k < n; k++) {
+                    double l = a[p * n + k], r = a[q * n + k];
+                    a[p * n + k] = c * l - s * r;
+                    a[q * n + k] = s * l + c * r;
+                }
+                a[p * n + q] = 0.0;
+                a[q * n + p] = 0.0;
+                for (size_t k = 0; k < n; k++) {
+                    double l = vectors[k * n + p], r = vectors[k * n + q];
+                    vectors[k * n + p] = c * l - s * r;
+                    vectors[k * n + q] = s * l + c * r;
+                }
+            }
+        }
+    }
+    return -1;
+}
+
+/* y (rows x w) <- y T with T = V diag(1 / sqrt(max(lambda, 1e-7 lambda_max))) of y^T y. */
+static int pca_whiten(double* y, size_t rows, size_t w, double* scratch) {
+    double* gram = scratch;
+    double* values = gram + w * w;
+    double* vectors = values + w;
+    double* t = vectors + w * w;
+    double* row = t + w * w;
+    for (size_t i = 0; i < w * w; i++) gram[i] = 0.0;
+    for (size_t r = 0; r < rows; r++)
+        for (size_t i = 0; i < w; i++)
+            for (size_t j = 0; j < w; j++) gram[i * w + j] += y[r * w + i] * y[r * w + j];
+    if (oracle_jacobi_eigen(gram, w, values, vectors) != 0) return -1;
+    double largest = 0.0;
+    for (size_t i = 0; i < w; i++) largest = values[i] > largest ? values[i] : largest;
+    double floor_value = largest * 1e-7;
+    for (size_t c = 0; c < w; c++) {
+        double v = values[c] > floor_value ? values[c] : floor_value;
+        double sc = floor_value > 0.0 ? 1.0 / sqrt(v) : 0.0;
+        for (size_t r = 0; r < w; r++) t[r * w + c] = vectors[r * w + c] * sc;
+    }
+    for (size_t r = 0; r < rows; r++) {
+        for (size_t c = 0; c < w; c++) {
+            double s = 0.0;
+            for (size_t k = 0; k < w; k++) s += y[r * w + k] * t[k * w + c];
+            row[c] = s;
+        }
+        memcpy(y + r * w, row, w * sizeof(double));
+    }
+    return 0;
+}
+
+/* X_c @ rhs (n x w) with rhs g x w; or X_c^T @ lhs (g x w) with lhs n x w. */
+static void pca_times(const uint32_t* indptr, const uint32_t* indices, const float* values, size_t n, size_t g,
+                      const double* mean, const double* rhs, size_t w, double* out) {
+    double* shift = (double*)calloc(w, sizeof(double));
+    if (mean)
+        for (size_t k = 0; k < g; k++)
+            for (size_t c = 0; c < w; c++) shift[c] += mean[k] * rhs[k * w + c];
+    for (size_t r = 0; r < n; r++) {
+        for (size_t c = 0; c < w; c++) out[r * w + c] = -shift[c];
+        for (uint32_t j = indptr[r]; j < indptr[r + 1]; j++)
+            for (size_t c = 0; c < w; c++) out[r * w + c] += (double)values[j] * rhs[(size_t)indices[j] * w + c];
+    }
+    free(shift);
+}
+
+static void pca_transpose_times(const uint32_t* indptr, const uint32_t* indices, const float* values, size_t n,
+                                size_t g, const double* mean, const double* lhs, size_t w, double* out) {
+    for (size_t i = 0; i < g * w; i++) out[i] = 0.0;
+    for (size_t r = 0; r < n; r++)
+        for (uint32_t j = indptr[r]; j < indptr[r + 1]; j++)
+            for (size_t c = 0; c < w; c++) out[(size_t)indices[j] * w + c] += (double)values[j] * lhs[r * w + c];
+    if (mean) {
+        double* sums = (double*)calloc(w, sizeof(double));
+        for (size_t r = 0; r < n; r++)
+            for (size_t c = 0; c < w; c++) sums[c] += lhs[r * w + c];
+        for (size_t k = 0; k < g; k++)
+            for (size_t c = 0; c < w; c++) out[k * w + c] -= mean[k] * sums[c];
+        free(sums);
+    }
+}
+
+int oracle_pca(const uint32_t* indptr, const uint32_t* indices, const float* values, size_t n, size_t g,
+               size_t n_components, int zero_center, uint64_t seed, float* embedding, float* components,
+               float* explained_variance, float* explained_variance_ratio) {
+    size_t max_components = n < g ? n : g;
+    if (n_components == 0 || n_components > max_components || n < 2) return -1;
+    /* ColumnStats: f64 sums and squares in row order. */
+    double* sums = (double*)calloc(g, sizeof(double));
+    double* squares = (double*)calloc(g, sizeof(double));
+    for (size_t j = 0; j < indptr[n]; j++) {
+        double v = (double)values[j];
+        sums[indices[j]] += v;
+        squares[indices[j]] += v * v;
+    }
+    double* mean = (double*)malloc(g * sizeof(double));
+    double ddof = zero_center ? 1.0 : 0.0, total_variance = 0.0;
+    for (size_t k = 0; k < g; k++) {
+        mean[k] = (double)(float)(sums[k] / (double)n); /* the reference's mean is f32 */
+        double var = squares[k] - sums[k] * sums[k] / (double)n;
+        total_variance += (var > 0.0 ? var : 0.0) / ((double)n - ddof);
+    }
+    const double* centre = zero_center ? mean : NULL;
+    size_t w = n_components + 24 < max_components ? n_components + 24 : max_components;
+    float* omega32 = (float*)malloc(g * w * sizeof(float));
+    oracle_pca_sketch(g, w, seed, omega32);
+    double* omega = (double*)malloc(g * w * sizeof(double));
+    for (size_t i = 0; i < g * w; i++) omega[i] = omega32[i];
+    size_t big = n > g ? n : g;
+    double* range = (double*)malloc(n * w * sizeof(double));
+    double* genewise = (double*)malloc(g * w * sizeof(double));
+    double* scratch = (double*)malloc((3 * w * w + 2 * w + big) * sizeof(double));
+    int rc = 0;
+    pca_times(indptr, indices, values, n, g, centre, omega, w, range);
+    rc |= pca_whiten(range, n, w, scratch) | pca_whiten(range, n, w, scratch);
+    int iterations = (double)n_components < 0.1 * (double)max_components ? 7 : 4;
+    for (int it = 0; it < iterations && rc == 0; it++) {
+        pca_transpose_times(indptr, indices, values, n, g, centre, range, w, genewise);
+        rc |= pca_whiten(genewise, g, w, scratch) | pca_whiten(genewise, g, w, scratch);
+        pca_times(indptr, indices, values, n, g, centre, genewise, w, range);
+        rc |= pca_whiten(range, n, w, scratch) | pca_whiten(range, n, w, scratch);
+    }
+    double* bt = genewise; /* B^T = X_c^T range, g x w */
+    double* gram = (double*)calloc(w * w, sizeof(double));
+    double* evals = (double*)malloc(w * sizeof(double));
+    double* evecs = (double*)malloc(w * w * sizeof(double));
+    size_t* order = (size_t*)malloc(w * sizeof(size_t));
+    if (rc == 0) {
+        pca_transpose_times(indptr, indices, values, n, g, centre, range, w, bt);
+        for (size_t k = 0; k < g; k++)
+            for (size_t i = 0; i < w; i++)
+                for (size_t j = 0; j < w; j++) gram[i * w + j] += bt[k * w + i] * bt[k * w + j];
+        rc = oracle_jacobi_eigen(gram, w, evals, evecs);
+    }
+    if (rc == 0) {
+        /* Descending eigenvalues, ties in index order (a stable sort). */
+        for (size_t i = 0; i < w; i++) order[i] = i;
+        for (size_t i = 1; i < w; i++) {
+            size_t v = order[i], j = i;
+            while (j > 0 && evals[order[j - 1]] < evals[v]) {
+                order[j] = order[j - 1];
+                j--;
+            }
+            order[j] = v;
+        }
+        double* singular = (double*)malloc(n_components * sizeof(double));
+        for (size_t c = 0; c < n_components; c++) {
+            double e = evals[order[c]];
+            singular[c] = sqrt(e > 0.0 ? e : 0.0);
+        }
+        for (size_t r = 0; r < n; r++)
+            for (size_t c = 0; c < n_components; c++) {
+                double s = 0.0;
+                for (size_t k = 0; k < w; k++) s += range[r * w + k] * evecs[k * w + order[c]];
+                embedding[r * n_components + c] = (float)(s * singular[c]);
+            }
+        for (size_t c = 0; c < n_components; c++) {
+            double inverse = singular[c] > 0.0 ? 1.0 / singular[c] : 0.0;
+            for (size_t k = 0; k < g; k++) {
+                double s = 0.0;
+                for (size_t i = 0; i < w; i++) s += bt[k * w + i] * evecs[i * w + order[c]];
+                components[c * g + k] = (float)(s * inverse);
+            }
+        }
+        /* Signs: the largest-magnitude loading of every component is made positive. */
+        for (size_t c = 0; c < n_components; c++) {
+            float extreme = 0.0f;
+            for (size_t k = 0; k < g; k++)
+                if (fabsf(components[c * g + k]) > fabsf(extreme)) extreme = components[c * g + k];
+            if (extreme < 0.0f) {
+                for (size_t k = 0; k < g; k++) components[c * g + k] = -components[c * g + k];
+                for (size_t r = 0; r < n; r++) embedding[r * n_components + c] = -embedding[r * n_components + c];
+            }
+        }
+        for (size_t c = 0; c < n_components; c++) {
+            if (zero_center) {
+                explained_variance[c] = (float)(singular[c] * singular[c] / (double)(n - 1));
+            } else {
+                double m = 0.0, v = 0.0;
+                for (size_t r = 0; r < n; r++) m += embedding[r * n_components + c];
+                m /= (double)n;
+                for (size_t r = 0; r < n; r++) {
+                    double d = embedding[r * n_components + c] - m;
+                    v += d * d;
+                }
+                explained_variance[c] = (float)(v / (double)n);
+            }
+            explained_variance_ratio[c] =
+                total_variance > 0.0 ? (float)((double)explained_variance[c] / total_variance) : 0.0f;
+        }
+        free(singular);
+    }
+    free(sums); free(squares); free(mean); free(omega32); free(omega); free(range); free(genewise);
+    free(scratch); free(gram); free(evals); free(evecs); free(order);
+    return rc == 0 ? 0 : -2;
+}

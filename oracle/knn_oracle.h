@@ -89,6 +89,22 @@ long oracle_connectivities(const uint32_t* indices, const float* distances, size
 int oracle_tsne_squared_distances(const float* x, size_t n, size_t d, size_t row_begin,
                                   size_t row_end, float* out, int threads);
 
+/* PCA by randomised SVD restating crates/scrust-core/src/pca.rs:74-215 with f64 algebra:
+ * CSR (indptr n + 1, indices, values) of n cells x g genes in; embedding n x n_components,
+ * components n_components x g, explained variance and its ratio out. Returns 0, -1 for
+ * invalid arguments (pca.rs:84-100), -2 when a Jacobi eigensolve does not converge. */
+int oracle_pca(const uint32_t* indptr, const uint32_t* indices, const float* values, size_t n, size_t g,
+               size_t n_components, int zero_center, uint64_t seed, float* embedding, float* components,
+               float* explained_variance, float* explained_variance_ratio);
+
+/* The sketch both the oracle and the device library draw: splitmix64 from `seed`, pairs of
+ * 53-bit uniforms through Box-Muller in f64, rounded to f32; rows x cols row-major. */
+void oracle_pca_sketch(size_t rows, size_t cols, uint64_t seed, float* out);
+
+/* Cyclic Jacobi eigensolve of a symmetric n x n matrix (pca.rs:390-445); a is destroyed.
+ * Returns 0, or -1 without convergence in 60 sweeps. */
+int oracle_jacobi_eigen(double* a, size_t n, double* values, double* vectors);
+
 #ifdef __cplusplus
 }
 #endif
