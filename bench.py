@@ -57,6 +57,7 @@ def parse_args():
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cosine", action="store_true")
     p.add_argument("--no-fp16", action="store_true", help="skip the fp16 local-origin block (c3)")
+    p.add_argument("--no-large-k", action="store_true", help="skip the k = 100 block (c2)")
     p.add_argument("--no-tsne", action="store_true", help="skip the t-SNE input distances block")
     p.add_argument("--cpu-rows", type=int, default=32768)
     p.add_argument("--parity-rows", type=int, default=256)
@@ -92,7 +93,7 @@ def host_cores() -> int:
 GOLDEN = ROOT / "tests" / "golden"
 
 
-def fixture_check(name: str, x, ids, dist, index: int | None = None) -> dict:
+def fixture_check(name: str, x, ids, dist, index: int | None = None, key: str | None = None) -> dict:
     """The whole graph against a committed oracle fixture (digests of the oracle's ids and
     distances over every row, written by tests/golden/make_config_fixtures.py)."""
     import hashlib
@@ -105,6 +106,8 @@ def fixture_check(name: str, x, ids, dist, index: int | None = None) -> dict:
     fix = json.loads(path.read_text())
     if index is not None:
         fix = fix["datasets"][index]
+    if key is not None:
+        fix = fix["graphs"][key]
     h = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
     return {"fixture": path.name, "checked": True,
             "embedding_equal": h(x) == fix["x_sha256"],
@@ -516,6 +519,8 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, k, cpu_rows, cpu_repeats)
     del single
+    if name == "c2" and k == 15 and not args.no_large_k:
+        out["k100"] = large_k_block(args, x, ctx, 100)
     torch.cuda.synchronize()
     ctx.close()
     return out
@@ -554,7 +559,27 @@ def config_block(args, name, x, workload, ctx, cpu_repeats=1):
     if not args.no_cpu:
         block["cpu_baseline"] = cpu_baseline(x, k, None, cpu_repeats)
     del single
+    if name == "c2" and not args.no_large_k:
+        block["k100"] = large_k_block(args, x, ctx, 100)
     return block
+
+
+def large_k_block(args, x, ctx, k):
+    """The same embedding at a large k (the 128-candidate pass): bf16 builds, the whole graph
+    against its oracle fixture."""
+    n, d = x.shape
+    single = Single(x, k, ctx)
+    walls, st = single.timed(max(3, args.steps // 2), 2, "bf16")
+    dev = stage_means(st)
+    ids, dist = single.result()
+    flops = 2.0 * n * n * d
+    out = {"k": k, "candidates": int(st[-1]["candidates"]), "cells_per_s": n / (sum(walls) / len(walls)),
+           "cells_per_s_best": n / min(walls), "ms_per_step": sum(walls) / len(walls) * 1e3,
+           "stages_ms": dev, "fallback_rows": int(st[-1]["fallback_rows"]),
+           "roofline_frac": flops / (dev["pass_ms"] * 1e-3) / 1e12 / BF16_DENSE_PEAK_TFLOPS,
+           "whole_graph": fixture_check("c2_large_k", x, ids, dist, key=str(k))}
+    del single
+    return out
 
 
 def run_batch_config(args):

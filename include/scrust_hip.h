@@ -44,13 +44,13 @@ typedef enum scrust_hip_precision {
     SCRUST_HIP_BF16 = 0,
     /* The same pipeline with an fp32 MFMA candidate pass: the plain FP32 kernel. */
     SCRUST_HIP_FP32 = 1,
-    /* fp32 direct differences over every pair (no MFMA); the path for k > 47 or d > 128. */
+    /* fp32 direct differences over every pair (no MFMA); the path for k > 111 or d > 128. */
     SCRUST_HIP_EXACT = 2,
     /* fp16 single products of locally centred coordinates on MFMA (one origin per 512 query
      * rows of the spatial row order, fp32 accumulate, a rigorous per-pair error term folded
      * into the product) select the candidates; the same rerank and certificate follow. Single
-     * builds whose rows are grouped (from 65,536 cells, or order = 2), d <= 125; other shapes
-     * take the bf16 path. No counterpart in the reference. */
+     * builds whose rows are grouped (from 65,536 cells, or order = 2), k <= 47, d <= 125;
+     * other shapes take the bf16 path. No counterpart in the reference. */
     SCRUST_HIP_FP16 = 3
 } scrust_hip_precision;
 
@@ -264,7 +264,7 @@ scrust_hip_status scrust_hip_tsne_squared_distances_device(scrust_hip_context* c
  * the row order (input or grouped by nearest pivot). Every shape gives the same graph.
  * The tuner times the shapes on an embedding, checks that all of them return the same
  * indices and distances, and records the fastest in a process-wide table keyed by bucket:
- * path, metric, candidates kept (k <= 15: 32, else 64), packed feature chunks (from d) and
+ * path, metric, candidates kept (k <= 15: 32, k <= 47: 64, else 128), packed feature chunks (from d) and
  * n in half-octaves. Single builds and server requests with tile = 0 take the table's shape
  * for their bucket. The table is shared by every context and server of the process and can
  * be saved to and loaded from a text file, so tuning runs once per machine. Batches are

@@ -404,6 +404,63 @@ void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsign
 //    completeness certificate. KP lanes per query row.
 // ===========================================================================
 
+// Certificate of one pass row whose kept list was sorted: every reference outside the list
+// has approximate score >= theta, so its fp32 squared distance is >= `lower`; if that
+// clears the band's upper edge t + tau and every snapping threshold, the list holds every
+// reference the oracle could place in the first k (or in the tie band).
+// A list shorter than KP holds every reference appended over the whole scan; it is complete
+// only if that is every other cell (scores that overflowed are never appended). A dataset
+// past the pass's norm limit skipped the pass: every row is listed.
+__device__ __forceinline__ bool row_certified(const DatasetDesc& s, uint32_t p, float nq, int d, double t,
+                                              double tau, bool full, int n_valid) {
+    const float max_norm = __uint_as_float(*s.max_norm_bits);
+    bool certified = full || n_valid == static_cast<int>(s.n) - 1;
+    if (!(max_norm <= kPassNormLimit)) certified = false;
+    if (full && certified) {
+        const double u = 0x1p-24;
+        const double nqd = static_cast<double>(nq);
+        const double nmax = static_cast<double>(max_norm);
+        const double gamma = d * u / (1.0 - d * u);
+        const double qn = __builtin_sqrt(nqd * (1.0 + gamma));
+        const double rn = __builtin_sqrt(nmax * (1.0 + gamma));
+        // |score + |a|^2 - exact d^2| <= err: f32 norms, the bf16 split of the
+        // coordinates and of |b|^2, and the MFMA accumulation over every packed term.
+        const double err = 1.25 * (gamma * (nqd + nmax) * 1.01 + 2.0 * s.split_rel * qn * rn +
+                                   2.0 * s.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
+                                   s.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
+        const double cd = (d + 3) * u * 1.01;
+        double lower = (static_cast<double>(s.theta[p]) + nqd - err) * (1.0 - cd);
+        if (s.local) {
+            // Local-origin pass, in units of sigma (local_scale): a reference not kept has
+            // pass score >= theta, and the score is a lower bound of |b''|^2 - 2 a''.b'' up
+            // to the per-row term 2 kq (fp16 rounding below the normal range), so
+            // |a'' - b''|^2 >= theta + |a''|^2 - 2 kq. Then a' = fl(a - o) differs from a - o
+            // by at most u |a - o| per coordinate: |a - b| >= |a' - b'| (1 - 2.02 u) - 2.02 u |a'|.
+            const double sigma = static_cast<double>(local_scale(max_norm));
+            const double an2 = static_cast<double>(s.anorm[p]);
+            const double an_up = __builtin_sqrt(an2 / (1.0 - 0x1p-15) * (1.0 + 0x1p-16)) * (1.0 + 0x1p-40);
+            const double kq = static_cast<double>(s.local_c2) * an_up + d * 0x1p-50;
+            const double base = static_cast<double>(s.theta[p]) + an2 - 2.0 * kq;
+            double len = base > 0.0 ? __builtin_sqrt(base) / sigma : 0.0;
+            len = len * (1.0 - 2.02 * u) - 2.02 * u * (an_up / sigma);
+            lower = len > 0.0 ? len * len * (1.0 - cd) : 0.0;
+        }
+        const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);
+        const double need = (t + tau) > snap_max ? (t + tau) : snap_max;
+        certified = lower > need;
+    }
+    return certified;
+}
+
+// f32 direct-difference squared distance of pass rows p and c (the rerank's arithmetic).
+__device__ __forceinline__ float rerank_square(const DatasetDesc& s, const float* xq, float nq, uint32_t c, int d) {
+    const float* xr = s.coords + static_cast<size_t>(c) * d;
+    // Rows are 16- or 8-byte aligned when 4 or 2 divides d (the arena is 256-byte aligned).
+    if ((d & 3) == 0) return direct_square_vec<4>(xq, xr, d, nq, s.norms[c]);
+    if ((d & 1) == 0) return direct_square_vec<2>(xq, xr, d, nq, s.norms[c]);
+    return direct_square(xq, xr, d, nq, s.norms[c]);
+}
+
 template <int KP>
 __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restrict__ sets,
                                                      uint32_t count, uint32_t total_rows,
@@ -432,14 +489,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     const bool present = raw != kNoKey;
     const uint32_t c = key_id(raw);
     const bool valid = present && c != p && c < n;
-    float dsq = kInf;
-    if (valid) {
-        const float* xr = s.coords + static_cast<size_t>(c) * d;
-        // Rows are 16- or 8-byte aligned when 4 or 2 divides d (the arena is 256-byte aligned).
-        if ((d & 3) == 0) dsq = direct_square_vec<4>(xq, xr, d, nq, s.norms[c]);
-        else if ((d & 1) == 0) dsq = direct_square_vec<2>(xq, xr, d, nq, s.norms[c]);
-        else dsq = direct_square(xq, xr, d, nq, s.norms[c]);
-    }
+    const float dsq = valid ? rerank_square(s, xq, nq, c, d) : kInf;
     uint64_t key = valid ? make_key(dsq, s.perm ? s.perm[c] : c) : kNoKey;
     key = segment_sort<KP>(key, lane);
 
@@ -479,50 +529,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     }
     if (!row_ok) return;
 
-    // Certificate: every reference outside the kept list has approximate score
-    // >= theta, so its fp32 squared distance is >= `lower`; if that clears the band's
-    // upper edge and every snapping threshold, the kept list holds every reference the
-    // oracle could place in the first k (or in the tie band).
-    // A list shorter than KP holds every reference appended over the whole scan; it is
-    // complete only if that is every other cell (scores that overflowed are never appended).
-    // A dataset past the pass's norm limit skipped the pass: every row is listed.
-    const float max_norm = __uint_as_float(*s.max_norm_bits);
-    bool certified = full || n_valid == static_cast<int>(n) - 1;
-    if (!(max_norm <= kPassNormLimit)) certified = false;
-    if (full && certified) {
-        const double u = 0x1p-24;
-        const double nqd = static_cast<double>(nq);
-        const double nmax = static_cast<double>(max_norm);
-        const double gamma = d * u / (1.0 - d * u);
-        const double qn = __builtin_sqrt(nqd * (1.0 + gamma));
-        const double rn = __builtin_sqrt(nmax * (1.0 + gamma));
-        // |score + |a|^2 - exact d^2| <= err: f32 norms, the bf16 split of the
-        // coordinates and of |b|^2, and the MFMA accumulation over every packed term.
-        const double err = 1.25 * (gamma * (nqd + nmax) * 1.01 + 2.0 * s.split_rel * qn * rn +
-                                   2.0 * s.acc_rel * (1.02 * qn * rn + 0.51 * nmax * (1.0 + gamma)) +
-                                   s.norm_rel * nmax * (1.0 + gamma)) + 1e-30;
-        const double cd = (d + 3) * u * 1.01;
-        double lower = (static_cast<double>(s.theta[p]) + nqd - err) * (1.0 - cd);
-        if (s.local) {
-            // Local-origin pass, in units of sigma (local_scale): a reference not kept has
-            // pass score >= theta, and the score is a lower bound of |b''|^2 - 2 a''.b'' up
-            // to the per-row term 2 kq (fp16 rounding below the normal range), so
-            // |a'' - b''|^2 >= theta + |a''|^2 - 2 kq. Then a' = fl(a - o) differs from a - o
-            // by at most u |a - o| per coordinate: |a - b| >= |a' - b'| (1 - 2.02 u) - 2.02 u |a'|.
-            const double sigma = static_cast<double>(local_scale(max_norm));
-            const double an2 = static_cast<double>(s.anorm[p]);
-            const double an_up = __builtin_sqrt(an2 / (1.0 - 0x1p-15) * (1.0 + 0x1p-16)) * (1.0 + 0x1p-40);
-            const double kq = static_cast<double>(s.local_c2) * an_up + d * 0x1p-50;
-            const double base = static_cast<double>(s.theta[p]) + an2 - 2.0 * kq;
-            double len = base > 0.0 ? __builtin_sqrt(base) / sigma : 0.0;
-            len = len * (1.0 - 2.02 * u) - 2.02 * u * (an_up / sigma);
-            lower = len > 0.0 ? len * len * (1.0 - cd) : 0.0;
-        }
-        const double snap_max = (d + 2) * 0x1p-23 * (nqd + nmax) * (1.0 + 4.0 * u);
-        const double need = (t + tau) > snap_max ? (t + tau) : snap_max;
-        certified = lower > need;
-    }
-
+    const bool certified = row_certified(s, p, nq, d, t, tau, full, n_valid);
     if (certified) {
         if (pos < k && j < n_valid) {
             s.out_indices[static_cast<size_t>(q) * k + pos] = my_id;
@@ -536,16 +543,117 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     }
 }
 
-void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp32,
-                   bool has_kp64, uint32_t* fail_list, uint32_t* fail_count,
-                   hipStream_t stream) {
+// Rerank of rows with KP = 128 candidates: one wave per row, two keys per lane (sorted key
+// e = i*64 + lane), otherwise as rerank_kernel. A tie band here can exceed kBandCap members;
+// such a band keeps the (d^2, id) order, as the oracle's rule has it.
+__global__ __launch_bounds__(256) void rerank_wide_kernel(const DatasetDesc* __restrict__ sets,
+                                                          uint32_t count, uint32_t total_rows,
+                                                          int metric, uint32_t* __restrict__ fail_list,
+                                                          uint32_t* __restrict__ fail_count) {
+    constexpr int KP = 128;
+    const int lane = threadIdx.x & 63;
+    const uint32_t flat = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (flat >= total_rows) return;
+    const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
+    if (!s.on_pass || s.kp != KP) return;
+    const uint32_t p = flat - s.row_begin;
+    const uint32_t n = s.n;
+    const uint32_t q = s.perm ? s.perm[p] : p;
+    const int d = static_cast<int>(s.d);
+    const float* xq = s.coords + static_cast<size_t>(p) * d;
+    const float nq = s.norms[p];
+
+    uint64_t key[2];
+    bool full = true;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const uint64_t raw = s.cand[static_cast<size_t>(p) * KP + i * kWave + lane];
+        const bool present = raw != kNoKey;
+        full = full && __ballot(present) == ~0ull;
+        const uint32_t c = key_id(raw);
+        const bool valid = present && c != p && c < n;
+        const float dsq = valid ? rerank_square(s, xq, nq, c, d) : kInf;
+        key[i] = valid ? make_key(dsq, s.perm ? s.perm[c] : c) : kNoKey;
+    }
+    wave_sort<2>(key, lane);
+    const int n_valid = __popcll(__ballot(key[0] != kNoKey)) + __popcll(__ballot(key[1] != kNoKey));
+    const float my_d[2] = {key_value(key[0]), key_value(key[1])};
+    const uint32_t my_id[2] = {key_id(key[0]), key_id(key[1])};
+    // Distance and id of sorted key e (e wave-uniform).
+    auto dist_at = [&](int e) { return e < kWave ? __shfl(my_d[0], e, kWave) : __shfl(my_d[1], e - kWave, kWave); };
+    auto id_at = [&](int e) {
+        return static_cast<uint32_t>(e < kWave ? __shfl(static_cast<int>(my_id[0]), e, kWave)
+                                               : __shfl(static_cast<int>(my_id[1]), e - kWave, kWave));
+    };
+    const int k = static_cast<int>(s.k);
+    const double t = static_cast<double>(dist_at(k - 1));
+    const double tau = t * static_cast<double>(d + 2) * 0x1p-23;
+    const bool tie = n_valid > k && (static_cast<double>(dist_at(k)) - t <= tau);
+
+    int pos[2] = {lane, kWave + lane};
+    if (tie) {
+        bool in_band[2];
+        uint64_t band[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            in_band[i] = i * kWave + lane < n_valid && __builtin_fabs(static_cast<double>(my_d[i]) - t) <= tau;
+            band[i] = __ballot(in_band[i]);
+        }
+        const int len = __popcll(band[0]) + __popcll(band[1]);
+        if (len <= kBandCap) {
+            const int lo = band[0] ? __builtin_ctzll(band[0]) : kWave + __builtin_ctzll(band[1]);
+            const int hi = band[1] ? kWave + 63 - __builtin_clzll(band[1]) : 63 - __builtin_clzll(band[0]);
+            double d64[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                d64[i] = 0.0;
+                if (in_band[i] && my_d[i] != 0.0f) {
+                    const uint32_t mp = s.inv_perm ? s.inv_perm[my_id[i]] : my_id[i];
+                    d64[i] = direct_square_f64(xq, s.coords + static_cast<size_t>(mp) * d, d);
+                }
+            }
+            int rank[2] = {0, 0};
+            for (int e = lo; e <= hi; e++) {
+                const double od = e < kWave ? __shfl(d64[0], e, kWave) : __shfl(d64[1], e - kWave, kWave);
+                const uint32_t oid = id_at(e);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    if (od < d64[i] || (od == d64[i] && oid < my_id[i])) rank[i]++;
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                if (in_band[i]) pos[i] = lo + rank[i];
+        }
+    }
+    if (row_certified(s, p, nq, d, t, tau, full, n_valid)) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            if (pos[i] < k && i * kWave + lane < n_valid) {
+                s.out_indices[static_cast<size_t>(q) * k + pos[i]] = my_id[i];
+                s.out_distances[static_cast<size_t>(q) * k + pos[i]] =
+                    metric == kMetricCosine ? 0.5f * my_d[i] : __builtin_sqrtf(my_d[i]);
+            }
+        }
+    } else if (lane == 0) {
+        const uint32_t slot = atomicAdd(fail_count, 1u);
+        fail_list[slot] = s.row_begin + q;
+    }
+}
+
+void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, unsigned kps,
+                   uint32_t* fail_list, uint32_t* fail_count, hipStream_t stream) {
     if (shape.total_rows == 0) return;
-    if (has_kp32) {
+    if (kps & 128u) {
+        const unsigned blocks = (shape.total_rows + 3) / 4;
+        hipLaunchKernelGGL(rerank_wide_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+                           shape.total_rows, shape.metric, fail_list, fail_count);
+    }
+    if (kps & 32u) {
         const unsigned blocks = (shape.total_rows + 7) / 8;
         hipLaunchKernelGGL(rerank_kernel<32>, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
                            shape.total_rows, shape.metric, fail_list, fail_count);
     }
-    if (has_kp64) {
+    if (kps & 64u) {
         const unsigned blocks = (shape.total_rows + 3) / 4;
         hipLaunchKernelGGL(rerank_kernel<64>, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
                            shape.total_rows, shape.metric, fail_list, fail_count);
@@ -582,7 +690,7 @@ __device__ void band_order(const DatasetDesc& s, const uint64_t* keys, int m, in
         while (hi + 1 < m && hi - lo + 1 <= kBandCap && static_cast<double>(key_value(keys[hi + 1])) - t <= tau)
             hi++;
         // A band wider than kBandCap keeps the (d^2, id) order. With keys dropped, m is at
-        // least k + kBandCap (kExactKeep, or kSplitKeep with k <= 47), so a band that reaches
+        // least k + kBandCap (kExactKeep, or kSplitKeep with k <= kMaxSplitK), so a band that reaches
         // the end of the kept keys is always wider than the cap.
         const int len = hi - lo + 1;
         if (len <= kBandCap && hi + 1 == m && dropped) {
@@ -657,7 +765,7 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t flat = list ? list[item] : item;
         const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
-        if (split && s.on_pass) continue;  // exact_split_kernel's
+        if (split && s.on_pass && s.k <= kMaxSplitK) continue;  // exact_split_kernel's
         const uint32_t q = flat - s.row_begin;
         const int d = static_cast<int>(s.d), k = static_cast<int>(s.k);
         const uint32_t n = s.n;
@@ -753,7 +861,7 @@ __global__ __launch_bounds__(256) void exact_split_kernel(const DatasetDesc* __r
         const uint32_t i = item / ranges_max, j = item - i * ranges_max;
         const uint32_t flat = list[i];
         const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
-        if (!s.on_pass) continue;
+        if (!s.on_pass || s.k > kMaxSplitK) continue;
         const uint32_t n = s.n, q = flat - s.row_begin;
         const uint32_t pq = s.inv_perm ? s.inv_perm[q] : q;
         const int d = static_cast<int>(s.d);
@@ -795,15 +903,15 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(const DatasetDesc* __r
     __shared__ double bd_all[4][kBandCap];
     __shared__ uint32_t bid_all[4][kBandCap];
     __shared__ float bf_all[4][kBandCap];
-    __shared__ uint32_t id_all[4][kMaxPassK];
-    __shared__ float dist_all[4][kMaxPassK];
+    __shared__ uint32_t id_all[4][kMaxSplitK];
+    __shared__ float dist_all[4][kMaxSplitK];
     const uint32_t total = *list_count;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t i = blockIdx.x * 4 + w;
     if (total > kSplitRowsMax || i >= total) return;
     const uint32_t flat = list[i];
     const DatasetDesc& s = sets[find_set(sets, count, flat, false)];
-    if (!s.on_pass) return;
+    if (!s.on_pass || s.k > kMaxSplitK) return;
     uint64_t* stage = stage_all[w];
     const uint32_t n = s.n, q = flat - s.row_begin, k = s.k;
     const uint32_t pq = s.inv_perm ? s.inv_perm[q] : q;

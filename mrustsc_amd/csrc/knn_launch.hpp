@@ -37,11 +37,13 @@ constexpr unsigned kFlagBandOverflow = 2u;
 
 constexpr int kCapMax = 512;  // largest candidate buffer (slots per query row) of the pass
 // Widest boundary tie band resolved in fp64; wider bands keep the (d^2, id) order.
-// Every rerank band has at most KP - 1 <= 63 members, so the cap only binds on the
-// exact path.
 constexpr int kBandCap = 64;
 constexpr int kMaxK = 256;      // largest k any path serves
-constexpr int kMaxPassK = 47;   // largest k the MFMA candidate pass serves (KP = 64)
+// Candidates kept per row (KP) by the MFMA pass: 32 for k <= 15, 64 for k <= 47, 128 for
+// k <= 111 (k + 17 <= KP: the list holds the first k and a margin for the boundary band).
+constexpr int kMaxPassK = 111;  // largest k the MFMA candidate pass serves (KP = 128)
+constexpr int kMaxLocalK = 47;  // largest k of the local-origin fp16 pass (KP = 64)
+__host__ __device__ inline uint32_t pass_kp(uint32_t k) { return k + 17 <= 32 ? 32u : k + 17 <= 64 ? 64u : 128u; }
 constexpr int kMaxPassD = 128;  // largest d with a compiled MFMA variant
 constexpr int kAccWordsHost = 10;
 // Largest squared norm the MFMA candidate pass serves. Past it (coordinates beyond about
@@ -52,11 +54,13 @@ constexpr float kPassNormLimit = 0x1p100f;
 constexpr int kPassRows = 256;  // query rows per pass work item (4 waves x 64)
 constexpr int kTileRefsHost = 64;  // references per pass tile
 // Split scan of a short list of exact rows: at most kSplitRowsMax rows, each cut into ranges
-// of kSplitRange references, each range keeping its kSplitKeep smallest keys (k + kBandCap
-// <= 47 + 64 fit, so the merged keys hold the first k and any tie band the cap allows).
+// of kSplitRange references, each range keeping its kSplitKeep smallest keys. Rows with
+// k <= kMaxSplitK take it (k + kBandCap <= kSplitKeep, so the merged keys hold the first k
+// and any tie band the cap allows); rows with a larger k take the exact kernel.
 constexpr uint32_t kSplitRowsMax = 256;
 constexpr uint32_t kSplitRange = 8192;
 constexpr uint32_t kSplitKeep = 128;
+constexpr uint32_t kMaxSplitK = kSplitKeep - kBandCap;
 
 // Device-visible description of one embedding and its workspace.
 struct DatasetDesc {
@@ -181,10 +185,10 @@ bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t 
 // selection width present; `widths` is a bit set of the select_e values 2, 4, 8, 16).
 void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned widths,
                    hipStream_t stream);
-// Rerank of every pass row (one launch per candidate width present).
-void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, bool has_kp32,
-                   bool has_kp64, uint32_t* fail_list, uint32_t* fail_count,
-                   hipStream_t stream);
+// Rerank of every pass row (one launch per candidate width present; `kps` is a bit set of
+// the KP values 32, 64 and 128).
+void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, unsigned kps,
+                   uint32_t* fail_list, uint32_t* fail_count, hipStream_t stream);
 // Exact rows listed in `list` (flat rows, count on the device). With `partial` (the split
 // scan's workspace, kSplitRowsMax x ranges_max x kSplitKeep keys), a list of at most
 // kSplitRowsMax rows has its pass datasets' rows scanned by many waves per row instead.

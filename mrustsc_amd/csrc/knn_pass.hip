@@ -864,7 +864,8 @@ __global__ __launch_bounds__(256) void select_kernel(const DatasetDesc* __restri
             if (key[i] != kNoKey) out[base + lanes_below(m[i])] = key[i];
             base += static_cast<uint32_t>(__popcll(m[i]));
         }
-        if (static_cast<uint32_t>(lane) >= base && lane < kp) out[lane] = kNoKey;
+        for (uint32_t slot = base + static_cast<uint32_t>(lane); slot < static_cast<uint32_t>(kp); slot += kWave)
+            out[slot] = kNoKey;
         if (lane == 0) global_ptr(s.theta)[row] = kInf;
     }
 }

@@ -96,7 +96,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         // kOrderMinCells cells or asked for. Anything else takes the bf16 split pass, which
         // gives the same graph.
         const uint32_t order_knob = force && force->cap ? force->order : (options ? options->order : 0u);
-        const bool local = single && specs[0].k <= static_cast<size_t>(kMaxPassK) &&
+        const bool local = single && specs[0].k <= static_cast<size_t>(kMaxLocalK) &&
                            local_chunks(static_cast<int>(specs[0].d)) > 0 &&
                            (order_knob == 2u || (order_knob == 0u && specs[0].n >= kOrderMinCells));
         plan.operand = local ? kOperandF16Local : kOperandBf16;
@@ -142,7 +142,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         if (pass) {
             // The local-origin pass's bound is looser for rows far from their work item's
             // origin: it keeps 64 candidates so those rows still certify.
-            s.kp = sp.k + 17 <= 32 && plan.operand != kOperandF16Local ? 32u : 64u;
+            s.kp = plan.operand == kOperandF16Local ? 64u : pass_kp(s.k);
             // A row's buffer is four lane segments of cap/4 slots; a compaction leaves kp/4
             // keys in each and a tile appends at most 16 per segment.
             if (s.cap < 4 * s.kp) s.cap = 4 * s.kp;
@@ -186,8 +186,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             }
             plan.tuned |= have_tuned && !force;
             plan.any_pass = true;
-            plan.kp32 |= s.kp == 32;
-            plan.kp64 |= s.kp == 64;
+            plan.kps |= s.kp;
             plan.nch_max = std::max(plan.nch_max, static_cast<int>(s.nch));
             plan.pass_flops += 2ull * sp.n * sp.n * sp.d;
         } else {
@@ -477,7 +476,7 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     if (plan.any_pass) {
         launch_select(plan.d_sets, plan.shape, plan.select_widths, stream);
         if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch candidate selection");
-        launch_rerank(plan.d_sets, plan.shape, plan.kp32, plan.kp64, plan.d_list,
+        launch_rerank(plan.d_sets, plan.shape, plan.kps, plan.d_list,
                       plan.d_scalars + kScalarListCount, stream);
         if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch rerank");
     }

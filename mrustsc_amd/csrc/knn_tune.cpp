@@ -284,7 +284,7 @@ scrust_hip_status scrust_hip_tune_load(const char* path) {
         const int got = std::sscanf(p, "%u %u %u %u %u %u %u %u %f %n", &key.precision, &key.metric, &key.kp,
                                     &key.nch, &key.nbin, &e.cap, &e.splits, &e.order, &ms, &used);
         const bool valid = got == 9 && p[used] == '\0' && key.precision <= SCRUST_HIP_FP32 &&
-                           key.metric <= SCRUST_HIP_COSINE && (key.kp == 32 || key.kp == 64) &&
+                           key.metric <= SCRUST_HIP_COSINE && (key.kp == 32 || key.kp == 64 || key.kp == 128) &&
                            key.nch >= 1 && key.nch <= 1024 && key.nbin <= 64 &&
                            (e.cap == 128 || e.cap == 256 || e.cap == 512) && e.cap >= 4 * key.kp &&
                            e.splits >= 1 && e.splits * e.cap <= 1024 && (e.order == 1 || e.order == 2) &&

@@ -77,7 +77,7 @@ struct Plan {
     int operand = kOperandNone;
     int precision = SCRUST_HIP_BF16;
     int nch_max = 0;
-    bool kp32 = false, kp64 = false;
+    unsigned kps = 0;  // bit set of the KP values present (32, 64, 128)
     unsigned select_widths = 0;  // select_e values present (bit set)
     size_t count_words = 0;      // segment counts of all datasets (u32 words)
     size_t counts_offset = 0;    // their arena offset

@@ -5,6 +5,8 @@
   c3_cosine_oracle_fixture.json  the same embedding, cosine metric
   c4_oracle_fixture.json         the 64 datasets of synthetic.batch_specs(64, seed=7), each
                                  at its own k
+  c2_large_k_oracle_fixture.json the C2 embedding at k = 64, 100, 111 and 128 (the 128-candidate
+                                 pass and, at k = 128, the exact kernel)
 
 Each graph is the oracle's (oracle.knn_fast, the vectorised oracle_knn with the fp64 tie rule)
 and is stored as SHA-256 digests of the raw arrays (ids uint32, distances float32), digests of
@@ -12,7 +14,7 @@ every 65,536-row chunk (so a mismatch names its rows), and a few rows in full (d
 float32 bit patterns). The input's digest is stored too: a test first checks that the
 generator reproduced the embedding. About 25 minutes on 8 cores (each 1M graph ~10 min).
 
-    python tests/golden/make_config_fixtures.py [c2 c3 c3_cosine c4]
+    python tests/golden/make_config_fixtures.py [c2 c3 c3_cosine c4 c2_large_k]
 """
 
 from __future__ import annotations
@@ -83,6 +85,19 @@ def batch() -> None:
     print(f"c4: {time.time() - t0:.0f} s", flush=True)
 
 
+LARGE_K = (64, 100, 111, 128)
+
+
+def large_k() -> None:
+    t0 = time.time()
+    x = synthetic.clustered(100_000, 50, seed=1)
+    out = {"_source": "oracle Euclidean graphs of mrustsc_amd.synthetic.clustered(100000, 50, seed=1) at "
+                      "k = 64, 100, 111, 128; written by tests/golden/make_config_fixtures.py",
+           "graphs": {str(k): graph_record(x, k, "euclidean", 8) for k in LARGE_K}}
+    (HERE / "c2_large_k_oracle_fixture.json").write_text(json.dumps(out, indent=1) + "\n")
+    print(f"c2_large_k: {time.time() - t0:.0f} s", flush=True)
+
+
 def main(argv: list[str]) -> None:
     todo = argv or ["c2", "c4", "c3", "c3_cosine"]
     for name in todo:
@@ -94,6 +109,8 @@ def main(argv: list[str]) -> None:
             single("c3_cosine", 1_000_000, "cosine")
         elif name == "c4":
             batch()
+        elif name == "c2_large_k":
+            large_k()
         else:
             raise SystemExit(f"unknown fixture {name}")
 

@@ -22,8 +22,9 @@ SPECS = [  # (n, d, k, seed)
     (16, 5, 15, 4),      # n = k + 1
     (1200, 50, 30, 5),
     (257, 30, 15, 6),
-    (900, 20, 60, 7),    # k > 47: exact path inside the batch
+    (900, 20, 60, 7),    # 47 < k <= 111: 128 candidates inside the batch
     (2048, 100, 30, 8),
+    (600, 20, 120, 9),   # k > 111: exact path inside the batch
 ]
 
 

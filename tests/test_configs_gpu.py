@@ -6,7 +6,8 @@ every 65,536-row chunk and a few rows in full. Each test first checks that the g
 reproduced the workload's embedding (its digest), then that the HIP path's whole graph equals
 the oracle's bit for bit:
 
-  C2  100,000 x 50, k = 15: the bf16 and the fp32 pass
+  C2  100,000 x 50, k = 15: the bf16 and the fp32 pass; k = 64, 100 and 111 on the
+      128-candidate pass (bf16 and fp32), k = 128 on the exact kernel
   C3  1,000,000 x 50, k = 15: bf16 Euclidean and cosine, fp32 Euclidean
   C4  the 64 datasets of the batched workload in one graph launch: every dataset's graph
       equals the oracle's and its own single call
@@ -90,6 +91,22 @@ def test_c2_whole_graph_equals_the_oracle(context, precision):
     ids, dist, st = device_build(x, 15, context, precision, "euclidean")
     print(st, flush=True)
     check_against(fix, ids, dist, f"C2 {precision}")
+
+
+@pytest.mark.parametrize("k,precision", [(64, "bf16"), (64, "fp32"), (100, "bf16"), (100, "fp32"),
+                                         (111, "bf16"), (128, "bf16")])
+def test_c2_large_k_whole_graph_equals_the_oracle(context, k, precision):
+    fix = _fixture("c2_large_k")["graphs"][str(k)]
+    x = embedding(fix, 100_000)
+    print(f"[case] C2 100000 x 50 k={k} {precision}", flush=True)
+    ids, dist, st = device_build(x, k, context, precision, "euclidean")
+    print(st, flush=True)
+    if k <= 111:
+        assert st["path"] == precision and st["candidates"] == 128
+        assert st["fallback_rows"] < 100
+    else:
+        assert st["path"] == "exact"
+    check_against(fix, ids, dist, f"C2 k={k} {precision}")
 
 
 @pytest.fixture(scope="module")

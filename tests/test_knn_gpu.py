@@ -254,10 +254,11 @@ def test_boundary_near_ties_are_resolved_in_fp64(context, precision):
 
 @pytest.mark.parametrize("precision", ("bf16", "fp32"))
 @pytest.mark.parametrize("tile", [128, 256, 512])
-@pytest.mark.parametrize("k", [5, 15, 30, 47])
+@pytest.mark.parametrize("k", [5, 15, 30, 47, 48, 100, 111])
 def test_every_candidate_buffer_size_matches_the_oracle(context, precision, tile, k):
     # The buffer size changes when rows are compacted, never the result: each size, with
-    # KP = 32 (k <= 15) and KP = 64 (k <= 47, where 128 slots are raised to 256).
+    # KP = 32 (k <= 15), KP = 64 (k <= 47, where 128 slots are raised to 256) and KP = 128
+    # (k <= 111, where every size is raised to 512).
     x = synthetic.clustered(2500, 24, seed=k + tile)
     print(f"[case] tile={tile} k={k} precision={precision}", flush=True)
     ids, dist = mrustsc_amd.knn_hip(x, k, precision=precision, tile=tile, context=context)
@@ -376,7 +377,7 @@ def test_last_round_blocks_scan_in_ranges_and_match_the_oracle(context, tmp_path
 
 def test_random_shapes_options_and_data_match_the_oracle(context):
     # 200 seeded cases across n 20-30,000, d 1-129 (both sides of every chunk boundary and of
-    # the pass's d <= 128 limit), k 1-64 (both sides of KP = 32 / 64 and of the pass's k <= 47),
+    # the pass's d <= 128 limit), k 1-64 (both sides of KP = 32 / 64 / 128),
     # both metrics, all three paths, every row order and buffer size, on clustered,
     # integer-lattice (ties) and scaled-normal data; up to 300 rows checked per case.
     rng = np.random.default_rng(2024)
@@ -400,6 +401,53 @@ def test_random_shapes_options_and_data_match_the_oracle(context):
         ids, dist = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, order=order, tile=tile,
                                         context=context)
         q = np.arange(n) if n <= 3000 else rng.choice(n, 300, replace=False)
+        want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
+        np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
+        np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)
+
+
+def test_short_lists_of_the_128_candidate_pass_match_the_oracle(context):
+    # n - 1 < 128: every other cell is a candidate and the list's tail is empty. A larger
+    # build first leaves the context's candidate slots full of its keys.
+    big = synthetic.clustered(3000, 20, seed=9)
+    mrustsc_amd.knn_hip(big, 100, context=context)
+    for n, k in [(50, 49), (65, 48), (100, 60), (111, 48), (112, 111), (128, 100), (129, 111)]:
+        for metric in ("euclidean", "cosine"):
+            for precision in ("bf16", "fp32"):
+                x = np.random.default_rng(n + k).normal(size=(n, 5)).astype(np.float32)
+                tag = f"n={n} k={k} {metric} {precision}"
+                print(f"[case] {tag}", flush=True)
+                ids, dist = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, context=context)
+                want_ids, want_dist = oracle.knn(x, k, metric=metric)
+                np.testing.assert_array_equal(ids, want_ids, err_msg=tag)
+                np.testing.assert_array_equal(dist, want_dist, err_msg=tag)
+
+
+def test_random_large_k_cases_match_the_oracle(context):
+    # 40 seeded cases with k 48-128 (the 128-candidate pass up to k = 111, the exact kernel
+    # above), n 130-20,000, both metrics and pass precisions, every row order, on clustered,
+    # lattice (ties, wide tie bands) and scaled-normal data.
+    rng = np.random.default_rng(4096)
+    for case in range(40):
+        n = int(np.exp(rng.uniform(np.log(130), np.log(20000))))
+        d = int(rng.choice([2, 3, 8, 20, 31, 50, 64, 100, 128]))
+        k = int(rng.choice([48, 63, 64, 65, 80, 100, 110, 111, 112, 128]))
+        metric = str(rng.choice(["euclidean", "cosine"]))
+        precision = str(rng.choice(["bf16", "fp32"]))
+        order = str(rng.choice(["auto", "input", "grouped"]))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            x = synthetic.clustered(n, d, seed=500 + case, n_clusters=int(rng.integers(2, 12)))
+        elif kind == 1:
+            x = rng.integers(-2, 3, size=(n, d)).astype(np.float32)
+        else:
+            x = rng.normal(size=(n, d)).astype(np.float32) * np.float32(rng.choice([1e-3, 1.0, 1e3]))
+        tag = f"case {case}: n={n} d={d} k={k} {metric} {precision} order={order} data={kind}"
+        print(f"[case] {tag}", flush=True)
+        ids, dist, st = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, order=order,
+                                            context=context, stats=True)
+        assert st["candidates"] == (128 if k <= 111 else 0), tag
+        q = np.arange(n) if n <= 2000 else rng.choice(n, 300, replace=False)
         want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
         np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
         np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)

@@ -353,6 +353,22 @@ def test_workload_fixtures_hold_the_scalar_oracles_rows(name, n, metric):
         assert [f"{b:08x}" for b in dist[i].view(np.uint32)] == row["dist_bits"]
 
 
+def test_large_k_fixture_holds_the_scalar_oracles_rows():
+    # The C2 embedding at k = 64, 100, 111 and 128: rows in full against the scalar oracle.
+    import hashlib
+
+    fix = json.loads((Path(__file__).parent / "golden" / "c2_large_k_oracle_fixture.json").read_text())
+    x = synthetic.clustered(100_000, 50, seed=1)
+    assert sorted(int(k) for k in fix["graphs"]) == [64, 100, 111, 128]
+    for k, rec in fix["graphs"].items():
+        assert rec["k"] == int(k) and hashlib.sha256(x.tobytes()).hexdigest() == rec["x_sha256"]
+        rows = np.array(sorted(int(r) for r in rec["rows"]), dtype=np.uint32)
+        ids, dist = oracle.knn(x, int(k), queries=rows)
+        for i, r in enumerate(rows):
+            assert ids[i].tolist() == rec["rows"][str(r)]["ids"], (k, r)
+            assert [f"{b:08x}" for b in dist[i].view(np.uint32)] == rec["rows"][str(r)]["dist_bits"], (k, r)
+
+
 def test_batch_fixture_holds_the_scalar_oracles_rows():
     import hashlib
 
