@@ -816,11 +816,13 @@ __global__ __launch_bounds__(256) void exact_kernel(const DatasetDesc* __restric
 
 // Split scan of a short exact list (pass datasets' rows): wave item = (list entry i, range j)
 // over kSplitRange references, keeping its kSplitKeep smallest keys in `partial`. A wave
-// holds its kept keys two per lane and takes new keys fourteen per lane at a time; when more
-// than kSplitKeep are held it keeps the kSplitKeep smallest (select_keys, through LDS) and
-// drops later keys scoring above the kept maximum. Keys are (f32 direct-difference d^2,
-// input id), as in exact_kernel.
+// holds its kept keys kSplitHeld per lane and takes kSplitE - kSplitHeld new keys per lane at
+// a time; when more than kSplitKeep are held it keeps the kSplitKeep smallest (select_keys,
+// through LDS) and drops later keys scoring above the kept maximum. Keys are (f32
+// direct-difference d^2, input id), as in exact_kernel.
 constexpr int kSplitE = 16;
+constexpr int kSplitHeld = static_cast<int>(kSplitKeep) / kWave;
+static_assert(kSplitHeld * kWave == static_cast<int>(kSplitKeep) && kSplitHeld < kSplitE, "split scan key layout");
 
 __device__ void split_keep(uint64_t (&key)[kSplitE], float& cut, int lane, uint64_t* stage) {
     uint32_t total = 0;
@@ -840,10 +842,8 @@ __device__ void split_keep(uint64_t (&key)[kSplitE], float& cut, int lane, uint6
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    key[0] = stage[lane];
-    key[1] = stage[kWave + lane];
 #pragma unroll
-    for (int e = 2; e < kSplitE; e++) key[e] = kNoKey;
+    for (int e = 0; e < kSplitE; e++) key[e] = e < kSplitHeld ? stage[e * kWave + lane] : kNoKey;
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -873,10 +873,10 @@ __global__ __launch_bounds__(256) void exact_split_kernel(const DatasetDesc* __r
         float cut = kInf;
         const uint32_t r0 = j * kSplitRange;
         const uint32_t r1 = r0 < n ? (n - r0 < kSplitRange ? n : r0 + kSplitRange) : r0;
-        for (uint32_t b = r0; b < r1; b += (kSplitE - 2) * kWave) {
+        for (uint32_t b = r0; b < r1; b += (kSplitE - kSplitHeld) * kWave) {
 #pragma unroll
-            for (int e = 2; e < kSplitE; e++) {
-                const uint32_t r = b + static_cast<uint32_t>(e - 2) * kWave + static_cast<uint32_t>(lane);
+            for (int e = kSplitHeld; e < kSplitE; e++) {
+                const uint32_t r = b + static_cast<uint32_t>(e - kSplitHeld) * kWave + static_cast<uint32_t>(lane);
                 uint64_t kk = kNoKey;
                 if (r < r1 && r != pq) {
                     const float v = direct_square(xq, s.coords + static_cast<size_t>(r) * d, d, nq, s.norms[r]);
@@ -887,8 +887,8 @@ __global__ __launch_bounds__(256) void exact_split_kernel(const DatasetDesc* __r
             split_keep(key, cut, lane, stage);
         }
         uint64_t* out = partial + static_cast<size_t>(item) * kSplitKeep;
-        out[lane] = key[0];
-        out[kWave + lane] = key[1];
+#pragma unroll
+        for (int e = 0; e < kSplitHeld; e++) out[e * kWave + lane] = key[e];
     }
 }
 
@@ -922,20 +922,22 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(const DatasetDesc* __r
 #pragma unroll
     for (int e = 0; e < kSplitE; e++) key[e] = kNoKey;
     float cut = kInf;
-    for (uint32_t b = 0; b < keys_in; b += (kSplitE - 2) * kWave) {
+    for (uint32_t b = 0; b < keys_in; b += (kSplitE - kSplitHeld) * kWave) {
 #pragma unroll
-        for (int e = 2; e < kSplitE; e++) {
-            const uint32_t p = b + static_cast<uint32_t>(e - 2) * kWave + static_cast<uint32_t>(lane);
+        for (int e = kSplitHeld; e < kSplitE; e++) {
+            const uint32_t p = b + static_cast<uint32_t>(e - kSplitHeld) * kWave + static_cast<uint32_t>(lane);
             const uint64_t kk = p < keys_in ? src[p] : kNoKey;
             key[e] = (kk != kNoKey && !(key_value(kk) > cut)) ? kk : kNoKey;
         }
         split_keep(key, cut, lane, stage);
     }
-    // Sort the kept keys (two per lane) and finish on lane 0.
-    uint64_t two[2] = {key[0], key[1]};
-    wave_sort<2>(two, lane);
-    stage[lane] = two[0];
-    stage[kWave + lane] = two[1];
+    // Sort the kept keys (kSplitHeld per lane) and finish on lane 0.
+    uint64_t held[kSplitHeld];
+#pragma unroll
+    for (int e = 0; e < kSplitHeld; e++) held[e] = key[e];
+    wave_sort<kSplitHeld>(held, lane);
+#pragma unroll
+    for (int e = 0; e < kSplitHeld; e++) stage[e * kWave + lane] = held[e];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {

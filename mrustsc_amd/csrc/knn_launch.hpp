@@ -59,7 +59,7 @@ constexpr int kTileRefsHost = 64;  // references per pass tile
 // and any tie band the cap allows); rows with a larger k take the exact kernel.
 constexpr uint32_t kSplitRowsMax = 256;
 constexpr uint32_t kSplitRange = 8192;
-constexpr uint32_t kSplitKeep = 128;
+constexpr uint32_t kSplitKeep = 256;
 constexpr uint32_t kMaxSplitK = kSplitKeep - kBandCap;
 
 // Device-visible description of one embedding and its workspace.
