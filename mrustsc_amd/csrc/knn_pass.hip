@@ -714,6 +714,17 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             static_cast<uint8_t>(cnt[qb]);
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD b mod 8), and each
+// XCD has its own L2. Maps workgroup b < count to a block so that each XCD takes a contiguous
+// run of blocks in order: blocks of one group, which start their scans at the same tiles,
+// then run on the same XCD and share its L2 while they are resident together.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t count) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t x = b % kXcds, j = b / kXcds;
+    const uint32_t base = count / kXcds, extra = count % kXcds;
+    return x * base + (x < extra ? x : extra) + j;
+}
+
 // At most two waves per SIMD: the allocator then keeps the tile prefetch in registers
 // instead of spilling it to reach a higher occupancy the LDS budget would allow.
 template <int OP, int NCH>
@@ -730,7 +741,7 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
     item.set = 0;
     if (blockIdx.x < tail_block) {
         item.split = 0;
-        item.block = blockIdx.x;
+        item.block = s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x;
         item.tile_begin = 0;
         item.tile_end = 2 * pairs;
     } else {
@@ -767,7 +778,7 @@ void pass_local_kernel(const DatasetDesc* __restrict__ set) {
     PassItem item;
     item.set = 0;
     item.split = 0;
-    item.block = 2 * blockIdx.x;
+    item.block = 2 * (s.rotate ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
     item.tile_begin = 0;
     item.tile_end = 2 * pairs;
     item.pad = s.rotate ? (4u * (item.block & ~15u)) % (2 * pairs) : 0u;
