@@ -267,8 +267,10 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
         }
         return;
     }
-    // Packed operands (see knn_pass.hip): reference [h | l | h | nh nl nl2 | 0], query
-    // [h | h | l | -1/2 x3 | 0], 32 nch features per row, eight per 16-byte chunk.
+    // Packed operands (see knn_pass.hip), in MFMA K order: reference [h | nh nl nl2 | l | h | 0],
+    // query [h | -1/2 x3 | h | l | 0], 32 nch features per row. K index kappa sits in 16-byte
+    // chunk (kappa % 32) / 8 * nch + kappa / 32 of the row (k_slot): MFMA step c reads chunk
+    // g * nch + c in lane group g, so step c holds K indices 32 c .. 32 c + 31.
     const int chunks = 4 * nch;
     SCRUST_GLOBAL u32x4* rp = global_ptr(reinterpret_cast<u32x4*>(s.pack_ref)) + static_cast<size_t>(r0) * chunks;
     SCRUST_GLOBAL u32x4* qp = global_ptr(reinterpret_cast<u32x4*>(s.pack_query)) + static_cast<size_t>(r0) * chunks;
@@ -288,18 +290,19 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
                 uint16_t rv[2], qv[2];
 #pragma unroll
                 for (int h2 = 0; h2 < 2; h2++) {
-                    const int f = chunk * 8 + e + h2;
+                    const int f = k_index(chunk, e + h2, nch);
                     rv[h2] = 0;
                     qv[h2] = 0;
-                    if (f < 3 * d) {
-                        const int part = (f >= d) + (f >= 2 * d);
-                        const float c = c_row[f - part * d];
+                    if (f < d || (f >= d + 3 && f < 3 * d + 3)) {
+                        // part 0: h.h, part 1: h_query.l_reference, part 2: l_query.h_reference
+                        const int part = f < d ? 0 : (f < 2 * d + 3 ? 1 : 2);
+                        const float c = c_row[part == 0 ? f : f - 3 - (part - 1) * d - d];
                         const uint16_t h = bf16_rn(c);
                         const uint16_t l = bf16_rn(c - bf16_value(h));
                         rv[h2] = part == 1 ? l : h;
                         qv[h2] = part == 2 ? l : h;
-                    } else if (f < 3 * d + 3) {
-                        rv[h2] = f == 3 * d ? nh : (f == 3 * d + 1 ? nl : nl2);
+                    } else if (f < d + 3) {
+                        rv[h2] = f == d ? nh : (f == d + 1 ? nl : nl2);
                         qv[h2] = kBf16MinusHalf;
                     }
                 }
@@ -309,11 +312,11 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
             rp[u] = u32x4{rw[0], rw[1], rw[2], rw[3]};
             qp[u] = u32x4{qw[0], qw[1], qw[2], qw[3]};
         } else {
-            // Sentinel reference row: +inf at the norm feature 3d, zero elsewhere.
+            // Sentinel reference row: +inf at the norm feature (K index d), zero elsewhere.
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
-                const int f = chunk * 8 + e;
-                rw[e >> 1] = (f == 3 * d ? kBf16Inf : 0u) | (f + 1 == 3 * d ? static_cast<uint32_t>(kBf16Inf) << 16 : 0u);
+                const int f = k_index(chunk, e, nch);
+                rw[e >> 1] = (f == d ? kBf16Inf : 0u) | (f + 1 == d ? static_cast<uint32_t>(kBf16Inf) << 16 : 0u);
             }
             rp[u] = u32x4{rw[0], rw[1], rw[2], rw[3]};
         }

@@ -52,6 +52,19 @@ constexpr int kAccWordsHost = 10;
 // exact kernel, whose f32 direct differences are the oracle's arithmetic at any magnitude.
 constexpr float kPassNormLimit = 0x1p100f;
 constexpr int kPassRows = 256;  // query rows per pass work item (4 waves x 64)
+
+// bf16 operand rows hold 32 nch features in MFMA K order: element e of 16-byte chunk m is K
+// index 32 (m % nch) + 8 (m / nch) + e, because MFMA step c reads chunk g * nch + c in lane
+// group g and lane group g supplies K slots 8 g .. 8 g + 7 of the step.
+__host__ __device__ inline int k_index(int chunk, int e, int nch) {
+    return 32 * (chunk % nch) + 8 * (chunk / nch) + e;
+}
+// MFMA steps that hold every non-cross feature of the bf16 operand (h.h and the norm terms,
+// K indices below d + 3) for every d a compiled chunk count serves; the later steps hold
+// only hi/lo cross terms and zeros.
+__host__ __device__ constexpr int bf16_coarse_chunks(int nch) {
+    return nch <= 2 ? 1 : nch <= 5 ? 2 : nch <= 8 ? 3 : nch <= 10 ? 4 : 5;
+}
 constexpr int kTileRefsHost = 64;  // references per pass tile
 // Split scan of a short list of exact rows: at most kSplitRowsMax rows, each cut into ranges
 // of kSplitRange references, each range keeping its kSplitKeep smallest keys. Rows with

@@ -440,9 +440,82 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             }
         }
     };
+    // bf16, two stages (single datasets, NCH > kCoarse): the first kCoarse MFMA steps of a
+    // block give every h.h and norm term (the packing puts them there); the other steps hold
+    // only hi/lo cross terms, whose sum over any subset of features is at most
+    // 2^-7 (1 + 2^-8)^2 |a||b| (|l| <= 2^-8 (1 + 2^-8) |c|, |h| <= (1 + 2^-8) |c|, Cauchy-
+    // Schwarz). A block whose coarse accumulators all lie at least that bound (plus the
+    // rounding of the remaining accumulation steps) below the row's append bound phi cannot
+    // append a pair with its full accumulators either: its remaining steps and its filter are
+    // skipped, and the buffers end exactly as with every step taken.
+    constexpr int kCoarse = bf16_coarse_chunks(NCH);
+    constexpr bool kTwoStage = OP == kOperandBf16 && FIXED && kCoarse < NCH;
+    float skip_b[4];
+    if constexpr (kTwoStage) {
+        const float mx = __uint_as_float(*global_ptr(s.max_norm_bits));
+        const float rb = __builtin_sqrtf(mx * (1.0f + 0x1p-16f));
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+            uint32_t row = qbase + w * kWave + qb * 16 + lc;
+            if (row >= n) row = n - 1;
+            const float qn = __builtin_sqrtf(global_ptr(s.norms)[row] * (1.0f + 0x1p-16f));
+            skip_b[qb] = 0x1p-7f * 1.02f * qn * rb + 0x1p-13f * mx;
+        }
+    }
     auto consume = [&](uint32_t t, int cur) __attribute__((always_inline)) {
         f32x4 acc0[4], acc1[4];
-        if constexpr ((OP == kOperandBf16 || OP == kOperandF16Local) && FIXED) {
+        if constexpr (kTwoStage) {
+            typedef typename Op::Frag Frag;
+            Frag c0[kCoarse], c1[kCoarse], f[NCH - kCoarse];
+            auto load_part = [&](auto& b, int cb, int first, int count) __attribute__((always_inline)) {
+                const u32x4* brow = &tile_of(cur)[(cb * 16 + lc) * stride + g * gstride + first];
+#pragma unroll
+                for (int c = 0; c < count; c++) b[c] = __builtin_bit_cast(Frag, brow[c]);
+            };
+            auto coarse = [&](f32x4 (&acc)[4], const Frag (&b)[kCoarse]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) acc[qb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int c = 0; c < kCoarse; c++)
+#pragma unroll
+                    for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(b[c], a[qb][c], acc[qb]);
+            };
+            // The block's remaining steps and its filter, when some pair may pass.
+            auto refine = [&](f32x4 (&acc)[4], int cb) __attribute__((always_inline)) {
+                uint64_t need = 0;
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) {
+                    const float m = __builtin_fmaxf(__builtin_fmaxf(acc[qb][0], acc[qb][1]),
+                                                    __builtin_fmaxf(acc[qb][2], acc[qb][3]));
+                    need |= __ballot(m > phi[qb] - skip_b[qb]);
+                }
+                if (need) {
+                    load_part(f, cb, kCoarse, NCH - kCoarse);
+#pragma unroll
+                    for (int c = kCoarse; c < NCH; c++)
+#pragma unroll
+                        for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(f[c - kCoarse], a[qb][c], acc[qb]);
+                    filter(acc, t, cb);
+                }
+            };
+            load_part(c0, 0, 0, kCoarse);
+            load_part(c1, 1, 0, kCoarse);
+            __builtin_amdgcn_sched_barrier(0);
+            coarse(acc0, c0);
+            __builtin_amdgcn_sched_barrier(0);
+            load_part(c0, 2, 0, kCoarse);
+            __builtin_amdgcn_sched_barrier(0);
+            coarse(acc1, c1);
+            __builtin_amdgcn_sched_barrier(0);
+            load_part(c1, 3, 0, kCoarse);
+            __builtin_amdgcn_sched_barrier(0);
+            refine(acc0, 0);
+            coarse(acc0, c0);
+            refine(acc1, 1);
+            coarse(acc1, c1);
+            refine(acc0, 2);
+            refine(acc1, 3);
+        } else if constexpr ((OP == kOperandBf16 || OP == kOperandF16Local) && FIXED) {
             // Software pipeline: block cb+1's fragments are read while block cb's MFMAs
             // run, and block cb-1 is filtered while they run.
             // Scheduling barriers keep the compiler from sinking each read to its use.
