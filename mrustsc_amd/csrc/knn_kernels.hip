@@ -270,13 +270,17 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
     // Packed operands (see knn_pass.hip), in MFMA K order: reference [h | nh nl nl2 | l | h | 0],
     // query [h | -1/2 x3 | h | l | 0], 32 nch features per row. K index kappa sits in 16-byte
     // chunk (kappa % 32) / 8 * nch + kappa / 32 of the row (k_slot): MFMA step c reads chunk
-    // g * nch + c in lane group g, so step c holds K indices 32 c .. 32 c + 31.
+    // g * nch + c in lane group g, so step c holds K indices 32 c .. 32 c + 31. Query rows are
+    // stored row by row; reference rows tile by tile, coarse steps first (tile_chunk).
     const int chunks = 4 * nch;
-    SCRUST_GLOBAL u32x4* rp = global_ptr(reinterpret_cast<u32x4*>(s.pack_ref)) + static_cast<size_t>(r0) * chunks;
+    SCRUST_GLOBAL u32x4* rp = global_ptr(reinterpret_cast<u32x4*>(s.pack_ref));
     SCRUST_GLOBAL u32x4* qp = global_ptr(reinterpret_cast<u32x4*>(s.pack_query)) + static_cast<size_t>(r0) * chunks;
     for (uint32_t u = tid; u < prep_rows * chunks; u += 256) {
         const uint32_t row = u / chunks;
         const int chunk = static_cast<int>(u - row * chunks);
+        const uint32_t ref_row = r0 + row;
+        const size_t ref_at = static_cast<size_t>(ref_row / kTileRefsHost) * (kTileRefsHost * chunks) +
+                              tile_chunk(static_cast<int>(ref_row % kTileRefsHost), chunk / nch, chunk % nch, nch);
         uint32_t rw[4], qw[4];
         if (row < rows) {
             const float* c_row = xs + row * d;
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
                 rw[e >> 1] = static_cast<uint32_t>(rv[0]) | static_cast<uint32_t>(rv[1]) << 16;
                 qw[e >> 1] = static_cast<uint32_t>(qv[0]) | static_cast<uint32_t>(qv[1]) << 16;
             }
-            rp[u] = u32x4{rw[0], rw[1], rw[2], rw[3]};
+            rp[ref_at] = u32x4{rw[0], rw[1], rw[2], rw[3]};
             qp[u] = u32x4{qw[0], qw[1], qw[2], qw[3]};
         } else {
             // Sentinel reference row: +inf at the norm feature (K index d), zero elsewhere.
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
                 const int f = k_index(chunk, e, nch);
                 rw[e >> 1] = (f == d ? kBf16Inf : 0u) | (f + 1 == d ? static_cast<uint32_t>(kBf16Inf) << 16 : 0u);
             }
-            rp[u] = u32x4{rw[0], rw[1], rw[2], rw[3]};
+            rp[ref_at] = u32x4{rw[0], rw[1], rw[2], rw[3]};
         }
     }
 }

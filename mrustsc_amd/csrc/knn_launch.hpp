@@ -66,6 +66,32 @@ __host__ __device__ constexpr int bf16_coarse_chunks(int nch) {
     return nch <= 2 ? 1 : nch <= 5 ? 2 : nch <= 8 ? 3 : nch <= 10 ? 4 : 5;
 }
 constexpr int kTileRefsHost = 64;  // references per pass tile
+// bf16 reference operands are stored tile by tile (64 rows, 4 nch 16-byte chunks each): a
+// tile holds the coarse chunks of its rows first (steps c < kc = bf16_coarse_chunks(nch),
+// 4 kc per row, lane group gg's step c at gg kc + c), then their fine chunks (steps c >= kc,
+// 4 kf per row with kf = nch - kc, at gg kf + c - kc). The coarse part of a tile is then one
+// contiguous run that the pass can stream without the fine part. Returns the chunk's index
+// within its tile for row r of the tile.
+__host__ __device__ inline int tile_chunk(int r, int gg, int c, int nch) {
+    const int kc = bf16_coarse_chunks(nch), kf = nch - kc;
+    return c < kc ? r * 4 * kc + gg * kc + c : kTileRefsHost * 4 * kc + r * 4 * kf + gg * kf + (c - kc);
+}
+// The inverse of tile_chunk: row r, lane group gg and step c of chunk u of a tile.
+__host__ __device__ inline void tile_chunk_of(int u, int nch, int& r, int& gg, int& c) {
+    const int kc = bf16_coarse_chunks(nch), kf = nch - kc;
+    if (u < kTileRefsHost * 4 * kc) {
+        r = u / (4 * kc);
+        const int q = u - r * 4 * kc;
+        gg = q / kc;
+        c = q - gg * kc;
+    } else {
+        const int v = u - kTileRefsHost * 4 * kc;
+        r = v / (4 * kf);
+        const int q = v - r * 4 * kf;
+        gg = q / kf;
+        c = kc + (q - gg * kf);
+    }
+}
 // Split scan of a short list of exact rows: at most kSplitRowsMax rows, each cut into ranges
 // of kSplitRange references, each range keeping its kSplitKeep smallest keys. Rows with
 // k <= kMaxSplitK take it (k + kBandCap <= kSplitKeep, so the merged keys hold the first k

@@ -222,11 +222,16 @@ struct PassSmem {
 };
 
 // Three tile buffers for LDS-direct loads, each its own __shared__ array: the compiler tells
-// loads into one buffer apart from reads of another only across distinct objects.
+// loads into one buffer apart from reads of another only across distinct objects. Split
+// tiles (below) have three more for the fine parts and two words of per-wave flags.
 struct DmaTiles {
     u32x4* b0;
     u32x4* b1;
     u32x4* b2;
+    u32x4* f0;
+    u32x4* f1;
+    u32x4* f2;
+    uint32_t* need;
 };
 
 // Slots (16 bytes) of one LDS tile buffer.
@@ -239,6 +244,32 @@ constexpr int kTileSlots = kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max)
 // lead instead of two), so it keeps the register staging.
 template <int OP, int NCH>
 constexpr bool kPassDma = OP == kOperandBf16 && 3 * kTileSlots<OP, NCH> * 16 <= 80 * 1024;
+
+// Split tiles of the two-stage bf16 pass: a tile's coarse part (the first kc MFMA steps of
+// every row, all a block needs unless its coarse scores come near a row's append bound) and
+// its fine part sit in separate LDS buffers, so the scan streams 4 kc chunks per reference row
+// and loads the 4 kf fine chunks of a tile only while its rows' blocks need them. The coarse
+// buffer has no padding: lane group g's step c of row r is at slot
+// r 4 kc + ((g kc + c) ^ coarse_swizzle(r)), which keeps every ds_read_b128 free of bank
+// conflicts (the 32 lanes of a half-wave hit 32 distinct 16-byte slots modulo 32: rows
+// r = 0..15 of a block cover r % (8 / kc) through the row stride and the swizzle separates
+// the rest). The fine buffer is padded like the full tiles (lds_stride).
+template <int KC>
+__host__ __device__ constexpr int coarse_swizzle(int r) {
+    return KC == 2 ? (((r >> 2) & 1) | (((r >> 3) & 1) << 2)) : (((r >> 3) & 1) << 1);
+}
+template <int NCH>
+constexpr int kCoarseSlots = kTileRefs * 4 * bf16_coarse_chunks(NCH);
+template <int NCH>
+constexpr int kFineSlots = kTileRefs * lds_stride(NCH - bf16_coarse_chunks(NCH));
+template <int OP, int NCH>
+constexpr bool kPassSplit = OP == kOperandBf16 && bf16_coarse_chunks(NCH) < NCH && bf16_coarse_chunks(NCH) <= 2 &&
+                            3 * (kCoarseSlots<NCH> + kFineSlots<NCH>) * 16 + 8 <= 80 * 1024;
+// A tile's fine part is loaded while some wave of the workgroup refined a block in one of
+// the last kHotTiles tiles (rows meet their refinements in runs: the tiles of their own
+// cluster); a block that needs the fine part of a tile loaded without it reads it from
+// global memory.
+constexpr int kHotTiles = 3;
 
 // One work item: query rows [block * 256, block * 256 + 256) of dataset s against the
 // references of tiles [tile_begin, tile_end), into buffer slot `split`. FIXED: the
@@ -328,12 +359,19 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // Thread chunk i of a tile is u = tid + 256 i: reference row u / row_u4, part
     // p = u % row_u4 of lane group p / group_u4, at LDS slot
     // row * stride + (p / group_u4) * gstride + p % group_u4.
+    // bf16 tiles hold their rows' coarse chunks first, then the fine ones (tile_chunk).
     int lds_at[Op::kLoadsMax];
 #pragma unroll
     for (int i = 0; i < Op::kLoadsMax; i++) {
         const int u = tid + 256 * i;
-        const int p = u % row_u4;
-        lds_at[i] = (u / row_u4) * stride + (p / group_u4) * gstride + p % group_u4;
+        if constexpr (OP == kOperandBf16) {
+            int r = 0, gg = 0, c = 0;
+            if (FIXED || i < loads) tile_chunk_of(u, nch, r, gg, c);
+            lds_at[i] = r * stride + gg * gstride + c;
+        } else {
+            const int p = u % row_u4;
+            lds_at[i] = (u / row_u4) * stride + (p / group_u4) * gstride + p % group_u4;
+        }
     }
     // Global -> registers P for tile T; registers P -> LDS buffer B. Plain unrolled loops
     // with constant indices keep the sets in registers.
@@ -450,6 +488,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // skipped, and the buffers end exactly as with every step taken.
     constexpr int kCoarse = bf16_coarse_chunks(NCH);
     constexpr bool kTwoStage = OP == kOperandBf16 && FIXED && kCoarse < NCH;
+    // Split tiles: coarse and fine parts in separate buffers (kPassSplit).
+    constexpr bool kSplit = kTwoStage && DMA && kPassSplit<OP, NCH>;
     float skip_b[4];
     if constexpr (kTwoStage) {
         const float mx = __uint_as_float(*global_ptr(s.max_norm_bits));
@@ -699,6 +739,167 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             __syncthreads();
             compact();
         }
+    } else if constexpr (kSplit) {
+        // Split tiles, loaded straight into LDS like the full tiles below: the coarse part of
+        // tile t+2 (and its fine part while the workgroup is hot) goes into the ring slot tile
+        // t-1 left while tile t is consumed; each wave waits for all its loads before a plain
+        // s_barrier. After each tile every wave publishes whether it refined a block, so all
+        // waves take the same decision for the next fine load.
+        typedef typename Op::Frag Frag;
+        constexpr int KC = kCoarse, KF = NCH - kCoarse;
+        constexpr int kFStride = lds_stride(KF), kFGStride = lds_gstride(KF);
+        constexpr int kCRounds = KC;  // 4 KC wave-wide loads per tile, one per wave per round
+        constexpr int kFRounds = (kFineSlots<NCH> + 4 * kWave - 1) / (4 * kWave);
+        // Sources (chunk index within a tile) of this lane's slots in each round.
+        int csrc[kCRounds], fsrc[kFRounds];
+        bool fon[kFRounds];
+#pragma unroll
+        for (int i = 0; i < kCRounds; i++) {
+            const int u = (i * 4 + w) * kWave + lane;
+            const int r = u / (4 * KC), p = u % (4 * KC);
+            csrc[i] = r * 4 * KC + (p ^ coarse_swizzle<KC>(r));
+        }
+#pragma unroll
+        for (int i = 0; i < kFRounds; i++) {
+            const int u = (i * 4 + w) * kWave + lane;
+            fon[i] = u < kFineSlots<NCH>;
+            const int v = fon[i] ? u : 0;
+            const int r = v / kFStride, q = v % kFStride, gg = q / kFGStride, c = q % kFGStride;
+            fsrc[i] = kTileRefs * 4 * KC + r * 4 * KF + ((gg < 4 && c < KF) ? gg * KF + c : 0);
+        }
+        // This lane's coarse slots within a row (rows cb 16 + lc share lc's swizzle).
+        int cofs[KC];
+#pragma unroll
+        for (int c = 0; c < KC; c++) cofs[c] = (g * KC + c) ^ coarse_swizzle<KC>(lc);
+        auto coarse_of = [&](int cur) __attribute__((always_inline)) -> u32x4* {
+            return cur == 0 ? dma.b0 : cur == 1 ? dma.b1 : dma.b2;
+        };
+        auto fine_of = [&](int cur) __attribute__((always_inline)) -> u32x4* {
+            return cur == 0 ? dma.f0 : cur == 1 ? dma.f1 : dma.f2;
+        };
+        auto lds_load = [&](const SCRUST_GLOBAL u32x4* from, u32x4* to) __attribute__((always_inline)) {
+            __builtin_amdgcn_global_load_lds(from, (__attribute__((address_space(3))) void*)to, 16, 0, 0);
+        };
+        // Tile T's coarse part into ring slot S, and its fine part while hot.
+        uint32_t have = 0u;  // bit S: ring slot S holds its tile's fine part
+        int hot = kHotTiles;
+        auto issue = [&](uint32_t T, auto slot) __attribute__((always_inline)) {
+            constexpr int S = decltype(slot)::value;
+            const SCRUST_GLOBAL u32x4* from = ref_rows + static_cast<size_t>(phys(T)) * tile_u4;
+#pragma unroll
+            for (int i = 0; i < kCRounds; i++) lds_load(from + csrc[i], coarse_of(S) + (i * 4 + w) * kWave);
+            if (hot > 0) {
+#pragma unroll
+                for (int i = 0; i < kFRounds; i++)
+                    if (fon[i]) lds_load(from + fsrc[i], fine_of(S) + (i * 4 + w) * kWave);
+                have |= 1u << S;
+            } else {
+                have &= ~(1u << S);
+            }
+        };
+        // Tile t (physical index) from ring slot cur: the coarse steps of its four blocks, then
+        // the remaining steps and the filter of each block that may append (the fine chunks
+        // from the ring slot, or from global memory when the slot was loaded cold). Returns
+        // whether any block of the wave was refined.
+        auto consume_split = [&](uint32_t t, int cur, bool fine_here) __attribute__((always_inline)) {
+            f32x4 acc0[4], acc1[4];
+            Frag c0[KC], c1[KC], f[KF];
+            bool refined = false;
+            auto load_c = [&](Frag (&b)[KC], int cb) __attribute__((always_inline)) {
+                const u32x4* row = coarse_of(cur) + (cb * 16 + lc) * (4 * KC);
+#pragma unroll
+                for (int c = 0; c < KC; c++) b[c] = __builtin_bit_cast(Frag, row[cofs[c]]);
+            };
+            auto coarse = [&](f32x4 (&acc)[4], const Frag (&b)[KC]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) acc[qb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int c = 0; c < KC; c++)
+#pragma unroll
+                    for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(b[c], a[qb][c], acc[qb]);
+            };
+            auto refine = [&](f32x4 (&acc)[4], int cb) __attribute__((always_inline)) {
+                uint64_t need = 0;
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) {
+                    const float m = __builtin_fmaxf(__builtin_fmaxf(acc[qb][0], acc[qb][1]),
+                                                    __builtin_fmaxf(acc[qb][2], acc[qb][3]));
+                    need |= __ballot(m > phi[qb] - skip_b[qb]);
+                }
+                if (need) {
+                    refined = true;
+                    if (fine_here) {
+                        const u32x4* row = fine_of(cur) + (cb * 16 + lc) * kFStride + g * kFGStride;
+#pragma unroll
+                        for (int c = 0; c < KF; c++) f[c] = __builtin_bit_cast(Frag, row[c]);
+                    } else {
+                        const SCRUST_GLOBAL u32x4* src = ref_rows + static_cast<size_t>(t) * tile_u4 +
+                                                         kTileRefs * 4 * KC + (cb * 16 + lc) * 4 * KF + g * KF;
+#pragma unroll
+                        for (int c = 0; c < KF; c++) f[c] = __builtin_bit_cast(Frag, src[c]);
+                    }
+#pragma unroll
+                    for (int c = KC; c < NCH; c++)
+#pragma unroll
+                        for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(f[c - KC], a[qb][c], acc[qb]);
+                    filter(acc, t, cb);
+                }
+            };
+            load_c(c0, 0);
+            load_c(c1, 1);
+            __builtin_amdgcn_sched_barrier(0);
+            coarse(acc0, c0);
+            __builtin_amdgcn_sched_barrier(0);
+            load_c(c0, 2);
+            __builtin_amdgcn_sched_barrier(0);
+            coarse(acc1, c1);
+            __builtin_amdgcn_sched_barrier(0);
+            load_c(c1, 3);
+            __builtin_amdgcn_sched_barrier(0);
+            refine(acc0, 0);
+            coarse(acc0, c0);
+            refine(acc1, 1);
+            coarse(acc1, c1);
+            refine(acc0, 2);
+            refine(acc1, 3);
+            return refined;
+        };
+        auto settle = [&]() __attribute__((always_inline)) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_s_barrier();
+        };
+        // Flags of parity p: byte w of need[p] is wave w's; a parity is written again only
+        // after the barrier that follows every wave's read of it.
+        volatile uint8_t* const flag_bytes = reinterpret_cast<volatile uint8_t*>(dma.need);
+        volatile uint32_t* const flag_words = dma.need;
+        int parity = 0;
+        auto step = [&](uint32_t t, auto slot) __attribute__((always_inline)) {
+            constexpr int S = decltype(slot)::value;
+            const bool refined = consume_split(phys(t), S, ((have >> S) & 1u) != 0u);
+            if (lane == 0) flag_bytes[parity * 4 + w] = refined ? 1 : 0;
+            settle();
+            const uint32_t any = __builtin_amdgcn_readfirstlane(flag_words[parity]);
+            parity ^= 1;
+            hot = any ? kHotTiles : (hot > 0 ? hot - 1 : 0);
+            compact();
+        };
+        const std::integral_constant<int, 0> s0{};
+        const std::integral_constant<int, 1> s1{};
+        const std::integral_constant<int, 2> s2{};
+        issue(t_begin, s0);
+        if (t_begin + 1 < t_end) issue(t_begin + 1, s1);
+        settle();
+        for (uint32_t t = t_begin;;) {
+            if (t + 2 < t_end) issue(t + 2, s2);
+            step(t, s0);
+            if (++t >= t_end) break;
+            if (t + 2 < t_end) issue(t + 2, s0);
+            step(t, s1);
+            if (++t >= t_end) break;
+            if (t + 2 < t_end) issue(t + 2, s1);
+            step(t, s2);
+            if (++t >= t_end) break;
+        }
     } else if constexpr (!DMA) {
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
     // merge into the loop head, where the compiler must then assume the prefetch just
@@ -739,7 +940,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             src_on[i] = u < kSlots;
             const int v = src_on[i] ? u : 0;
             const int q = v % stride, gg = q / gstride, c = q % gstride;
-            src_at[i] = (v / stride) * row_u4 + ((gg < 4 && c < group_u4) ? gg * group_u4 + c : 0);
+            src_at[i] = (gg < 4 && c < group_u4) ? tile_chunk(v / stride, gg, c, nch) : tile_chunk(v / stride, 0, 0, nch);
         }
         auto dma_tile = [&](uint32_t T, u32x4* to) __attribute__((always_inline)) {
             const SCRUST_GLOBAL u32x4* from = ref_rows + static_cast<size_t>(phys(T)) * tile_u4;
@@ -829,11 +1030,20 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
     // (4096 rows, about one cell), so its rows meet their neighbours first; the 16 blocks
     // share the start, and its tiles stay in L2 for all of them.
     item.pad = s.rotate ? (4u * (item.block & ~15u)) % (2 * pairs) : 0u;
-    if constexpr (kPassDma<OP, NCH>) {
+    if constexpr (kPassSplit<OP, NCH>) {
+        __shared__ u32x4 b0[kCoarseSlots<NCH>];
+        __shared__ u32x4 b1[kCoarseSlots<NCH>];
+        __shared__ u32x4 b2[kCoarseSlots<NCH>];
+        __shared__ u32x4 f0[kFineSlots<NCH>];
+        __shared__ u32x4 f1[kFineSlots<NCH>];
+        __shared__ u32x4 f2[kFineSlots<NCH>];
+        __shared__ uint32_t need[2];
+        pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2, f0, f1, f2, need});
+    } else if constexpr (kPassDma<OP, NCH>) {
         __shared__ u32x4 b0[kTileSlots<OP, NCH>];
         __shared__ u32x4 b1[kTileSlots<OP, NCH>];
         __shared__ u32x4 b2[kTileSlots<OP, NCH>];
-        pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2});
+        pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2, nullptr, nullptr, nullptr, nullptr});
     } else {
         __shared__ PassSmem<OP, NCH> sm;
         pass_block<OP, NCH, true>(s, item, &sm);
