@@ -116,13 +116,32 @@ def fixture_check(name: str, x, ids, dist, index: int | None = None, key: str | 
             "rows": int(ids.shape[0])}
 
 
-def measured_traffic(n: int, d: int, k: int, summary: Path = TRAFFIC_SUMMARY):
-    if (n, d, k) != (1_000_000, 50, 15) or not summary.exists():
+def profile_figure(summary: Path, key: str):
+    """A `key: value` line of a committed profile summary, or None."""
+    if not summary.exists():
         return None
     for line in summary.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def measured_traffic(n: int, d: int, k: int, summary: Path = TRAFFIC_SUMMARY):
+    if (n, d, k) != (1_000_000, 50, 15):
+        return None
+    return profile_figure(summary, "traffic_bytes_per_launch")
+
+
+# SQ_INSTS_MFMA of the bf16 pass at 1M x 50, k = 15 (committed counter profile): the MFMA work
+# the two-stage pass issues depends on how many blocks it refines, so it is measured.
+SQ_SUMMARY = ROOT / "profiles" / "pass_kernel_bf16_1Mx50_k15_sq_counters.md"
+
+
+def executed_bf16_tflops(n: int, d: int, k: int, pass_ms: float):
+    if (n, d, k) != (1_000_000, 50, 15):
+        return None
+    insts = profile_figure(SQ_SUMMARY, "mfma_instructions_per_launch")
+    return None if insts is None else insts * 16 * 16 * 32 * 2 / (pass_ms * 1e-3) / 1e12
 
 
 def gemm_ceiling() -> int:
@@ -421,14 +440,13 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
                        "traffic": measured_traffic(n, d, k),
-                       "kernel": f"pass_kernel<1, {(3 * d + 3 + 31) // 32}> "
-                                 "(bf16 operands, chunks of 32)",
+                       "kernel": f"pass_kernel<1, {(3 * d + 3 + 31) // 32}, ...> "
+                                 "(bf16 two-stage pass, chunks of 32)",
                        "algorithmic_flops": flops,
                        "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs",
-                       # The MFMA work issued: the hi/lo split's 3 products per feature plus
-                       # the norm terms, padded to chunks of 32, over n x npad pairs.
-                       "executed_tflops": 2.0 * n * ((n + 255) // 256 * 256) * 32 * ((3 * d + 3 + 31) // 32)
-                                          / (dev["pass_ms"] * 1e-3) / 1e12}
+                       # The MFMA work issued (coarse steps for every pair, the fine ones for
+                       # refined blocks), from the committed SQ_INSTS_MFMA reading.
+                       "executed_tflops": executed_bf16_tflops(n, d, k, dev["pass_ms"])}
     got_ids, got_dist = single.result()
     fixture = name if args.n == 0 and args.d == 50 and args.k == 15 else None
     if fixture:
