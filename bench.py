@@ -437,6 +437,10 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
     out["device_cells_per_s_best"] = n / (min(s["device_ms"] for s in st) * 1e-3)
     out["stages_ms"] = dev
     out["fallback_rows"] = int(st[-1]["fallback_rows"])
+    # Device memory of one build: the library's workspace (arena and launch metadata;
+    # the embedding and the outputs are the caller's device buffers here).
+    out["workspace_bytes"] = int(st[-1]["workspace_bytes"])
+    out["persistent_pass"] = bool(st[-1]["persistent"])
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": BF16_DENSE_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / BF16_DENSE_PEAK_TFLOPS,
                        "traffic": measured_traffic(n, d, k),

@@ -87,6 +87,9 @@ typedef struct scrust_hip_knn_stats {
     uint32_t tuned;           /* 1 when the work shape came from the tuning table */
     uint32_t tail_blocks;     /* query blocks of the launch's last round scanned in ranges */
     uint32_t tail_splits;     /* their reference ranges (0 when tail_blocks is 0) */
+    uint64_t workspace_bytes; /* device workspace the call used (its arena and metadata) */
+    uint32_t persistent;      /* 1: the pass ran one workgroup per slot with per-slot buffers */
+    uint32_t reserved;
 } scrust_hip_knn_stats;
 
 /* Message for the last failure on the calling thread ("" if none). */

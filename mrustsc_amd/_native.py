@@ -93,6 +93,9 @@ class KnnStats(ctypes.Structure):
         ("tuned", ctypes.c_uint32),
         ("tail_blocks", ctypes.c_uint32),
         ("tail_splits", ctypes.c_uint32),
+        ("workspace_bytes", ctypes.c_uint64),
+        ("persistent", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:

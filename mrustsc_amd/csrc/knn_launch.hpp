@@ -151,7 +151,21 @@ struct DatasetDesc {
     double split_rel;  // operand split error of a.b, relative to |a||b|
     double acc_rel;    // MFMA accumulation error, relative to the sum of |terms|
     double norm_rel;   // error of the |b|^2 feature, relative to |b|^2
+    // Persistent pass (persist = 1: single datasets on split tiles whose blocks scan whole
+    // ranges, the launch's last round excepted): `slots` workgroups, each with its own
+    // kPassRows-row region of `buf`, take query blocks from per-XCD queues (queue[0..7]) and
+    // then the last round's range items (queue[8]); the pass selects each whole block's
+    // candidates itself (cand, theta), and each range item's top kp keys go to tail_cand
+    // ((n - tail_row0) rows x splits ranges x kp keys) for merge_kernel.
+    uint32_t persist;
+    uint32_t slots;
+    uint32_t* queue;
+    uint64_t* tail_cand;
 };
+
+// Split-tile bf16 pass variants (the chunk counts knn_pass.hip compiles them for).
+inline bool split_pass_nch(int operand, uint32_t nch) { return operand == kOperandBf16 && nch >= 2 && nch <= 5; }
+constexpr int kQueueWords = 9;  // persistent pass: 8 XCD queues and the range-item queue
 
 // Row offset of reference range j's buffers and segment counts: range 0 holds every row,
 // a range j >= 1 only the rows from tail_row0 on (every row when tail_row0 is 0), so row r
@@ -222,6 +236,8 @@ bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t 
                        hipStream_t stream);
 // Final candidate selection of every pass row over its reference ranges (one launch per
 // selection width present; `widths` is a bit set of the select_e values 2, 4, 8, 16).
+// Persistent pass (host.persist): the last round's rows from their per-range lists.
+void launch_merge(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream);
 void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned widths,
                    hipStream_t stream);
 // Rerank of every pass row (one launch per candidate width present; `kps` is a bit set of

@@ -268,22 +268,83 @@ constexpr int kFineSlots = kTileRefs * lds_stride(NCH - bf16_coarse_chunks(NCH))
 template <int OP, int NCH>
 constexpr bool kPassSplit = OP == kOperandBf16 && bf16_coarse_chunks(NCH) < NCH && bf16_coarse_chunks(NCH) <= 2 &&
                             3 * (kCoarseSlots<NCH> + kFineSlots<NCH>) * 16 + 8 <= 80 * 1024;
+// The host's split_pass_nch names the chunk counts the split variants exist for.
+static_assert(!kPassSplit<kOperandBf16, 1> && kPassSplit<kOperandBf16, 2> && kPassSplit<kOperandBf16, 3> &&
+                  kPassSplit<kOperandBf16, 4> && kPassSplit<kOperandBf16, 5> && !kPassSplit<kOperandBf16, 6>,
+              "split_pass_nch must match kPassSplit");
 // A tile's fine part is loaded while some wave of the workgroup refined a block in one of
 // the last kHotTiles tiles (rows meet their refinements in runs: the tiles of their own
 // cluster); a block that needs the fine part of a tile loaded without it reads it from
 // global memory.
 constexpr int kHotTiles = 3;
 
+// Persistent pass: the end of a work item. Each wave selects, row after row, the kp smallest
+// keys (unordered) of its 64 rows' buffers (row q's four segment counts in c(q)) exactly as
+// select_kernel does: a whole block's rows into cand with their kp-th score in theta, a range
+// item's rows (splits > 1, rows from tail_row0 on) into their range's place in tail_cand. A row with fewer than kp keys gets all
+// of them, then kNoKey, and theta +inf. `empty`: no keys (a dataset that skipped the scan).
+template <int E, class Counts>
+__device__ __forceinline__ void finish_rows(const DatasetDesc& s, const PassItem& item,
+                                            const SCRUST_GLOBAL uint64_t* rbuf, uint32_t row0,
+                                            int lane, bool empty, Counts counts) {
+    const uint32_t kp = s.kp, cap = s.cap;
+    const bool tail = s.splits > 1 && row0 >= s.tail_row0;
+    for (int q = 0; q < kWave; q++) {
+        const uint32_t row = row0 + static_cast<uint32_t>(q);
+        if (row >= s.n) break;
+        uint32_t c[4] = {0u, 0u, 0u, 0u};
+        if (!empty) counts(q, c);
+        uint64_t key[E];
+        gather_row<E>(rbuf + static_cast<size_t>(q) * cap, c, lane, key);
+        SCRUST_GLOBAL uint64_t* out =
+            tail ? global_ptr(s.tail_cand) + ((static_cast<size_t>(row - s.tail_row0) * s.splits + item.split) * kp)
+                 : global_ptr(s.cand) + static_cast<size_t>(row) * kp;
+        uint64_t m[E];
+        uint32_t total = 0;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            m[i] = __ballot(key[i] != kNoKey);
+            total += static_cast<uint32_t>(__popcll(m[i]));
+        }
+        float theta = kInf;
+        if (total >= kp) {
+            theta = select_keys<E>(key, static_cast<int>(kp), lane, out, 0u, 0u);
+        } else {
+            uint32_t base = 0;
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                if (key[i] != kNoKey) out[base + lanes_below(m[i])] = key[i];
+                base += static_cast<uint32_t>(__popcll(m[i]));
+            }
+            for (uint32_t slot = base + static_cast<uint32_t>(lane); slot < kp; slot += kWave) out[slot] = kNoKey;
+        }
+        if (!tail && lane == 0) global_ptr(s.theta)[row] = theta;
+    }
+}
+
 // One work item: query rows [block * 256, block * 256 + 256) of dataset s against the
-// references of tiles [tile_begin, tile_end), into buffer slot `split`. FIXED: the
-// dataset's chunk count equals NCH at compile time.
+// references of tiles [tile_begin, tile_end), into buffer slot `split` (persistent pass:
+// into the workgroup's own buffer region). FIXED: the dataset's chunk count equals NCH at
+// compile time.
 template <int OP, int NCH, bool FIXED, bool DMA = false, bool PAIRS = false>
 __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem& item,
                                            PassSmem<OP, NCH>* sm, DmaTiles dma = {}) {
     typedef Operand<OP, NCH> Op;
+    // Persistent pass: the workgroup's buffer region is its slot's (workgroup index).
+    const bool persist = PAIRS && s.persist != 0u;
     // A dataset with a squared norm past kPassNormLimit is served by the exact kernel (the
-    // rerank lists all its rows); its segment counts stay at the zeros they were cleared to.
-    if (__uint_as_float(*global_ptr(s.max_norm_bits)) > kPassNormLimit) return;
+    // rerank lists all its rows); its segment counts stay at the zeros they were cleared to
+    // (persistent pass: its rows get empty candidate lists here).
+    if (__uint_as_float(*global_ptr(s.max_norm_bits)) > kPassNormLimit) {
+        if constexpr (PAIRS) {
+            if (persist) {
+                const int w0 = static_cast<int>(threadIdx.x) >> 6, l0 = static_cast<int>(threadIdx.x) & 63;
+                finish_rows<2>(s, item, global_ptr(s.buf), item.block * kPassRows + w0 * kWave, l0, true,
+                               [](int, uint32_t (&)[4]) {});
+            }
+        }
+        return;
+    }
     const int nch = FIXED ? NCH : static_cast<int>(s.nch);
     const int row_u4 = FIXED ? Op::kRowU4Max : Op::row_u4(nch);
     const int group_u4 = FIXED ? Op::kGroupU4Max : Op::group_u4(nch);
@@ -298,8 +359,10 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     const uint32_t qbase = item.block * kPassRows;
     const SCRUST_GLOBAL u32x4* ref_rows = global_ptr(reinterpret_cast<const u32x4*>(
         OP == kOperandBf16 ? static_cast<const void*>(s.pack_ref) : static_cast<const void*>(s.pack_f32)));
-    // This range's slot of the row buffers (indexed by the dataset's row).
-    SCRUST_GLOBAL uint64_t* const buf = global_ptr(s.buf) + range_row_base(s, item.split) * s.cap;
+    // The item's buffer rows: this range's slot of the row buffers (indexed by the dataset's
+    // row), or the workgroup's own region (persistent pass).
+    SCRUST_GLOBAL uint64_t* const rbuf =
+        global_ptr(s.buf) + (persist ? static_cast<size_t>(blockIdx.x) * kPassRows : range_row_base(s, item.split) + qbase) * s.cap;
 
     // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
     // qb's fragment in lane l carries row qb*16 + l%16.
@@ -335,7 +398,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // Pair steps (split tiles with PAIRS) scan two tiles between compactions: 32 keys.
     const uint32_t trigger = seg - (PAIRS ? 32u : 16u);
     SCRUST_GLOBAL uint64_t* const lane_seg =
-        buf + static_cast<size_t>(qbase + w * kWave + lc) * cap + g * seg;
+        rbuf + static_cast<size_t>(w * kWave + lc) * cap + g * seg;
 
     // The reference operand holds whole pairs of tiles (rows n.. are sentinels), so a
     // tile is one contiguous run of tile_u4 16-byte chunks; ranges hold whole pairs.
@@ -607,7 +670,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         uint32_t c[4];
         row_counts(q, c);
         uint64_t key[E];
-        gather_row<E>(buf + static_cast<size_t>(qbase + w * kWave + q) * cap, c, lane, key);
+        gather_row<E>(rbuf + static_cast<size_t>(w * kWave + q) * cap, c, lane, key);
         for (;;) {
             const bool more = full != 0;
             int qn = 0;
@@ -617,10 +680,10 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
                 qn = __builtin_ctzll(full);
                 full &= full - 1;
                 row_counts(qn, cn);
-                gather_row<E>(buf + static_cast<size_t>(qbase + w * kWave + qn) * cap, cn, lane, keyn);
+                gather_row<E>(rbuf + static_cast<size_t>(w * kWave + qn) * cap, cn, lane, keyn);
             }
             float theta;
-            const uint32_t kept = compact_row<E>(buf + static_cast<size_t>(qbase + w * kWave + q) * cap,
+            const uint32_t kept = compact_row<E>(rbuf + static_cast<size_t>(w * kWave + q) * cap,
                                                  key, c[0] + c[1] + c[2] + c[3], kp, lane, theta);
             // Segment g now holds min(share, kept - g * share) keys.
             const uint32_t share = (kept + 3) / 4;
@@ -1016,6 +1079,16 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     }
 #undef SCRUST_LOAD_TILE
 #undef SCRUST_STORE_TILE
+    if constexpr (PAIRS) {
+        if (persist) {
+            // The item's selection from this wave's buffer rows (counts from the registers).
+            auto counts = [&](int q, uint32_t (&c)[4]) __attribute__((always_inline)) { row_counts(q, c); };
+            const SCRUST_GLOBAL uint64_t* wbuf = rbuf + static_cast<size_t>(w) * kWave * cap;
+            if (cap <= 256) finish_rows<4>(s, item, wbuf, qbase + w * kWave, lane, false, counts);
+            else finish_rows<8>(s, item, wbuf, qbase + w * kWave, lane, false, counts);
+            return;
+        }
+    }
     // Segment counts for the final selection (select_kernel): byte g of the row's word.
     SCRUST_GLOBAL uint8_t* const counts = reinterpret_cast<SCRUST_GLOBAL uint8_t*>(
         global_ptr(s.seg_counts) + range_row_base(s, item.split));
@@ -1038,25 +1111,23 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t count) {
 
 // At most two waves per SIMD: the allocator then keeps the tile prefetch in registers
 // instead of spilling it to reach a higher occupancy the LDS budget would allow.
-template <int OP, int NCH, bool PAIRS = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-void pass_kernel(const DatasetDesc* __restrict__ set) {
-    const DatasetDesc& s = *set;
-    // Workgroups below the tail block scan their block's whole range; from it on, workgroup
-    // tail_block + g scans range j = g / tail_blocks of block tail_block + g % tail_blocks:
-    // the workgroups resident together scan the same references (shared through L2).
+// Work item `v` of a single dataset's pass, with tail_block = tail_row0 / kPassRows (0 when
+// every block scans in `splits` ranges): v < tail_block is a whole block's scan (block v),
+// tail_block + g the g-th range item (range g / tail_blocks of block tail_block +
+// g % tail_blocks). There are tail_block + (blocks - tail_block) splits items.
+__device__ __forceinline__ PassItem single_item(const DatasetDesc& s, uint32_t v) {
     const uint32_t pairs = (s.n + 2 * kTileRefs - 1) / (2 * kTileRefs);
     const uint32_t blocks = s.npad / kPassRows;
     const uint32_t tail_block = s.tail_row0 / kPassRows;
     PassItem item;
     item.set = 0;
-    if (blockIdx.x < tail_block) {
+    if (v < tail_block) {
         item.split = 0;
-        item.block = s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x;
+        item.block = v;
         item.tile_begin = 0;
         item.tile_end = 2 * pairs;
     } else {
-        const uint32_t g = blockIdx.x - tail_block, tail_blocks = blocks - tail_block;
+        const uint32_t g = v - tail_block, tail_blocks = blocks - tail_block;
         item.split = g / tail_blocks;
         item.block = tail_block + (g - item.split * tail_blocks);
         item.tile_begin = 2 * item.split * s.chunk_pairs;
@@ -1067,12 +1138,65 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
     // (4096 rows, about one cell), so its rows meet their neighbours first; the 16 blocks
     // share the start, and its tiles stay in L2 for all of them.
     item.pad = s.rotate ? (4u * (item.block & ~15u)) % (2 * pairs) : 0u;
+    return item;
+}
+
+template <int OP, int NCH, bool PAIRS = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void pass_kernel(const DatasetDesc* __restrict__ set) {
+    const DatasetDesc& s = *set;
+    const uint32_t blocks = s.npad / kPassRows;
+    const uint32_t tail_block = s.tail_row0 / kPassRows;
     if constexpr (kPassSplit<OP, NCH> && PAIRS) {
         __shared__ u32x4 b0[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b1[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b2[2 * kCoarseSlots<NCH>];
-        pass_block<OP, NCH, true, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2, nullptr, nullptr, nullptr, nullptr});
-    } else if constexpr (kPassSplit<OP, NCH>) {
+        const DmaTiles dma{b0, b1, b2, nullptr, nullptr, nullptr, nullptr};
+        if (s.persist) {
+            // Persistent: workgroup b runs on XCD x = b mod 8 (round-robin dispatch) and takes
+            // the blocks of x's contiguous run from queue x (the spatial order keeps a run's
+            // blocks, which share their scans' first tiles, in one L2), then the last round's
+            // range items from queue 8. Every workgroup leaves once both are drained.
+            // Whole blocks are [0, whole): every block with one range, the blocks before the
+            // last round's when that scans in ranges, none when every block does (items v < whole
+            // are whole blocks under single_item's reading, the items from whole on ranges).
+            __shared__ uint32_t next;
+            const uint32_t whole = s.splits == 1 ? blocks : tail_block;
+            const uint32_t x = blockIdx.x % 8u;
+            const uint32_t base = whole / 8u, extra = whole % 8u;
+            const uint32_t start = x * base + (x < extra ? x : extra), len = base + (x < extra ? 1u : 0u);
+            const uint32_t tail_items = (blocks - whole) * s.splits;
+            for (;;) {
+                if (threadIdx.x == 0) {
+                    uint32_t v = ~0u;
+                    const uint32_t j = atomicAdd(&s.queue[x], 1u);
+                    if (j < len) {
+                        v = start + j;
+                    } else {
+                        const uint32_t gi = atomicAdd(&s.queue[8], 1u);
+                        if (gi < tail_items) v = whole + gi;
+                    }
+                    next = v;
+                }
+                __syncthreads();
+                const uint32_t v = next;
+                __syncthreads();
+                if (v == ~0u) break;
+                pass_block<OP, NCH, true, true, true>(s, single_item(s, v), nullptr, dma);
+                __syncthreads();
+            }
+            return;
+        }
+        const uint32_t v = blockIdx.x < tail_block && s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x;
+        pass_block<OP, NCH, true, true, true>(s, single_item(s, v), nullptr, dma);
+        return;
+    }
+    // Workgroups below the tail block scan their block's whole range; from it on, workgroup
+    // tail_block + g scans range j = g / tail_blocks of block tail_block + g % tail_blocks:
+    // the workgroups resident together scan the same references (shared through L2).
+    const PassItem item =
+        single_item(s, blockIdx.x < tail_block && s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x);
+    if constexpr (kPassSplit<OP, NCH>) {
         __shared__ u32x4 b0[kCoarseSlots<NCH>];
         __shared__ u32x4 b1[kCoarseSlots<NCH>];
         __shared__ u32x4 b2[kCoarseSlots<NCH>];
@@ -1090,6 +1214,60 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
         __shared__ PassSmem<OP, NCH> sm;
         pass_block<OP, NCH, true>(s, item, &sm);
     }
+}
+
+// Persistent pass, rows scanned in ranges (from tail_row0 on, splits > 1): the kp smallest
+// keys (unordered) of the union of the rows' per-range top-kp lists (tail_cand), and the
+// union's kp-th score, one wave per row:
+// the same result as select_kernel over the ranges' whole buffers (the kp smallest of a
+// union lie among each part's kp smallest; keys are totally ordered by (score, id)).
+template <int E>
+__global__ __launch_bounds__(256) void merge_kernel(const DatasetDesc* __restrict__ set) {
+    const DatasetDesc& s = *set;
+    const uint32_t row = s.tail_row0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= s.n) return;
+    const uint32_t kp = s.kp, total_keys = s.splits * kp;
+    const SCRUST_GLOBAL uint64_t* src = global_ptr(s.tail_cand) + static_cast<size_t>(row - s.tail_row0) * total_keys;
+    uint64_t key[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const uint32_t e = static_cast<uint32_t>(i * kWave + lane);
+        key[i] = e < total_keys ? src[e] : kNoKey;
+    }
+    uint64_t m[E];
+    uint32_t total = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        m[i] = __ballot(key[i] != kNoKey);
+        total += static_cast<uint32_t>(__popcll(m[i]));
+    }
+    SCRUST_GLOBAL uint64_t* out = global_ptr(s.cand) + static_cast<size_t>(row) * kp;
+    float theta = kInf;
+    if (total >= kp) {
+        theta = select_keys<E>(key, static_cast<int>(kp), lane, out, 0u, 0u);
+    } else {
+        uint32_t base = 0;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            if (key[i] != kNoKey) out[base + lanes_below(m[i])] = key[i];
+            base += static_cast<uint32_t>(__popcll(m[i]));
+        }
+        for (uint32_t slot = base + static_cast<uint32_t>(lane); slot < kp; slot += kWave) out[slot] = kNoKey;
+    }
+    if (lane == 0) global_ptr(s.theta)[row] = theta;
+}
+
+void launch_merge(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream) {
+    if (!host.persist || host.splits <= 1 || host.n <= host.tail_row0) return;
+    const unsigned blocks = (host.n - host.tail_row0 + 3) / 4;
+    const uint32_t keys = host.splits * host.kp;
+    if (keys <= 128)
+        hipLaunchKernelGGL(merge_kernel<2>, dim3(blocks), dim3(256), 0, stream, set);
+    else if (keys <= 256)
+        hipLaunchKernelGGL(merge_kernel<4>, dim3(blocks), dim3(256), 0, stream, set);
+    else
+        hipLaunchKernelGGL(merge_kernel<8>, dim3(blocks), dim3(256), 0, stream, set);
 }
 
 // Local-origin fp16 pass of one dataset: workgroup i scans every reference for the 512
@@ -1262,7 +1440,9 @@ static void run_pass(const DatasetDesc* set, const DatasetDesc& host, hipStream_
     // and steps of four tiles (compactions inside the step) 5% slower, than pairs in three.
     if constexpr (kPassSplit<OP, NCH>) {
         if (host.cap >= 256) {
-            hipLaunchKernelGGL((pass_kernel<OP, NCH, true>), dim3(groups), dim3(256), 0, stream, set);
+            // The persistent pass launches one workgroup per slot.
+            hipLaunchKernelGGL((pass_kernel<OP, NCH, true>), dim3(host.persist ? host.slots : groups), dim3(256), 0,
+                               stream, set);
             return;
         }
     }

@@ -270,11 +270,23 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                     s.chunk_pairs = len;
                     s.tail_row0 = static_cast<uint32_t>((blocks - rem) * kPassRows);
                     s.select_e1 = next_pow2(std::max<uint32_t>(2, (s.cap + 63) / 64));
-                    plan.select_widths |= s.select_e1;
                 }
             }
             s.select_e = next_pow2(std::max<uint32_t>(2, (splits * s.cap + 63) / 64));
-            plan.select_widths |= s.select_e;
+            // Persistent pass: a single dataset on split tiles in pairs whose blocks outnumber
+            // the workgroup slots. Its candidate buffers are per slot (kPassRows rows each)
+            // instead of per row, the pass selects each whole block's candidates itself, and
+            // rows scanned in ranges keep each range's top kp for merge_kernel.
+            const bool persist = single && !s.local && split_pass_nch(plan.operand, s.nch) && s.cap >= 256 &&
+                                 blocks > slots;
+            if (persist) {
+                s.persist = 1;
+                s.slots = static_cast<uint32_t>(slots);
+                s.select_e = 0;
+                s.select_e1 = 0;
+            } else {
+                plan.select_widths |= s.select_e | s.select_e1;
+            }
             if (s.local) {
                 s.pack_src = static_cast<float*>(take(ref_rows * 128ull * s.nch));
                 s.pack_query = static_cast<uint16_t*>(take(sp.n * 64ull * s.nch));
@@ -287,9 +299,17 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             } else {
                 s.pack_f32 = static_cast<float*>(take(ref_rows * 16ull * s.group));
             }
-            s.buf = static_cast<uint64_t*>(take(range_rows(s) * s.cap * sizeof(uint64_t)));
-            s.seg_counts = reinterpret_cast<uint32_t*>(count_words * sizeof(uint32_t));  // region offset
-            count_words += range_rows(s);
+            if (s.persist) {
+                s.buf = static_cast<uint64_t*>(take(static_cast<size_t>(s.slots) * kPassRows * s.cap * sizeof(uint64_t)));
+                if (s.splits > 1)
+                    s.tail_cand = static_cast<uint64_t*>(
+                        take(static_cast<size_t>(s.n - s.tail_row0) * s.splits * s.kp * sizeof(uint64_t)));
+                s.seg_counts = reinterpret_cast<uint32_t*>(count_words * sizeof(uint32_t));  // unused
+            } else {
+                s.buf = static_cast<uint64_t*>(take(range_rows(s) * s.cap * sizeof(uint64_t)));
+                s.seg_counts = reinterpret_cast<uint32_t*>(count_words * sizeof(uint32_t));  // region offset
+                count_words += range_rows(s);
+            }
             s.cand = static_cast<uint64_t*>(take(sp.n * s.kp * sizeof(uint64_t)));
             s.theta = static_cast<float*>(take(sp.n * sizeof(float)));
         }
@@ -381,6 +401,7 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
             }
             fix(s.buf);
             fix(s.seg_counts);
+            if (s.persist && s.splits > 1) fix(s.tail_cand);
             fix(s.cand);
             fix(s.theta);
             if (s.perm) {
@@ -397,6 +418,10 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
     plan.d_list = reinterpret_cast<uint32_t*>(m);
     m += align_up(static_cast<size_t>(plan.shape.total_rows) * sizeof(uint32_t) + 1);
     plan.d_scalars = reinterpret_cast<uint32_t*>(m);
+    // The persistent pass's work queues: scalars cleared before every pass.
+    static_assert(kScalarItemCounter + kQueueWords <= kScalarCount, "queue words among the scalars");
+    for (DatasetDesc& s : plan.sets)
+        if (s.persist) s.queue = plan.d_scalars + kScalarItemCounter;
     const size_t sets_bytes = plan.sets.size() * sizeof(DatasetDesc);
     const size_t items_bytes = plan.items.size() * sizeof(PassItem);
     const size_t list_bytes = plan.exact_rows.size() * sizeof(uint32_t);
@@ -474,6 +499,7 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     }
     if ((err = mark(2)) != hipSuccess) return hip_failure(err, "record event");
     if (plan.any_pass) {
+        if (single) launch_merge(plan.d_sets, plan.sets[0], stream);
         launch_select(plan.d_sets, plan.shape, plan.select_widths, stream);
         if ((err = hipGetLastError()) != hipSuccess) return hip_failure(err, "launch candidate selection");
         launch_rerank(plan.d_sets, plan.shape, plan.kps, plan.d_list,

@@ -190,6 +190,9 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
         stats->tail_splits = tail ? s0.splits : 0u;
         stats->order = plan.ordered ? 2u : 1u;
         stats->tuned = plan.tuned ? 1u : 0u;
+        stats->workspace_bytes = plan.arena_bytes + plan.meta_bytes;
+        stats->persistent = plan.any_pass && s0.persist ? 1u : 0u;
+        stats->reserved = 0u;
     }
     return SCRUST_HIP_OK;
 }

@@ -364,6 +364,10 @@ def test_last_round_blocks_scan_in_ranges_and_match_the_oracle(context, tmp_path
     finally:
         mrustsc_amd.tune_clear()
     assert (stats2["splits"], stats2["tail_blocks"], stats2["tuned"]) == (2, 0, 1)
+    if precision == "bf16":
+        # More blocks than workgroup slots: the split-tile pass runs persistent, with one
+        # buffer region per slot, under both plans (range items keep per-range lists).
+        assert (stats["persistent"], stats2["persistent"]) == (1, 1)
     np.testing.assert_array_equal(ids, ids2)
     np.testing.assert_array_equal(dist, dist2)
     tail_row0 = (blocks - tail_blocks) * 256
@@ -451,6 +455,45 @@ def test_random_large_k_cases_match_the_oracle(context):
         want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
         np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
         np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)
+
+
+def test_persistent_pass_equals_per_row_buffers_and_the_oracle(context):
+    # 300,000 cells (1,172 query blocks on 512 slots): the persistent pass against per-row
+    # buffers (k = 15: buffers of 128 slots, single-tile steps; k = 30, cosine, 64
+    # candidates: the fp32 pass), every row equal; sampled rows equal to the oracle.
+    n = 300_000
+    x = synthetic.clustered(n, 50, seed=61)
+    for k, metric in ((15, "euclidean"), (30, "cosine")):
+        print(f"[case] n={n} d=50 k={k} {metric} persistent pass vs per-row buffers", flush=True)
+        ids, dist, st = mrustsc_amd.knn_hip(x, k, metric=metric, context=context, stats=True)
+        if k == 15:
+            ids2, dist2, st2 = mrustsc_amd.knn_hip(x, k, metric=metric, context=context, stats=True, tile=128)
+        else:
+            ids2, dist2, st2 = mrustsc_amd.knn_hip(x, k, metric=metric, precision="fp32", context=context, stats=True)
+        print(st, st2, flush=True)
+        assert (st["persistent"], st2["persistent"]) == (1, 0)
+        assert st["workspace_bytes"] < st2["workspace_bytes"]
+        np.testing.assert_array_equal(ids, ids2)
+        np.testing.assert_array_equal(dist, dist2)
+        queries = np.concatenate([np.random.default_rng(k).choice(n, 24, replace=False), [0, n - 1]])
+        want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=queries)
+        np.testing.assert_array_equal(ids[queries], want_ids)
+        np.testing.assert_array_equal(dist[queries], want_dist)
+
+
+def test_persistent_pass_skipped_for_huge_coordinates_lists_every_row(context):
+    # A dataset past the pass's norm limit skips the scan; the persistent pass still gives
+    # every row an empty candidate list, so every row takes the exact kernel.
+    n = 140_000
+    x = (synthetic.clustered(n, 10, seed=52, n_clusters=6).astype(np.float64) * 1e16).astype(np.float32)
+    print(f"[case] n={n} d=10 k=15 bf16 huge coordinates, persistent pass", flush=True)
+    ids, dist, st = mrustsc_amd.knn_hip(x, 15, context=context, stats=True)
+    print(st, flush=True)
+    assert st["persistent"] == 1 and st["fallback_rows"] == n
+    queries = np.concatenate([np.random.default_rng(3).choice(n, 16, replace=False), [0, n - 1]])
+    want_ids, want_dist = oracle.knn(x, 15, queries=queries)
+    np.testing.assert_array_equal(ids[queries], want_ids)
+    np.testing.assert_array_equal(dist[queries], want_dist)
 
 
 @pytest.mark.parametrize("precision", ["bf16", "fp32"])
