@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "knn_launch.hpp"
+
 namespace scrust_hip {
 
 constexpr int kWave = 64;
@@ -123,6 +125,35 @@ __device__ __forceinline__ uint16_t bf16_rn(float f) {
 
 __device__ __forceinline__ float bf16_value(uint16_t h) {
     return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+// Chunk m (16 bytes: K indices k_index(m, 0..7, nch)) of the bf16 query operand of a row with
+// coordinates row[0 .. d): [h | -1/2 -1/2 -1/2 | h | l | 0], h = bf16(c), l = bf16(c - h). The
+// parts against the reference's [h | norm terms | l | h | 0]: h.h, the -1/2 weights of the
+// norm terms, h_query.l_reference and l_query.h_reference.
+template <class Row>
+__device__ __forceinline__ u32x4 bf16_query_chunk(Row row, int d, int m, int nch) {
+    constexpr uint16_t kMinusHalf = 0xBF00u;
+    uint32_t w[4];
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        uint16_t v[2];
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++) {
+            const int f = k_index(m, e + h2, nch);
+            v[h2] = 0;
+            if (f < d || (f >= d + 3 && f < 3 * d + 3)) {
+                const int part = f < d ? 0 : (f < 2 * d + 3 ? 1 : 2);
+                const float c = row[part == 0 ? f : f - 3 - part * d];
+                const uint16_t h = bf16_rn(c);
+                v[h2] = part == 2 ? bf16_rn(c - bf16_value(h)) : h;
+            } else if (f < d + 3) {
+                v[h2] = kMinusHalf;
+            }
+        }
+        w[e >> 1] = static_cast<uint32_t>(v[0]) | static_cast<uint32_t>(v[1]) << 16;
+    }
+    return u32x4{w[0], w[1], w[2], w[3]};
 }
 
 // ---------------------------------------------------------------------------

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
     __shared__ float xs[kPrepRows * kMaxPassD];
     __shared__ double mean_s[kMaxPassD];
     __shared__ float nrm_s[kPrepRows];
-    constexpr uint16_t kBf16Inf = 0x7F80u, kBf16MinusHalf = 0xBF00u;
+    constexpr uint16_t kBf16Inf = 0x7F80u;
     const DatasetDesc& s = sets[blockIdx.z];
     if (!s.on_pass) return;
     const uint32_t n = s.n, ref_rows = (n + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
         const uint32_t ref_row = r0 + row;
         const size_t ref_at = static_cast<size_t>(ref_row / kTileRefsHost) * (kTileRefsHost * chunks) +
                               tile_chunk(static_cast<int>(ref_row % kTileRefsHost), chunk / nch, chunk % nch, nch);
-        uint32_t rw[4], qw[4];
+        uint32_t rw[4];
         if (row < rows) {
             const float* c_row = xs + row * d;
             const float nrm = nrm_s[row];
@@ -291,30 +291,27 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
             const uint16_t nl2 = bf16_rn(rest - bf16_value(nl));
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
-                uint16_t rv[2], qv[2];
+                uint16_t rv[2];
 #pragma unroll
                 for (int h2 = 0; h2 < 2; h2++) {
                     const int f = k_index(chunk, e + h2, nch);
                     rv[h2] = 0;
-                    qv[h2] = 0;
                     if (f < d || (f >= d + 3 && f < 3 * d + 3)) {
                         // part 0: h.h, part 1: h_query.l_reference, part 2: l_query.h_reference
                         const int part = f < d ? 0 : (f < 2 * d + 3 ? 1 : 2);
-                        const float c = c_row[part == 0 ? f : f - 3 - (part - 1) * d - d];
+                        const float c = c_row[part == 0 ? f : f - 3 - part * d];
                         const uint16_t h = bf16_rn(c);
-                        const uint16_t l = bf16_rn(c - bf16_value(h));
-                        rv[h2] = part == 1 ? l : h;
-                        qv[h2] = part == 2 ? l : h;
+                        rv[h2] = part == 1 ? bf16_rn(c - bf16_value(h)) : h;
                     } else if (f < d + 3) {
                         rv[h2] = f == d ? nh : (f == d + 1 ? nl : nl2);
-                        qv[h2] = kBf16MinusHalf;
                     }
                 }
                 rw[e >> 1] = static_cast<uint32_t>(rv[0]) | static_cast<uint32_t>(rv[1]) << 16;
-                qw[e >> 1] = static_cast<uint32_t>(qv[0]) | static_cast<uint32_t>(qv[1]) << 16;
             }
             rp[ref_at] = u32x4{rw[0], rw[1], rw[2], rw[3]};
-            qp[u] = u32x4{qw[0], qw[1], qw[2], qw[3]};
+            // The persistent pass forms its query fragments from the coordinates itself
+            // (bf16_query_chunk) and has no query operand array.
+            if (s.pack_query) qp[u] = bf16_query_chunk(c_row, d, chunk, nch);
         } else {
             // Sentinel reference row: +inf at the norm feature (K index d), zero elsewhere.
 #pragma unroll

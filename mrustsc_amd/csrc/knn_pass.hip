@@ -365,7 +365,27 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         global_ptr(s.buf) + (persist ? static_cast<size_t>(blockIdx.x) * kPassRows : range_row_base(s, item.split) + qbase) * s.cap;
 
     // Query fragments: the B operand of every MFMA, held for the whole scan. Query block
-    // qb's fragment in lane l carries row qb*16 + l%16.
+    // qb's fragment in lane l carries row qb*16 + l%16. The persistent pass has no query
+    // operand array: the workgroup first writes its 256 rows' operand rows (as
+    // prepare_rows_kernel writes them for the other paths, bf16_query_chunk) into the start
+    // of its own buffer region, which the item's appends only reuse after the barrier that
+    // follows the loads.
+    const SCRUST_GLOBAL u32x4* qsrc = global_ptr(reinterpret_cast<const u32x4*>(s.pack_query));
+    if constexpr (OP == kOperandBf16) {
+        if (persist) {
+            SCRUST_GLOBAL u32x4* stage = reinterpret_cast<SCRUST_GLOBAL u32x4*>(rbuf);
+            const SCRUST_GLOBAL float* coords = global_ptr(s.coords);
+#pragma unroll 1
+            for (int u = tid; u < kPassRows * 4 * NCH; u += 256) {
+                const int r = u / (4 * NCH), m = u - r * (4 * NCH);
+                uint32_t row = qbase + static_cast<uint32_t>(r);
+                if (row >= n) row = n - 1;
+                stage[u] = bf16_query_chunk(coords + static_cast<size_t>(row) * s.d, static_cast<int>(s.d), m, NCH);
+            }
+            __syncthreads();
+            qsrc = stage;
+        }
+    }
     typename Op::Frag a[4][NCH];
 #pragma unroll
     for (int qb = 0; qb < 4; qb++) {
@@ -373,7 +393,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         if (row >= n) row = n - 1;
         if constexpr (OP == kOperandBf16 || OP == kOperandF16Local) {
             typedef typename Op::Frag Frag;
-            const SCRUST_GLOBAL u32x4* src = global_ptr(reinterpret_cast<const u32x4*>(s.pack_query)) + static_cast<size_t>(row) * row_u4 + g * nch;
+            const size_t at = persist ? static_cast<size_t>(w * kWave + qb * 16 + lc) : static_cast<size_t>(row);
+            const SCRUST_GLOBAL u32x4* src = qsrc + at * row_u4 + g * nch;
 #pragma unroll
             for (int c = 0; c < NCH; c++)
                 a[qb][c] = (FIXED || c < nch) ? __builtin_bit_cast(Frag, src[c]) : Frag{};
@@ -384,6 +405,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
                 a[qb][c] = (FIXED || c < nch) ? ((g * nch + c == static_cast<int>(s.d)) ? -0.5f : src[c]) : 0.0f;
         }
     }
+    if (persist) __syncthreads();
     // phi[qb] = -theta/2 of this lane's query row qb*16 + l%16; cnt[qb] = keys in this
     // lane's segment of that row's buffer.
     float phi[4];

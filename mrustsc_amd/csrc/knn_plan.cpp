@@ -295,7 +295,8 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             } else if (plan.operand == kOperandBf16) {
                 const size_t rowb = 64ull * s.nch;
                 s.pack_ref = static_cast<uint16_t*>(take(ref_rows * rowb));
-                s.pack_query = static_cast<uint16_t*>(take(sp.n * rowb));
+                // The persistent pass forms its query fragments from the coordinates.
+                if (!s.persist) s.pack_query = static_cast<uint16_t*>(take(sp.n * rowb));
             } else {
                 s.pack_f32 = static_cast<float*>(take(ref_rows * 16ull * s.group));
             }
@@ -395,7 +396,7 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
                 fix(s.anorm);
             } else if (plan.operand == kOperandBf16) {
                 fix(s.pack_ref);
-                fix(s.pack_query);
+                if (!s.persist) fix(s.pack_query);
             } else {
                 fix(s.pack_f32);
             }
