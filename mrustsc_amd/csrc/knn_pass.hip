@@ -39,6 +39,7 @@
 #include "knn_device.hpp"
 #include "knn_launch.hpp"
 
+
 namespace scrust_hip {
 
 constexpr int kTileRefs = 64;  // references per LDS tile (4 column blocks of 16)
@@ -228,10 +229,6 @@ struct DmaTiles {
     u32x4* b0;
     u32x4* b1;
     u32x4* b2;
-    u32x4* f0;
-    u32x4* f1;
-    u32x4* f2;
-    uint32_t* need;
 };
 
 // Slots (16 bytes) of one LDS tile buffer.
@@ -245,44 +242,39 @@ constexpr int kTileSlots = kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max)
 template <int OP, int NCH>
 constexpr bool kPassDma = OP == kOperandBf16 && 3 * kTileSlots<OP, NCH> * 16 <= 80 * 1024;
 
-// Split tiles of the two-stage bf16 pass: a tile's coarse part (the first kc MFMA steps of
-// every row, all a block needs unless its coarse scores come near a row's append bound) and
-// its fine part sit in separate LDS buffers, so the scan streams 4 kc chunks per reference row
-// and loads the 4 kf fine chunks of a tile only while its rows' blocks need them. The coarse
+// Split tiles of the two-stage bf16 pass (the persistent pass): the scan streams only a
+// tile's coarse part (the first kc MFMA steps of every row, all a block needs unless its
+// coarse scores come near a row's append bound) through LDS, 4 kc chunks per reference row; a
+// block that needs more reads its 4 kf fine chunks per row from global memory. The coarse
 // buffer has no padding: lane group g's step c of row r is at slot
 // r 4 kc + ((g kc + c) ^ coarse_swizzle(r)). gfx950 serves a ds_read_b128 in four passes of
 // 16 lanes, lanes 8j..8j+7 with 8j+16..8j+23, each over 64 four-byte banks (16 slots of 16
 // bytes): a read is free of bank conflicts when each pass hits 16 distinct slots modulo 16.
-// (This reading fits every layout measured: the padded full tiles, 0 conflict cycles; a
+// (This reading fits every layout measured: the padded whole tiles, 0 conflict cycles; a
 // one-slot pad, 2-way; a swizzle by row bits 2 and 3 alone, 4 cycles per read.) The swizzle
 // below takes row bits 1-3 so that it also keeps the 32 lanes of a half-wave on 32 distinct
-// slots modulo 32. The fine buffer is padded like the full tiles (lds_stride).
+// slots modulo 32.
 template <int KC>
 __host__ __device__ constexpr int coarse_swizzle(int r) {
     return KC == 2 ? (((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 2)) : ((((r >> 2) ^ (r >> 3)) & 1) << 1);
 }
 template <int NCH>
 constexpr int kCoarseSlots = kTileRefs * 4 * bf16_coarse_chunks(NCH);
-template <int NCH>
-constexpr int kFineSlots = kTileRefs * lds_stride(NCH - bf16_coarse_chunks(NCH));
+// Three ring slots of two tiles' coarse parts in half of a CU's LDS (two workgroups per CU).
 template <int OP, int NCH>
 constexpr bool kPassSplit = OP == kOperandBf16 && bf16_coarse_chunks(NCH) < NCH && bf16_coarse_chunks(NCH) <= 2 &&
-                            3 * (kCoarseSlots<NCH> + kFineSlots<NCH>) * 16 + 8 <= 80 * 1024;
+                            3 * 2 * kCoarseSlots<NCH> * 16 + 8 <= 80 * 1024;
 // The host's split_pass_nch names the chunk counts the split variants exist for.
 static_assert(!kPassSplit<kOperandBf16, 1> && kPassSplit<kOperandBf16, 2> && kPassSplit<kOperandBf16, 3> &&
                   kPassSplit<kOperandBf16, 4> && kPassSplit<kOperandBf16, 5> && !kPassSplit<kOperandBf16, 6>,
               "split_pass_nch must match kPassSplit");
-// A tile's fine part is loaded while some wave of the workgroup refined a block in one of
-// the last kHotTiles tiles (rows meet their refinements in runs: the tiles of their own
-// cluster); a block that needs the fine part of a tile loaded without it reads it from
-// global memory.
-constexpr int kHotTiles = 3;
 
 // Persistent pass: the end of a work item. Each wave selects, row after row, the kp smallest
 // keys (unordered) of its 64 rows' buffers (row q's four segment counts in c(q)) exactly as
 // select_kernel does: a whole block's rows into cand with their kp-th score in theta, a range
-// item's rows (splits > 1, rows from tail_row0 on) into their range's place in tail_cand. A row with fewer than kp keys gets all
-// of them, then kNoKey, and theta +inf. `empty`: no keys (a dataset that skipped the scan).
+// item's rows (splits > 1, rows from tail_row0 on) into their range's place in tail_cand.
+// A row with fewer than kp keys gets all of them, then kNoKey, and theta +inf. `empty`: no
+// keys (a dataset that skipped the scan).
 template <int E, class Counts>
 __device__ __forceinline__ void finish_rows(const DatasetDesc& s, const PassItem& item,
                                             const SCRUST_GLOBAL uint64_t* rbuf, uint32_t row0,
@@ -577,8 +569,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // skipped, and the buffers end exactly as with every step taken.
     constexpr int kCoarse = bf16_coarse_chunks(NCH);
     constexpr bool kTwoStage = OP == kOperandBf16 && FIXED && kCoarse < NCH;
-    // Split tiles: coarse and fine parts in separate buffers (kPassSplit).
-    constexpr bool kSplit = kTwoStage && DMA && kPassSplit<OP, NCH>;
+    // Split tiles in pairs: the persistent pass (kPassSplit, PAIRS).
+    constexpr bool kSplit = kTwoStage && DMA && kPassSplit<OP, NCH> && PAIRS;
     // gate[qb] = phi[qb] - skip_b[qb], the coarse bound of the block test; it changes only
     // when phi does (compaction).
     float skip_b[4], gate[4];
@@ -835,36 +827,23 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             compact();
         }
     } else if constexpr (kSplit) {
-        // Split tiles, loaded straight into LDS like the full tiles below: the coarse part of
-        // tile t+2 (and its fine part while the workgroup is hot) goes into the ring slot tile
-        // t-1 left while tile t is consumed; each wave waits for all its loads before a plain
-        // s_barrier. After each tile every wave publishes whether it refined a block, so all
-        // waves take the same decision for the next fine load.
-        // With PAIRS a step is a pair of tiles (128 references, eight blocks) and there is no
-        // fine ring: every refined block reads its fine chunks from global memory.
+        // Split tiles in pairs, loaded straight into LDS like the whole tiles below: the coarse
+        // parts of tiles t+4 and t+5 go into the ring slot tiles t-2 and t-1 left while tiles
+        // t and t+1 are consumed (eight blocks between barriers); each wave waits for all its
+        // loads before a plain s_barrier. A refined block reads its fine chunks from global
+        // memory.
         typedef typename Op::Frag Frag;
         constexpr int KC = kCoarse, KF = NCH - kCoarse;
-        constexpr int kStep = PAIRS ? 2 : 1;
-        constexpr int kFStride = lds_stride(KF), kFGStride = lds_gstride(KF);
+        constexpr int kStep = 2;
         constexpr int kCRounds = KC * kStep;  // 4 KC wave-wide loads per tile, one per wave per round
-        constexpr int kFRounds = (kFineSlots<NCH> + 4 * kWave - 1) / (4 * kWave);
         // Sources (chunk index from the step's first tile) of this lane's slots in each round.
-        int csrc[kCRounds], fsrc[kFRounds];
-        bool fon[kFRounds];
+        int csrc[kCRounds];
 #pragma unroll
         for (int i = 0; i < kCRounds; i++) {
             const int u = (i * 4 + w) * kWave + lane;
             const int j = u / kCoarseSlots<NCH>, v = u % kCoarseSlots<NCH>;
             const int r = v / (4 * KC), p = v % (4 * KC);
             csrc[i] = j * kTileRefs * 4 * NCH + r * 4 * KC + (p ^ coarse_swizzle<KC>(r));
-        }
-#pragma unroll
-        for (int i = 0; i < kFRounds; i++) {
-            const int u = (i * 4 + w) * kWave + lane;
-            fon[i] = u < kFineSlots<NCH>;
-            const int v = fon[i] ? u : 0;
-            const int r = v / kFStride, q = v % kFStride, gg = q / kFGStride, c = q % kFGStride;
-            fsrc[i] = kTileRefs * 4 * KC + r * 4 * KF + ((gg < 4 && c < KF) ? gg * KF + c : 0);
         }
         // This lane's coarse slots within a row (rows cb 16 + lc share lc's swizzle).
         int cofs[KC];
@@ -873,38 +852,21 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         auto coarse_of = [&](int cur) __attribute__((always_inline)) -> u32x4* {
             return cur == 0 ? dma.b0 : cur == 1 ? dma.b1 : dma.b2;
         };
-        auto fine_of = [&](int cur) __attribute__((always_inline)) -> u32x4* {
-            return cur == 0 ? dma.f0 : cur == 1 ? dma.f1 : dma.f2;
-        };
-        auto lds_load = [&](const SCRUST_GLOBAL u32x4* from, u32x4* to) __attribute__((always_inline)) {
-            __builtin_amdgcn_global_load_lds(from, (__attribute__((address_space(3))) void*)to, 16, 0, 0);
-        };
-        // Tile T's coarse part into ring slot S, and its fine part while hot.
-        uint32_t have = 0u;  // bit S: ring slot S holds its tile's fine part
-        int hot = kHotTiles;
+        // The coarse parts of tiles T and T+1 into ring slot S.
         auto issue = [&](uint32_t T, auto slot) __attribute__((always_inline)) {
             constexpr int S = decltype(slot)::value;
             const SCRUST_GLOBAL u32x4* from = ref_rows + static_cast<size_t>(phys(T)) * tile_u4;
 #pragma unroll
-            for (int i = 0; i < kCRounds; i++) lds_load(from + csrc[i], coarse_of(S) + (i * 4 + w) * kWave);
-            if constexpr (PAIRS) return;
-            if (hot > 0) {
-#pragma unroll
-                for (int i = 0; i < kFRounds; i++)
-                    if (fon[i]) lds_load(from + fsrc[i], fine_of(S) + (i * 4 + w) * kWave);
-                have |= 1u << S;
-            } else {
-                have &= ~(1u << S);
-            }
+            for (int i = 0; i < kCRounds; i++)
+                __builtin_amdgcn_global_load_lds(from + csrc[i],
+                                                 (__attribute__((address_space(3))) void*)(coarse_of(S) + (i * 4 + w) * kWave),
+                                                 16, 0, 0);
         };
-        // Tile t (physical index; with PAIRS tiles t and t+1) from ring slot cur: the coarse
-        // steps of its blocks, then the remaining steps and the filter of each block that may
-        // append (the fine chunks from the ring slot, or from global memory when the slot was
-        // loaded cold). Returns whether any block of the wave was refined.
-        auto consume_split = [&](uint32_t t, int cur, bool fine_here) __attribute__((always_inline)) {
+        // Tiles t and t+1 (physical indices) from ring slot cur: the coarse steps of their
+        // blocks, then the remaining steps and the filter of each block that may append.
+        auto consume_pair = [&](uint32_t t, int cur) __attribute__((always_inline)) {
             f32x4 acc0[4], acc1[4];
             Frag c0[KC], c1[KC], f[KF];
-            bool refined = false;
             auto load_c = [&](Frag (&b)[KC], int cb) __attribute__((always_inline)) {
                 const u32x4* row = coarse_of(cur) + (cb * 16 + lc) * (4 * KC);
 #pragma unroll
@@ -927,17 +889,10 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
                     need |= __ballot(m > gate[qb]);
                 }
                 if (need) {
-                    refined = true;
-                    if (!PAIRS && fine_here) {
-                        const u32x4* row = fine_of(cur) + (cb * 16 + lc) * kFStride + g * kFGStride;
+                    const SCRUST_GLOBAL u32x4* src = ref_rows + static_cast<size_t>(t + cb / 4) * tile_u4 +
+                                                     kTileRefs * 4 * KC + ((cb % 4) * 16 + lc) * 4 * KF + g * KF;
 #pragma unroll
-                        for (int c = 0; c < KF; c++) f[c] = __builtin_bit_cast(Frag, row[c]);
-                    } else {
-                        const SCRUST_GLOBAL u32x4* src = ref_rows + static_cast<size_t>(t + cb / 4) * tile_u4 +
-                                                         kTileRefs * 4 * KC + ((cb % 4) * 16 + lc) * 4 * KF + g * KF;
-#pragma unroll
-                        for (int c = 0; c < KF; c++) f[c] = __builtin_bit_cast(Frag, src[c]);
-                    }
+                    for (int c = 0; c < KF; c++) f[c] = __builtin_bit_cast(Frag, src[c]);
 #pragma unroll
                     for (int c = KC; c < NCH; c++)
 #pragma unroll
@@ -945,8 +900,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
                     filter(acc, t, cb);
                 }
             };
-            // Block cb's MFMAs run while block cb-1 is tested; block cb+1's fragments are read
-            // while block cb-1's MFMAs run.
+            // Block cb's MFMAs run while block cb-1 is tested; block cb+2's fragments are read
+            // while block cb's MFMAs run.
             load_c(c0, 0);
             load_c(c1, 1);
             __builtin_amdgcn_sched_barrier(0);
@@ -972,35 +927,14 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             });
             refine(acc0, 4 * kStep - 2);
             refine(acc1, 4 * kStep - 1);
-            return refined;
         };
         auto settle = [&]() __attribute__((always_inline)) {
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_s_barrier();
         };
-        // Flags of parity p: byte w of need[p] is wave w's; a parity is written again only
-        // after the barrier that follows every wave's read of it. LDS pointers: through a
-        // generic pointer the accesses are FLAT ones (A/B at 1M x 50: the pass 2.6% slower).
-        typedef __attribute__((address_space(3))) volatile uint8_t LdsByte;
-        typedef __attribute__((address_space(3))) volatile uint32_t LdsWord;
-        LdsByte* const flag_bytes = (LdsByte*)(dma.need);
-        LdsWord* const flag_words = (LdsWord*)(dma.need);
-        int parity = 0;
-        // Tile t from ring slot S. Each wave waits for all its loads before the barrier, those
-        // of tile t+2 included: leaving them in flight one tile longer (a counted wait) made the
-        // pass 9% slower (A/B at 1M x 50, same box).
         auto step = [&](uint32_t t, auto slot) __attribute__((always_inline)) {
-            constexpr int S = decltype(slot)::value;
-            const bool refined = consume_split(phys(t), S, ((have >> S) & 1u) != 0u);
-            if constexpr (PAIRS) {
-                settle();
-            } else {
-                if (lane == 0) flag_bytes[parity * 4 + w] = refined ? 1 : 0;
-                settle();
-                const uint32_t any = __builtin_amdgcn_readfirstlane(flag_words[parity]);
-                parity ^= 1;
-                hot = any ? kHotTiles : (hot > 0 ? hot - 1 : 0);
-            }
+            consume_pair(phys(t), decltype(slot)::value);
+            settle();
             compact();
         };
         const std::integral_constant<int, 0> s0{};
@@ -1170,71 +1104,59 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
     const uint32_t blocks = s.npad / kPassRows;
     const uint32_t tail_block = s.tail_row0 / kPassRows;
     if constexpr (kPassSplit<OP, NCH> && PAIRS) {
+        // The persistent pass (s.persist): workgroup b runs on XCD x = b mod 8 (round-robin
+        // dispatch) and takes the blocks of x's contiguous run from queue x (the spatial order
+        // keeps a run's blocks, which share their scans' first tiles, in one L2), then the
+        // range items from queue 8. Every workgroup leaves once both are drained.
         __shared__ u32x4 b0[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b1[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b2[2 * kCoarseSlots<NCH>];
-        const DmaTiles dma{b0, b1, b2, nullptr, nullptr, nullptr, nullptr};
-        if (s.persist) {
-            // Persistent: workgroup b runs on XCD x = b mod 8 (round-robin dispatch) and takes
-            // the blocks of x's contiguous run from queue x (the spatial order keeps a run's
-            // blocks, which share their scans' first tiles, in one L2), then the last round's
-            // range items from queue 8. Every workgroup leaves once both are drained.
-            // Whole blocks are [0, whole): every block with one range, the blocks before the
-            // last round's when that scans in ranges, none when every block does (items v < whole
-            // are whole blocks under single_item's reading, the items from whole on ranges).
-            __shared__ uint32_t next;
-            const uint32_t whole = s.splits == 1 ? blocks : tail_block;
-            const uint32_t x = blockIdx.x % 8u;
-            const uint32_t base = whole / 8u, extra = whole % 8u;
-            const uint32_t start = x * base + (x < extra ? x : extra), len = base + (x < extra ? 1u : 0u);
-            const uint32_t tail_items = (blocks - whole) * s.splits;
-            for (;;) {
-                if (threadIdx.x == 0) {
-                    uint32_t v = ~0u;
-                    const uint32_t j = atomicAdd(&s.queue[x], 1u);
-                    if (j < len) {
-                        v = start + j;
-                    } else {
-                        const uint32_t gi = atomicAdd(&s.queue[8], 1u);
-                        if (gi < tail_items) v = whole + gi;
-                    }
-                    next = v;
+        __shared__ uint32_t next;
+        const DmaTiles dma{b0, b1, b2};
+        // Whole blocks are [0, whole): every block with one range, the blocks before the
+        // last round's when that scans in ranges, none when every block does (items v < whole
+        // are whole blocks under single_item's reading, the items from whole on ranges).
+        const uint32_t whole = s.splits == 1 ? blocks : tail_block;
+        const uint32_t x = blockIdx.x % 8u;
+        const uint32_t base = whole / 8u, extra = whole % 8u;
+        const uint32_t start = x * base + (x < extra ? x : extra), len = base + (x < extra ? 1u : 0u);
+        const uint32_t tail_items = (blocks - whole) * s.splits;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                uint32_t v = ~0u;
+                const uint32_t j = atomicAdd(&s.queue[x], 1u);
+                if (j < len) {
+                    v = start + j;
+                } else {
+                    const uint32_t gi = atomicAdd(&s.queue[8], 1u);
+                    if (gi < tail_items) v = whole + gi;
                 }
-                __syncthreads();
-                const uint32_t v = next;
-                __syncthreads();
-                if (v == ~0u) break;
-                pass_block<OP, NCH, true, true, true>(s, single_item(s, v), nullptr, dma);
-                __syncthreads();
+                next = v;
             }
-            return;
+            __syncthreads();
+            const uint32_t v = next;
+            __syncthreads();
+            if (v == ~0u) break;
+            pass_block<OP, NCH, true, true, true>(s, single_item(s, v), nullptr, dma);
+            __syncthreads();
         }
-        const uint32_t v = blockIdx.x < tail_block && s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x;
-        pass_block<OP, NCH, true, true, true>(s, single_item(s, v), nullptr, dma);
         return;
-    }
-    // Workgroups below the tail block scan their block's whole range; from it on, workgroup
-    // tail_block + g scans range j = g / tail_blocks of block tail_block + g % tail_blocks:
-    // the workgroups resident together scan the same references (shared through L2).
-    const PassItem item =
-        single_item(s, blockIdx.x < tail_block && s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x);
-    if constexpr (kPassSplit<OP, NCH>) {
-        __shared__ u32x4 b0[kCoarseSlots<NCH>];
-        __shared__ u32x4 b1[kCoarseSlots<NCH>];
-        __shared__ u32x4 b2[kCoarseSlots<NCH>];
-        __shared__ u32x4 f0[kFineSlots<NCH>];
-        __shared__ u32x4 f1[kFineSlots<NCH>];
-        __shared__ u32x4 f2[kFineSlots<NCH>];
-        __shared__ uint32_t need[2];
-        pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2, f0, f1, f2, need});
-    } else if constexpr (kPassDma<OP, NCH>) {
-        __shared__ u32x4 b0[kTileSlots<OP, NCH>];
-        __shared__ u32x4 b1[kTileSlots<OP, NCH>];
-        __shared__ u32x4 b2[kTileSlots<OP, NCH>];
-        pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2, nullptr, nullptr, nullptr, nullptr});
     } else {
-        __shared__ PassSmem<OP, NCH> sm;
-        pass_block<OP, NCH, true>(s, item, &sm);
+        // One work item per workgroup, whole tiles. Workgroups below the tail block scan
+        // their block's whole range; from it on, workgroup tail_block + g scans range
+        // j = g / tail_blocks of block tail_block + g % tail_blocks: the workgroups resident
+        // together scan the same references (shared through L2).
+        const PassItem item =
+            single_item(s, blockIdx.x < tail_block && s.rotate ? xcd_block(blockIdx.x, tail_block) : blockIdx.x);
+        if constexpr (kPassDma<OP, NCH>) {
+            __shared__ u32x4 b0[kTileSlots<OP, NCH>];
+            __shared__ u32x4 b1[kTileSlots<OP, NCH>];
+            __shared__ u32x4 b2[kTileSlots<OP, NCH>];
+            pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2});
+        } else {
+            __shared__ PassSmem<OP, NCH> sm;
+            pass_block<OP, NCH, true>(s, item, &sm);
+        }
     }
 }
 
@@ -1456,15 +1378,18 @@ template <int OP, int NCH>
 static void run_pass(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream) {
     const unsigned tail_block = host.tail_row0 / kPassRows;
     const unsigned groups = tail_block + (host.npad / kPassRows - tail_block) * host.splits;
-    // Split tiles scan in pairs of tiles when a compaction leaves room for 32 appends per
-    // segment (buffers of 256 or 512 slots). Measured at 1M x 50 (A/B, same box): pairs 6.5%
-    // faster than single tiles with the fine ring; pairs in a ring of two slots 3.5% slower,
-    // and steps of four tiles (compactions inside the step) 5% slower, than pairs in three.
+    // The persistent pass scans split tiles in pairs (its plans have buffers of 256 or 512
+    // slots, room for the 32 appends per segment of a pair). Measured at 1M x 50 (A/B, same
+    // box): pairs 6.5% faster than single split tiles with a ring of fine parts; pairs in a
+    // ring of two slots 3.5% slower, and steps of four tiles (compactions inside the step) 5%
+    // slower, than pairs in three. Every other single plan scans whole tiles in two stages:
+    // at 2,638 and 100,000 cells and for server requests that is faster than split tiles
+    // (A/B: 0.40 against 0.41 ms single-tile and 0.55 ms paired at 2,638; serving p99 58-65
+    // against 82-109 ms), since those datasets refine most blocks.
     if constexpr (kPassSplit<OP, NCH>) {
-        if (host.cap >= 256) {
+        if (host.persist) {
             // The persistent pass launches one workgroup per slot.
-            hipLaunchKernelGGL((pass_kernel<OP, NCH, true>), dim3(host.persist ? host.slots : groups), dim3(256), 0,
-                               stream, set);
+            hipLaunchKernelGGL((pass_kernel<OP, NCH, true>), dim3(host.slots), dim3(256), 0, stream, set);
             return;
         }
     }

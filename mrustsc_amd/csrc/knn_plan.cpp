@@ -77,7 +77,7 @@ uint32_t partition_even(uint32_t set, uint32_t blocks, uint32_t pairs, uint32_t 
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                               const scrust_hip_knn_options* options, Plan& plan,
                               int compute_units, bool single, const TuneEntry* force,
-                              const BatchKnobs* knobs) {
+                              const BatchKnobs* knobs, bool shared) {
     plan = Plan{};
     if (single && specs.size() != 1)
         return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: a single call lays out exactly one dataset");
@@ -276,9 +276,11 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // Persistent pass: a single dataset on split tiles in pairs whose blocks outnumber
             // the workgroup slots. Its candidate buffers are per slot (kPassRows rows each)
             // instead of per row, the pass selects each whole block's candidates itself, and
-            // rows scanned in ranges keep each range's top kp for merge_kernel.
-            const bool persist = single && !s.local && split_pass_nch(plan.operand, s.nch) && s.cap >= 256 &&
-                                 blocks > slots;
+            // rows scanned in ranges keep each range's top kp for merge_kernel. Not on a
+            // shared GPU (server requests): its workgroups hold every slot until the launch
+            // ends, where the other requests' kernels would wait for them.
+            const bool persist = single && !shared && !s.local && split_pass_nch(plan.operand, s.nch) &&
+                                 s.cap >= 256 && blocks > slots;
             if (persist) {
                 s.persist = 1;
                 s.slots = static_cast<uint32_t>(slots);

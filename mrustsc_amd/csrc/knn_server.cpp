@@ -394,7 +394,7 @@ scrust_hip_status submit_request(scrust_hip_server* server, const float* embeddi
     // tuning table cannot outgrow the region admission hands out. Input and output
     // addresses are set at admission.
     std::vector<SetSpec> specs{SetSpec{embedding, n, d, k, nullptr, nullptr}};
-    st = plan_layout(specs, options, r->plan, server->compute_units, true);
+    st = plan_layout(specs, options, r->plan, server->compute_units, true, nullptr, nullptr, true);
     if (st != SCRUST_HIP_OK) return st;
     r->arena_bytes = r->plan.arena_bytes;
     r->meta_bytes = r->plan.meta_bytes;

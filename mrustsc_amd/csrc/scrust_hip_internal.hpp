@@ -128,7 +128,7 @@ bool device_is_gfx950(int device, hipDeviceProp_t* props_out);
 scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                               const scrust_hip_knn_options* options, Plan& plan,
                               int compute_units, bool single, const TuneEntry* force = nullptr,
-                              const BatchKnobs* knobs = nullptr);
+                              const BatchKnobs* knobs = nullptr, bool shared = false);
 // Points the plan's descriptors into `arena` and `meta` (sized arena_bytes / meta_bytes)
 // and uploads descriptors, items and the exact-row list on `stream`. With `pinned` (host
 // memory of at least meta_bytes that stays untouched until the copy completes) the upload
