@@ -161,11 +161,22 @@ struct DatasetDesc {
     uint32_t slots;
     uint32_t* queue;
     uint64_t* tail_cand;
+    // Bounding balls of the persistent pass (pass order, coordinates): ball b < pairs is tile
+    // pair b (reference rows 128 b .. 128 b + 127), ball pairs + j query block j (rows 256 j ..
+    // 256 j + 255); centre b at balls + b * ball_stride (d floats, zero to ball_stride), then
+    // (pairs + blocks) radii, each an upper bound of the distance of the ball's rows from its
+    // centre. Null: no step is skipped by the bound.
+    float* balls;
+    uint32_t ball_stride;
 };
 
 // Split-tile bf16 pass variants (the chunk counts knn_pass.hip compiles them for).
 inline bool split_pass_nch(int operand, uint32_t nch) { return operand == kOperandBf16 && nch >= 2 && nch <= 5; }
-constexpr int kQueueWords = 9;  // persistent pass: 8 XCD queues and the range-item queue
+// Persistent pass: 8 XCD queues, the range-item queue, and the count of tile-pair steps its
+// items scanned (the others skipped by the ball bound).
+constexpr int kQueueWords = 10;
+constexpr int kQueueScanned = 9;
+constexpr uint32_t kBallMaxDims = 64;  // widest dataset the persistent pass keeps balls for
 
 // Row offset of reference range j's buffers and segment counts: range 0 holds every row,
 // a range j >= 1 only the rows from tail_row0 on (every row when tail_row0 is 0), so row r
@@ -238,6 +249,8 @@ bool launch_pass_batch(const DatasetDesc* sets, const PassItem* items, uint32_t 
 // selection width present; `widths` is a bit set of the select_e values 2, 4, 8, 16).
 // Persistent pass (host.persist): the last round's rows from their per-range lists.
 void launch_merge(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream);
+// Bounding balls of the persistent pass's tile pairs and query blocks (DatasetDesc::balls).
+void launch_balls(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream);
 void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned widths,
                    hipStream_t stream);
 // Rerank of every pass row (one launch per candidate width present; `kps` is a bit set of

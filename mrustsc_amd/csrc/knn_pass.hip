@@ -229,6 +229,9 @@ struct DmaTiles {
     u32x4* b0;
     u32x4* b1;
     u32x4* b2;
+    // Persistent pass: the ball bound's LDS (kBallLds floats: the waves' thetas in two
+    // barrier-parity halves, the block's radius, its centre).
+    float* ball;
 };
 
 // Slots (16 bytes) of one LDS tile buffer.
@@ -260,6 +263,10 @@ __host__ __device__ constexpr int coarse_swizzle(int r) {
 }
 template <int NCH>
 constexpr int kCoarseSlots = kTileRefs * 4 * bf16_coarse_chunks(NCH);
+// The persistent pass's ball-bound LDS (floats): the waves' thetas in two halves of 4 (by
+// barrier parity) at 0, the block's radius at 8, its centre (at most kBallMaxD floats) at 16.
+constexpr int kBallMaxD = static_cast<int>(kBallMaxDims);
+constexpr int kBallLds = 16 + kBallMaxD;
 // Three ring slots of two tiles' coarse parts in half of a CU's LDS (two workgroups per CU).
 template <int OP, int NCH>
 constexpr bool kPassSplit = OP == kOperandBf16 && bf16_coarse_chunks(NCH) < NCH && bf16_coarse_chunks(NCH) <= 2 &&
@@ -723,16 +730,18 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     };
     // Compacts every row of this wave with a segment that could overflow in the next tile
     // (compact_row): the append bound tightens to the kept keys' cut-off score.
-    auto compact = [&]() __attribute__((always_inline)) {
+    // Returns whether any row was compacted.
+    auto compact = [&]() __attribute__((always_inline)) -> bool {
         uint64_t full = 0;
         static_for<0, 4>([&](auto QB) {
             const uint64_t f = __ballot(cnt[QB] > trigger);
             full |= ((f | f >> 16 | f >> 32 | f >> 48) & 0xFFFFull) << (16 * QB);
         });
-        if (!full) return;
+        if (!full) return false;
         if (cap <= 128) compact_rows(std::integral_constant<int, 2>{}, full);
         else if (cap <= 256) compact_rows(std::integral_constant<int, 4>{}, full);
         else compact_rows(std::integral_constant<int, 8>{}, full);
+        return true;
     };
 
     if constexpr (OP == kOperandF16Local) {
@@ -928,34 +937,141 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             refine(acc0, 4 * kStep - 2);
             refine(acc1, 4 * kStep - 1);
         };
+        // Ball bound (s.balls): the step of tile pair P is skipped when no row q of the block
+        // can append a reference r of P. A row appends only pairs whose computed score
+        // exceeds its bound phi_q, and the computed score lies within skip_b_q of
+        // q.r - |r|^2 / 2 (the deviation the block test allows for the fine steps alone),
+        // so a pair with |q - r|^2 > theta_q = |q|^2 - 2 phi_q + 4 skip_b_q cannot append;
+        // and |q - r| >= |c_B - c_P| - rho_B - rho_P for the block's and the pair's balls. A
+        // skipped step would have appended nothing, and its compaction test found nothing,
+        // so the buffers end as with every step scanned. theta_q only falls during the scan:
+        // a test made with an earlier block maximum stays valid. Each wave keeps its rows'
+        // largest theta_q (wtheta, refreshed after its compactions) and writes it before
+        // each barrier into the half of the ball LDS of that barrier's parity; after the
+        // barrier every wave reads the same four values (theta_max), and so computes the same
+        // survivors and scans the same steps.
+        const bool prune = s.balls != nullptr;
+        float* const blds = dma.ball;
+        const SCRUST_GLOBAL float* const balls = global_ptr(s.balls);
+        const uint32_t bstride = s.ball_stride;
+        const uint32_t ball_pairs = (n + 2 * kTileRefs - 1) / (2 * kTileRefs);
+        const uint32_t ball_count = ball_pairs + s.npad / kPassRows;
+        if (prune) {
+            // The block's ball into LDS; the first read of it follows the first barrier.
+            const SCRUST_GLOBAL float* cb = balls + static_cast<size_t>(ball_pairs + item.block) * bstride;
+            if (tid < static_cast<int>(bstride)) blds[16 + tid] = cb[tid];
+            if (tid == 0) blds[8] = balls[static_cast<size_t>(ball_count) * bstride + ball_pairs + item.block];
+        }
+        float wtheta = kInf, theta_max = kInf;
+        uint32_t par = 0;
+        auto refresh_theta = [&]() __attribute__((always_inline)) {
+            float t = 0.0f;
+#pragma unroll
+            for (int qb = 0; qb < 4; qb++) {
+                uint32_t row = qbase + w * kWave + qb * 16 + lc;
+                if (row >= n) row = n - 1;
+                const float nq = global_ptr(s.norms)[row];
+                const float th = nq - 2.0f * phi[qb] + 4.0f * skip_b[qb];
+                // 2^-12 of the terms' magnitudes covers the rounding of th many times over.
+                t = __builtin_fmaxf(t, th + 0x1p-12f * (nq + 2.0f * __builtin_fabsf(phi[qb]) + 4.0f * skip_b[qb]));
+            }
+#pragma unroll
+            for (int m = 1; m < kWave; m <<= 1) t = __builtin_fmaxf(t, __shfl_xor(t, m, kWave));
+            wtheta = t;
+        };
         auto settle = [&]() __attribute__((always_inline)) {
+            if (prune && lane == 0) blds[par * 4 + w] = wtheta;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            if (prune) {
+                const float* h = blds + par * 4;
+                theta_max = __builtin_fmaxf(__builtin_fmaxf(h[0], h[1]), __builtin_fmaxf(h[2], h[3]));
+            }
+            par ^= 1u;
         };
+        uint32_t scanned = 0;
         auto step = [&](uint32_t t, auto slot) __attribute__((always_inline)) {
             consume_pair(phys(t), decltype(slot)::value);
+            scanned++;
             settle();
-            compact();
+            if (compact() && prune) refresh_theta();
+        };
+        // Survivors among scan pairs [wb, wb + 64) (lane l tests pair wb + l): every pair of
+        // the range while theta_max is infinite (some row has no bound yet).
+        const uint32_t pb = t_begin / 2, pe = t_end / 2;
+        auto window = [&](uint32_t wb) __attribute__((always_inline)) -> uint64_t {
+            const uint32_t sp = wb + static_cast<uint32_t>(lane);
+            bool keep = sp < pe;
+            if (prune && theta_max < kInf && keep) {
+                const uint32_t P = phys(2 * sp) >> 1;
+                const float4* c = reinterpret_cast<const float4*>(s.balls + static_cast<size_t>(P) * bstride);
+                const float4* o = reinterpret_cast<const float4*>(blds + 16);
+                float d2 = 0.0f;
+#pragma unroll 1
+                for (uint32_t j = 0; j < bstride / 4; j++) {
+                    const float4 a4 = c[j], b4 = o[j];
+                    const float e0 = a4.x - b4.x, e1 = a4.y - b4.y, e2 = a4.z - b4.z, e3 = a4.w - b4.w;
+                    d2 = __builtin_fmaf(e0, e0, d2);
+                    d2 = __builtin_fmaf(e1, e1, d2);
+                    d2 = __builtin_fmaf(e2, e2, d2);
+                    d2 = __builtin_fmaf(e3, e3, d2);
+                }
+                const float rho = balls[static_cast<size_t>(ball_count) * bstride + P] + blds[8];
+                // Rounding of d2 (at most (d + 3) 2^-24 relative), of the square root and of
+                // the differences: the 2^-12 factors leave the bound on the safe side.
+                const float lb = __builtin_sqrtf(d2) * (1.0f - 0x1p-12f) - rho * (1.0f + 0x1p-12f);
+                keep = !(lb > 0.0f && lb * lb * (1.0f - 0x1p-12f) > theta_max);
+            }
+            return __ballot(keep);
+        };
+        uint32_t wb = pb;
+        uint64_t wm = window(pb);
+        constexpr uint32_t kNone = ~0u;
+        auto next = [&]() __attribute__((always_inline)) -> uint32_t {
+            while (wm == 0) {
+                wb += kWave;
+                if (wb >= pe) return kNone;
+                wm = window(wb);
+            }
+            const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(wm));
+            wm &= wm - 1;
+            return wb + b;
         };
         const std::integral_constant<int, 0> s0{};
         const std::integral_constant<int, 1> s1{};
         const std::integral_constant<int, 2> s2{};
-        // Steps of kStep tiles (t_begin and t_end are even: ranges hold whole pairs).
-        constexpr uint32_t D = kStep;
-        issue(t_begin, s0);
-        if (t_begin + D < t_end) issue(t_begin + D, s1);
-        settle();
-        for (uint32_t t = t_begin;;) {
-            if (t + 2 * D < t_end) issue(t + 2 * D, s2);
-            step(t, s0);
-            if ((t += D) >= t_end) break;
-            if (t + 2 * D < t_end) issue(t + 2 * D, s0);
-            step(t, s1);
-            if ((t += D) >= t_end) break;
-            if (t + 2 * D < t_end) issue(t + 2 * D, s1);
-            step(t, s2);
-            if ((t += D) >= t_end) break;
+        // Steps over the surviving pairs q0, q1, q2, ... in scan order (tile 2 q and 2 q + 1):
+        // the coarse parts of the pair two steps ahead are loaded while a step is consumed.
+        uint32_t q0 = next();
+        if (q0 != kNone) {
+            uint32_t q1 = next();
+            issue(2 * q0, s0);
+            if (q1 != kNone) issue(2 * q1, s1);
+            settle();
+            for (;;) {
+                uint32_t q2 = q1 != kNone ? next() : kNone;
+                if (q2 != kNone) issue(2 * q2, s2);
+                step(2 * q0, s0);
+                if (q1 == kNone) break;
+                q0 = q1;
+                q1 = q2;
+                q2 = q1 != kNone ? next() : kNone;
+                if (q2 != kNone) issue(2 * q2, s0);
+                step(2 * q0, s1);
+                if (q1 == kNone) break;
+                q0 = q1;
+                q1 = q2;
+                q2 = q1 != kNone ? next() : kNone;
+                if (q2 != kNone) issue(2 * q2, s1);
+                step(2 * q0, s2);
+                if (q1 == kNone) break;
+                q0 = q1;
+                q1 = q2;
+            }
         }
+        if (tid == 0) atomicAdd(&s.queue[kQueueScanned], scanned);
     } else if constexpr (!DMA) {
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
     // merge into the loop head, where the compiler must then assume the prefetch just
@@ -1097,8 +1213,10 @@ __device__ __forceinline__ PassItem single_item(const DatasetDesc& s, uint32_t v
     return item;
 }
 
+// The persistent variant (PAIRS) is held to two waves per SIMD (256 registers a lane), which
+// its two workgroups per CU need.
 template <int OP, int NCH, bool PAIRS = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PAIRS ? 2 : 1, 2)))
 void pass_kernel(const DatasetDesc* __restrict__ set) {
     const DatasetDesc& s = *set;
     const uint32_t blocks = s.npad / kPassRows;
@@ -1111,8 +1229,9 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
         __shared__ u32x4 b0[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b1[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b2[2 * kCoarseSlots<NCH>];
+        __shared__ __attribute__((aligned(16))) float ball[kBallLds];
         __shared__ uint32_t next;
-        const DmaTiles dma{b0, b1, b2};
+        const DmaTiles dma{b0, b1, b2, ball};
         // Whole blocks are [0, whole): every block with one range, the blocks before the
         // last round's when that scans in ranges, none when every block does (items v < whole
         // are whole blocks under single_item's reading, the items from whole on ranges).
@@ -1212,6 +1331,60 @@ void launch_merge(const DatasetDesc* set, const DatasetDesc& host, hipStream_t s
         hipLaunchKernelGGL(merge_kernel<4>, dim3(blocks), dim3(256), 0, stream, set);
     else
         hipLaunchKernelGGL(merge_kernel<8>, dim3(blocks), dim3(256), 0, stream, set);
+}
+
+// Bounding balls of the persistent pass, one workgroup each: ball b < pairs holds reference
+// rows [128 b, 128 b + 128), ball pairs + j query rows [256 j, 256 j + 256), both cut at n
+// (rows past n are sentinels or clamped copies of row n - 1). The centre is the rows' mean
+// (any point would do: the bound only needs the radius to cover the rows' distances from the
+// stored centre); the radius is the largest f32 distance, scaled by 1 + 2^-12, which covers
+// the rounding of the d-term sum ((d + 3) 2^-24 relative) and of the square root.
+__global__ __launch_bounds__(256) void ball_kernel(const DatasetDesc* __restrict__ set) {
+    const DatasetDesc& s = *set;
+    const uint32_t n = s.n, d = s.d, stride = s.ball_stride;
+    const uint32_t pairs = (n + 2 * kTileRefs - 1) / (2 * kTileRefs);
+    const uint32_t count = pairs + s.npad / kPassRows;
+    const uint32_t b = blockIdx.x, tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+    const uint32_t r0 = b < pairs ? b * 2 * kTileRefs : (b - pairs) * kPassRows;
+    const uint32_t span = b < pairs ? 2 * kTileRefs : kPassRows;
+    const uint32_t r1 = r0 + span < n ? r0 + span : n;
+    const SCRUST_GLOBAL float* coords = global_ptr(s.coords);
+    SCRUST_GLOBAL float* balls = global_ptr(s.balls);
+    __shared__ float part[4][kBallMaxD];
+    __shared__ float centre[kBallMaxD];
+    __shared__ float wfar[4];
+    float sum = 0.0f;
+    if (lane < d)
+        for (uint32_t r = r0 + w; r < r1; r += 4) sum += coords[static_cast<size_t>(r) * d + lane];
+    part[w][lane] = sum;
+    __syncthreads();
+    if (tid < stride) {
+        const float c = tid < d ? (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]) / static_cast<float>(r1 - r0) : 0.0f;
+        centre[tid] = c;
+        balls[static_cast<size_t>(b) * stride + tid] = c;
+    }
+    __syncthreads();
+    float far = 0.0f;
+    if (r0 + tid < r1) {
+        const SCRUST_GLOBAL float* x = coords + static_cast<size_t>(r0 + tid) * d;
+        for (uint32_t j = 0; j < d; j++) {
+            const float e = x[j] - centre[j];
+            far = __builtin_fmaf(e, e, far);
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) far = __builtin_fmaxf(far, __shfl_xor(far, m, kWave));
+    if (lane == 0) wfar[w] = far;
+    __syncthreads();
+    if (tid == 0) {
+        far = __builtin_fmaxf(__builtin_fmaxf(wfar[0], wfar[1]), __builtin_fmaxf(wfar[2], wfar[3]));
+        balls[static_cast<size_t>(count) * stride + b] = __builtin_sqrtf(far) * (1.0f + 0x1p-12f);
+    }
+}
+
+void launch_balls(const DatasetDesc* set, const DatasetDesc& host, hipStream_t stream) {
+    const unsigned count = (host.n + 2 * kTileRefs - 1) / (2 * kTileRefs) + host.npad / kPassRows;
+    hipLaunchKernelGGL(ball_kernel, dim3(count), dim3(256), 0, stream, set);
 }
 
 // Local-origin fp16 pass of one dataset: workgroup i scans every reference for the 512

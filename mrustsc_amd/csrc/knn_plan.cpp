@@ -304,6 +304,13 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             }
             if (s.persist) {
                 s.buf = static_cast<uint64_t*>(take(static_cast<size_t>(s.slots) * kPassRows * s.cap * sizeof(uint64_t)));
+                // Bounding balls of the tile pairs and query blocks: in spatial order a block
+                // and a pair of one part of the order are compact, and a step whose pair lies
+                // out of reach of every row of the block is skipped.
+                if (plan.ordered && sp.d <= kBallMaxDims) {
+                    s.ball_stride = (static_cast<uint32_t>(sp.d) + 3u) / 4u * 4u;
+                    s.balls = static_cast<float*>(take((pairs + blocks) * (s.ball_stride + 1ull) * sizeof(float)));
+                }
                 if (s.splits > 1)
                     s.tail_cand = static_cast<uint64_t*>(
                         take(static_cast<size_t>(s.n - s.tail_row0) * s.splits * s.kp * sizeof(uint64_t)));
@@ -405,6 +412,7 @@ scrust_hip_status plan_place(Plan& plan, void* arena, void* meta, hipStream_t st
             fix(s.buf);
             fix(s.seg_counts);
             if (s.persist && s.splits > 1) fix(s.tail_cand);
+            if (s.balls) fix(s.balls);
             fix(s.cand);
             fix(s.theta);
             if (s.perm) {
@@ -475,6 +483,7 @@ scrust_hip_status plan_enqueue(const Plan& plan, bool single, int compute_units,
     if (plan.any_pass) {
         bool ok;
         if (single) {
+            if (plan.sets[0].balls) launch_balls(plan.d_sets, plan.sets[0], stream);
             ok = launch_pass(plan.d_sets, plan.sets[0], plan.operand, stream);
         } else {
             ok = true;
