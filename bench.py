@@ -429,7 +429,11 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
     walls, st = single.timed(args.steps, args.warmup, "bf16")
     sec = sum(walls) / len(walls)
     dev = stage_means(st)
-    flops = 2.0 * n * n * d
+    # The pass scores (query, reference) pairs at 2 d flops each: every pair of the padded
+    # rows but those of the tile-pair steps its ball bound skips (pairs_scored).
+    scored = int(st[-1]["pairs_scored"]) or n * n
+    flops = 2.0 * d * scored
+    brute = 2.0 * n * n * d  # the fp32 and fp16 passes below score every pair
     achieved = flops / (dev["pass_ms"] * 1e-3) / 1e12
     out = base_line(args, n / sec, sec * 1e3)
     out["config"] = {"workload": workload, "n_cells": n, "n_dims": d, "k": k}
@@ -447,9 +451,13 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
                        "kernel": f"pass_kernel<1, {(3 * d + 3 + 31) // 32}, ...> "
                                  "(bf16 two-stage pass, chunks of 32)",
                        "algorithmic_flops": flops,
-                       "algorithmic_flops_per_unit": "2 d per (query, reference) pair; n^2 pairs",
-                       # The MFMA work issued (coarse steps for every pair, the fine ones for
-                       # refined blocks), from the committed SQ_INSTS_MFMA reading.
+                       "algorithmic_flops_per_unit": "2 d per scored (query, reference) pair",
+                       "pairs_scored": scored,
+                       "pairs_brute_force": n * n,
+                       # 2 n^2 d over the pass time: the rate a scan of every pair would need.
+                       "brute_force_equivalent_tflops": 2.0 * n * n * d / (dev["pass_ms"] * 1e-3) / 1e12,
+                       # The MFMA work issued (coarse steps for every scored pair, the fine
+                       # ones for refined blocks), from the committed SQ_INSTS_MFMA reading.
                        "executed_tflops": executed_bf16_tflops(n, d, k, dev["pass_ms"])}
     got_ids, got_dist = single.result()
     fixture = name if args.n == 0 and args.d == 50 and args.k == 15 else None
@@ -461,7 +469,7 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
         walls32, st32 = single.timed(args.fp32_steps, 1, "fp32")
         sec32 = sum(walls32) / len(walls32)
         pass32 = stage_means(st32)["pass_ms"]
-        ach32 = flops / (pass32 * 1e-3) / 1e12
+        ach32 = brute / (pass32 * 1e-3) / 1e12
         ids32, dist32 = single.result()
         out["fp32"] = {"cells_per_s": n / sec32, "ms_per_step": sec32 * 1e3, "pass_ms": pass32,
                        "achieved_tflops": ach32, "roofline_frac": ach32 / FP32_MATRIX_PEAK_TFLOPS,
@@ -477,7 +485,7 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
         ids16, dist16 = single.result()
         dev16 = stage_means(st16)
         sec16 = sum(w16) / len(w16)
-        ach16 = flops / (dev16["pass_ms"] * 1e-3) / 1e12
+        ach16 = brute / (dev16["pass_ms"] * 1e-3) / 1e12
         nch16 = (d + 3 + 31) // 32
         nch16 = 4 if nch16 > 2 else nch16
         out["fp16"] = {"cells_per_s": n / sec16, "ms_per_step": sec16 * 1e3,
@@ -488,7 +496,7 @@ def run_single_config(args, name, x, workload, cpu_rows, cpu_repeats):
                                     "unit": "TFLOP/s", "frac": ach16 / BF16_DENSE_PEAK_TFLOPS,
                                     "traffic": measured_traffic(n, d, k, FP16_TRAFFIC_SUMMARY),
                                     "kernel": f"pass_local_kernel<{nch16}> (fp16 local-origin operands)",
-                                    "algorithmic_flops": flops,
+                                    "algorithmic_flops": brute,
                                     "executed_tflops": 2.0 * n * ((n + 511) // 512 * 512) * 32 * nch16
                                                        / (dev16["pass_ms"] * 1e-3) / 1e12},
                        "identical_to_bf16": bool(np.array_equal(ids16, got_ids) and np.array_equal(dist16, got_dist)),

@@ -90,6 +90,8 @@ typedef struct scrust_hip_knn_stats {
     uint64_t workspace_bytes; /* device workspace the call used (its arena and metadata) */
     uint32_t persistent;      /* 1: the pass ran one workgroup per slot with per-slot buffers */
     uint32_t reserved;
+    uint64_t pairs_scored;    /* (query row, reference) pairs the pass scored, padding included:
+                                 npad x ref_rows, less the tile-pair steps its ball bound skipped */
 } scrust_hip_knn_stats;
 
 /* Message for the last failure on the calling thread ("" if none). */

@@ -96,6 +96,7 @@ class KnnStats(ctypes.Structure):
         ("workspace_bytes", ctypes.c_uint64),
         ("persistent", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("pairs_scored", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -230,8 +230,10 @@ struct DmaTiles {
     u32x4* b1;
     u32x4* b2;
     // Persistent pass: the ball bound's LDS (kBallLds floats: the waves' thetas in two
-    // barrier-parity halves, the block's radius, its centre).
+    // barrier-parity halves, the block's radius, its centre), and each wave's staging of a
+    // refined block's fine chunks (kFineStage 16-byte slots a wave).
     float* ball;
+    u32x4* fine;
 };
 
 // Slots (16 bytes) of one LDS tile buffer.
@@ -267,6 +269,7 @@ constexpr int kCoarseSlots = kTileRefs * 4 * bf16_coarse_chunks(NCH);
 // barrier parity) at 0, the block's radius at 8, its centre (at most kBallMaxD floats) at 16.
 constexpr int kBallMaxD = static_cast<int>(kBallMaxDims);
 constexpr int kBallLds = 16 + kBallMaxD;
+constexpr int kFineStage = 3 * 64;  // fine chunks of one 16-reference block (NCH - coarse <= 3)
 // Three ring slots of two tiles' coarse parts in half of a CU's LDS (two workgroups per CU).
 template <int OP, int NCH>
 constexpr bool kPassSplit = OP == kOperandBf16 && bf16_coarse_chunks(NCH) < NCH && bf16_coarse_chunks(NCH) <= 2 &&
@@ -843,6 +846,7 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         // memory.
         typedef typename Op::Frag Frag;
         constexpr int KC = kCoarse, KF = NCH - kCoarse;
+        static_assert(KF * kWave <= kFineStage, "a block's fine chunks fit the wave's staging");
         constexpr int kStep = 2;
         constexpr int kCRounds = KC * kStep;  // 4 KC wave-wide loads per tile, one per wave per round
         // Sources (chunk index from the step's first tile) of this lane's slots in each round.
@@ -898,14 +902,24 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
                     need |= __ballot(m > gate[qb]);
                 }
                 if (need) {
+                    // The block's KF fine chunks go straight to this wave's LDS staging (lane l
+                    // its own 16 bytes of each), all in flight at once, then one wait: loads
+                    // into registers would be issued one at a time at this register budget.
+                    // The previous block's reads of the staging are done (its MFMAs used them).
                     const SCRUST_GLOBAL u32x4* src = ref_rows + static_cast<size_t>(t + cb / 4) * tile_u4 +
                                                      kTileRefs * 4 * KC + ((cb % 4) * 16 + lc) * 4 * KF + g * KF;
+                    u32x4* const stage = dma.fine + w * kFineStage;
 #pragma unroll
-                    for (int c = 0; c < KF; c++) f[c] = __builtin_bit_cast(Frag, src[c]);
+                    for (int c = 0; c < KF; c++)
+                        __builtin_amdgcn_global_load_lds(src + c, (__attribute__((address_space(3))) void*)(stage + c * kWave),
+                                                         16, 0, 0);
+                    __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
-                    for (int c = KC; c < NCH; c++)
+                    for (int c = KC; c < NCH; c++) {
+                        f[c - KC] = __builtin_bit_cast(Frag, stage[(c - KC) * kWave + lane]);
 #pragma unroll
                         for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_k32(f[c - KC], a[qb][c], acc[qb]);
+                    }
                     filter(acc, t, cb);
                 }
             };
@@ -1230,8 +1244,9 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
         __shared__ u32x4 b1[2 * kCoarseSlots<NCH>];
         __shared__ u32x4 b2[2 * kCoarseSlots<NCH>];
         __shared__ __attribute__((aligned(16))) float ball[kBallLds];
+        __shared__ u32x4 fine[4 * kFineStage];
         __shared__ uint32_t next;
-        const DmaTiles dma{b0, b1, b2, ball};
+        const DmaTiles dma{b0, b1, b2, ball, fine};
         // Whole blocks are [0, whole): every block with one range, the blocks before the
         // last round's when that scans in ranges, none when every block does (items v < whole
         // are whole blocks under single_item's reading, the items from whole on ranges).

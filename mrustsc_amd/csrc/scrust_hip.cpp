@@ -118,7 +118,8 @@ static double event_ms(hipEvent_t a, hipEvent_t b) {
     } while (0)
 
 // Reads the plan's flags and exact-row count after completion; maps flags to errors.
-static scrust_hip_status read_status(const Plan& plan, hipStream_t stream, uint32_t* list_count) {
+static scrust_hip_status read_status(const Plan& plan, hipStream_t stream, uint32_t* list_count,
+                                     uint32_t* scanned_steps = nullptr) {
     uint32_t host[kScalarCount] = {};
     SCRUST_CHECK(hipMemcpyAsync(host, plan.d_scalars, sizeof host, hipMemcpyDeviceToHost, stream),
                  "read status");
@@ -127,6 +128,7 @@ static scrust_hip_status read_status(const Plan& plan, hipStream_t stream, uint3
     if (host[kScalarFlags] & kFlagBandOverflow)
         return fail(SCRUST_HIP_KERNEL, "HIP kernel knn: a boundary tie band ran past the kept keys");
     if (list_count) *list_count = host[kScalarListCount];
+    if (scanned_steps) *scanned_steps = host[kScalarItemCounter + kQueueScanned];
     return SCRUST_HIP_OK;
 }
 
@@ -168,8 +170,8 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
     if (st != SCRUST_HIP_OK) return st;
     st = plan_enqueue(plan, true, ctx->compute_units, ctx->stream, ctx->events);
     if (st != SCRUST_HIP_OK) return st;
-    uint32_t listed = 0;
-    st = read_status(plan, ctx->stream, &listed);
+    uint32_t listed = 0, scanned = 0;
+    st = read_status(plan, ctx->stream, &listed, &scanned);
     if (st != SCRUST_HIP_OK) return st;
     if (stats) {
         hipEvent_t* ev = ctx->events;
@@ -193,6 +195,12 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
         stats->workspace_bytes = plan.arena_bytes + plan.meta_bytes;
         stats->persistent = plan.any_pass && s0.persist ? 1u : 0u;
         stats->reserved = 0u;
+        // A persistent pass counts the tile-pair steps its items scanned (256 query rows x
+        // 128 references each); every other pass scans each block against every pair.
+        const uint64_t ref_rows = (static_cast<uint64_t>(s0.n) + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
+        stats->pairs_scored = !plan.any_pass ? 0u
+                              : s0.persist ? static_cast<uint64_t>(scanned) * kPassRows * 2 * kTileRefsHost
+                                           : static_cast<uint64_t>(s0.npad) * ref_rows;
     }
     return SCRUST_HIP_OK;
 }
