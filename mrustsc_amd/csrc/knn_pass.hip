@@ -747,6 +747,103 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         return true;
     };
 
+    constexpr bool kBall = kTwoStage && DMA && PAIRS;
+    // Ball bound (s.balls; the persistent pass): the steps of tile pair
+    // P are skipped when no row q of the block can append a reference r of P. A row appends
+    // only pairs whose computed score exceeds its bound phi_q, and the computed score lies
+    // within skip_b_q of q.r - |r|^2 / 2 (the deviation the block test allows for the fine
+    // steps alone), so a pair with |q - r|^2 > theta_q = |q|^2 - 2 phi_q + 4 skip_b_q cannot
+    // append; and |q - r| >= |c_B - c_P| - rho_B - rho_P for the block's and the pair's
+    // balls. A skipped step would have appended nothing, and its compaction test found
+    // nothing, so the buffers end as with every step scanned. theta_q only falls during the
+    // scan: a test made with an earlier block maximum stays valid. Each wave keeps its rows'
+    // largest theta_q (wtheta, refreshed after its compactions) and writes it before each
+    // barrier into the half of the ball LDS of that barrier's parity; after the barrier every
+    // wave reads the same four values (theta_max), and so computes the same survivors and
+    // scans the same steps.
+    const bool prune = kBall && dma.ball != nullptr && s.balls != nullptr;
+    float* const blds = dma.ball;
+    const SCRUST_GLOBAL float* const balls = global_ptr(s.balls);
+    const uint32_t bstride = s.ball_stride;
+    const uint32_t ball_pairs = (n + 2 * kTileRefs - 1) / (2 * kTileRefs);
+    const uint32_t ball_count = ball_pairs + s.npad / kPassRows;
+    if (prune) {
+        // The block's ball into LDS; the first read of it follows the first barrier.
+        const SCRUST_GLOBAL float* cb = balls + static_cast<size_t>(ball_pairs + item.block) * bstride;
+        if (tid < static_cast<int>(bstride)) blds[16 + tid] = cb[tid];
+        if (tid == 0) blds[8] = balls[static_cast<size_t>(ball_count) * bstride + ball_pairs + item.block];
+    }
+    float wtheta = kInf, theta_max = kInf;
+    uint32_t par = 0;
+    auto refresh_theta = [&]() __attribute__((always_inline)) {
+        float t = 0.0f;
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+            uint32_t row = qbase + w * kWave + qb * 16 + lc;
+            if (row >= n) row = n - 1;
+            const float nq = global_ptr(s.norms)[row];
+            const float th = nq - 2.0f * phi[qb] + 4.0f * skip_b[qb];
+            // 2^-12 of the terms' magnitudes covers the rounding of th many times over.
+            t = __builtin_fmaxf(t, th + 0x1p-12f * (nq + 2.0f * __builtin_fabsf(phi[qb]) + 4.0f * skip_b[qb]));
+        }
+#pragma unroll
+        for (int m = 1; m < kWave; m <<= 1) t = __builtin_fmaxf(t, __shfl_xor(t, m, kWave));
+        wtheta = t;
+    };
+    // Before a barrier: this wave's theta into the half of that barrier's parity. After
+    // it: the four waves' values from that half.
+    auto ball_publish = [&]() __attribute__((always_inline)) {
+        if (prune && lane == 0) blds[par * 4 + w] = wtheta;
+    };
+    auto ball_absorb = [&]() __attribute__((always_inline)) {
+        if (prune) {
+            const float* h = blds + par * 4;
+            theta_max = __builtin_fmaxf(__builtin_fmaxf(h[0], h[1]), __builtin_fmaxf(h[2], h[3]));
+        }
+        par ^= 1u;
+    };
+    uint32_t scanned = 0;  // tile-pair steps this item scans
+    // Survivors among scan pairs [wb, wb + 64) (lane l tests pair wb + l): every pair of
+    // the range while theta_max is infinite (some row has no bound yet).
+    const uint32_t p_begin = t_begin / 2, pe = t_end / 2;
+    auto window = [&](uint32_t wb) __attribute__((always_inline)) -> uint64_t {
+        const uint32_t sp = wb + static_cast<uint32_t>(lane);
+        bool keep = sp < pe;
+        if (prune && theta_max < kInf && keep) {
+            const uint32_t P = phys(2 * sp) >> 1;
+            const float4* c = reinterpret_cast<const float4*>(s.balls + static_cast<size_t>(P) * bstride);
+            const float4* o = reinterpret_cast<const float4*>(blds + 16);
+            float d2 = 0.0f;
+#pragma unroll 1
+            for (uint32_t j = 0; j < bstride / 4; j++) {
+                const float4 a4 = c[j], b4 = o[j];
+                const float e0 = a4.x - b4.x, e1 = a4.y - b4.y, e2 = a4.z - b4.z, e3 = a4.w - b4.w;
+                d2 = __builtin_fmaf(e0, e0, d2);
+                d2 = __builtin_fmaf(e1, e1, d2);
+                d2 = __builtin_fmaf(e2, e2, d2);
+                d2 = __builtin_fmaf(e3, e3, d2);
+            }
+            const float rho = balls[static_cast<size_t>(ball_count) * bstride + P] + blds[8];
+            // Rounding of d2 (at most (d + 3) 2^-24 relative), of the square root and of
+            // the differences: the 2^-12 factors leave the bound on the safe side.
+            const float lb = __builtin_sqrtf(d2) * (1.0f - 0x1p-12f) - rho * (1.0f + 0x1p-12f);
+            keep = !(lb > 0.0f && lb * lb * (1.0f - 0x1p-12f) > theta_max);
+        }
+        return __ballot(keep);
+    };
+    uint32_t wb = p_begin;
+    uint64_t wm = window(p_begin);
+    constexpr uint32_t kNone = ~0u;
+    auto next = [&]() __attribute__((always_inline)) -> uint32_t {
+        while (wm == 0) {
+            wb += kWave;
+            if (wb >= pe) return kNone;
+            wm = window(wb);
+        }
+        const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(wm));
+        wm &= wm - 1;
+        return wb + b;
+    };
     if constexpr (OP == kOperandF16Local) {
         // 512 threads convert a 64-row tile: slot u = tid + 512 i is 16 bytes of LDS, row
         // u / (4 nch), part pp = u % (4 nch): features 8 pp .. 8 pp + 7 of the row, from 32
@@ -951,107 +1048,19 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             refine(acc0, 4 * kStep - 2);
             refine(acc1, 4 * kStep - 1);
         };
-        // Ball bound (s.balls): the step of tile pair P is skipped when no row q of the block
-        // can append a reference r of P. A row appends only pairs whose computed score
-        // exceeds its bound phi_q, and the computed score lies within skip_b_q of
-        // q.r - |r|^2 / 2 (the deviation the block test allows for the fine steps alone),
-        // so a pair with |q - r|^2 > theta_q = |q|^2 - 2 phi_q + 4 skip_b_q cannot append;
-        // and |q - r| >= |c_B - c_P| - rho_B - rho_P for the block's and the pair's balls. A
-        // skipped step would have appended nothing, and its compaction test found nothing,
-        // so the buffers end as with every step scanned. theta_q only falls during the scan:
-        // a test made with an earlier block maximum stays valid. Each wave keeps its rows'
-        // largest theta_q (wtheta, refreshed after its compactions) and writes it before
-        // each barrier into the half of the ball LDS of that barrier's parity; after the
-        // barrier every wave reads the same four values (theta_max), and so computes the same
-        // survivors and scans the same steps.
-        const bool prune = s.balls != nullptr;
-        float* const blds = dma.ball;
-        const SCRUST_GLOBAL float* const balls = global_ptr(s.balls);
-        const uint32_t bstride = s.ball_stride;
-        const uint32_t ball_pairs = (n + 2 * kTileRefs - 1) / (2 * kTileRefs);
-        const uint32_t ball_count = ball_pairs + s.npad / kPassRows;
-        if (prune) {
-            // The block's ball into LDS; the first read of it follows the first barrier.
-            const SCRUST_GLOBAL float* cb = balls + static_cast<size_t>(ball_pairs + item.block) * bstride;
-            if (tid < static_cast<int>(bstride)) blds[16 + tid] = cb[tid];
-            if (tid == 0) blds[8] = balls[static_cast<size_t>(ball_count) * bstride + ball_pairs + item.block];
-        }
-        float wtheta = kInf, theta_max = kInf;
-        uint32_t par = 0;
-        auto refresh_theta = [&]() __attribute__((always_inline)) {
-            float t = 0.0f;
-#pragma unroll
-            for (int qb = 0; qb < 4; qb++) {
-                uint32_t row = qbase + w * kWave + qb * 16 + lc;
-                if (row >= n) row = n - 1;
-                const float nq = global_ptr(s.norms)[row];
-                const float th = nq - 2.0f * phi[qb] + 4.0f * skip_b[qb];
-                // 2^-12 of the terms' magnitudes covers the rounding of th many times over.
-                t = __builtin_fmaxf(t, th + 0x1p-12f * (nq + 2.0f * __builtin_fabsf(phi[qb]) + 4.0f * skip_b[qb]));
-            }
-#pragma unroll
-            for (int m = 1; m < kWave; m <<= 1) t = __builtin_fmaxf(t, __shfl_xor(t, m, kWave));
-            wtheta = t;
-        };
         auto settle = [&]() __attribute__((always_inline)) {
-            if (prune && lane == 0) blds[par * 4 + w] = wtheta;
+            ball_publish();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_s_waitcnt(0);
             __builtin_amdgcn_s_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            if (prune) {
-                const float* h = blds + par * 4;
-                theta_max = __builtin_fmaxf(__builtin_fmaxf(h[0], h[1]), __builtin_fmaxf(h[2], h[3]));
-            }
-            par ^= 1u;
+            ball_absorb();
         };
-        uint32_t scanned = 0;
         auto step = [&](uint32_t t, auto slot) __attribute__((always_inline)) {
             consume_pair(phys(t), decltype(slot)::value);
             scanned++;
             settle();
             if (compact() && prune) refresh_theta();
-        };
-        // Survivors among scan pairs [wb, wb + 64) (lane l tests pair wb + l): every pair of
-        // the range while theta_max is infinite (some row has no bound yet).
-        const uint32_t pb = t_begin / 2, pe = t_end / 2;
-        auto window = [&](uint32_t wb) __attribute__((always_inline)) -> uint64_t {
-            const uint32_t sp = wb + static_cast<uint32_t>(lane);
-            bool keep = sp < pe;
-            if (prune && theta_max < kInf && keep) {
-                const uint32_t P = phys(2 * sp) >> 1;
-                const float4* c = reinterpret_cast<const float4*>(s.balls + static_cast<size_t>(P) * bstride);
-                const float4* o = reinterpret_cast<const float4*>(blds + 16);
-                float d2 = 0.0f;
-#pragma unroll 1
-                for (uint32_t j = 0; j < bstride / 4; j++) {
-                    const float4 a4 = c[j], b4 = o[j];
-                    const float e0 = a4.x - b4.x, e1 = a4.y - b4.y, e2 = a4.z - b4.z, e3 = a4.w - b4.w;
-                    d2 = __builtin_fmaf(e0, e0, d2);
-                    d2 = __builtin_fmaf(e1, e1, d2);
-                    d2 = __builtin_fmaf(e2, e2, d2);
-                    d2 = __builtin_fmaf(e3, e3, d2);
-                }
-                const float rho = balls[static_cast<size_t>(ball_count) * bstride + P] + blds[8];
-                // Rounding of d2 (at most (d + 3) 2^-24 relative), of the square root and of
-                // the differences: the 2^-12 factors leave the bound on the safe side.
-                const float lb = __builtin_sqrtf(d2) * (1.0f - 0x1p-12f) - rho * (1.0f + 0x1p-12f);
-                keep = !(lb > 0.0f && lb * lb * (1.0f - 0x1p-12f) > theta_max);
-            }
-            return __ballot(keep);
-        };
-        uint32_t wb = pb;
-        uint64_t wm = window(pb);
-        constexpr uint32_t kNone = ~0u;
-        auto next = [&]() __attribute__((always_inline)) -> uint32_t {
-            while (wm == 0) {
-                wb += kWave;
-                if (wb >= pe) return kNone;
-                wm = window(wb);
-            }
-            const uint32_t b = static_cast<uint32_t>(__builtin_ctzll(wm));
-            wm &= wm - 1;
-            return wb + b;
         };
         const std::integral_constant<int, 0> s0{};
         const std::integral_constant<int, 1> s1{};

@@ -195,12 +195,12 @@ scrust_hip_status run_single(scrust_hip_context* ctx, const float* x, size_t n, 
         stats->workspace_bytes = plan.arena_bytes + plan.meta_bytes;
         stats->persistent = plan.any_pass && s0.persist ? 1u : 0u;
         stats->reserved = 0u;
-        // A persistent pass counts the tile-pair steps its items scanned (256 query rows x
-        // 128 references each); every other pass scans each block against every pair.
+        // A pass with the ball bound counts the tile-pair steps its items scanned (256 query
+        // rows x 128 references each); every other pass scans each block against every pair.
         const uint64_t ref_rows = (static_cast<uint64_t>(s0.n) + 2 * kTileRefsHost - 1) / (2 * kTileRefsHost) * (2 * kTileRefsHost);
         stats->pairs_scored = !plan.any_pass ? 0u
-                              : s0.persist ? static_cast<uint64_t>(scanned) * kPassRows * 2 * kTileRefsHost
-                                           : static_cast<uint64_t>(s0.npad) * ref_rows;
+                              : s0.balls ? static_cast<uint64_t>(scanned) * kPassRows * 2 * kTileRefsHost
+                                         : static_cast<uint64_t>(s0.npad) * ref_rows;
     }
     return SCRUST_HIP_OK;
 }
