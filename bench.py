@@ -595,12 +595,13 @@ def config_block(args, name, x, workload, ctx, cpu_repeats=1):
     del single
     if name == "c2" and not args.no_large_k:
         block["k100"] = large_k_block(args, x, ctx, 100)
+        block["k128"] = large_k_block(args, x, ctx, 128)
     return block
 
 
 def large_k_block(args, x, ctx, k):
-    """The same embedding at a large k (the 128-candidate pass): bf16 builds, the whole graph
-    against its oracle fixture."""
+    """The same embedding at a large k (the 128- or 256-candidate pass): bf16 builds, the whole
+    graph against its oracle fixture."""
     n, d = x.shape
     single = Single(x, k, ctx)
     walls, st = single.timed(max(3, args.steps // 2), 2, "bf16")

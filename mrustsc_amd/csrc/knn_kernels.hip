@@ -547,14 +547,15 @@ __global__ __launch_bounds__(256) void rerank_kernel(const DatasetDesc* __restri
     }
 }
 
-// Rerank of rows with KP = 128 candidates: one wave per row, two keys per lane (sorted key
-// e = i*64 + lane), otherwise as rerank_kernel. A tie band here can exceed kBandCap members;
-// such a band keeps the (d^2, id) order, as the oracle's rule has it.
+// Rerank of rows with KP = 64 E candidates (128 or 256): one wave per row, E keys per lane
+// (sorted key e = i*64 + lane), otherwise as rerank_kernel. A tie band here can exceed
+// kBandCap members; such a band keeps the (d^2, id) order, as the oracle's rule has it.
+template <int E>
 __global__ __launch_bounds__(256) void rerank_wide_kernel(const DatasetDesc* __restrict__ sets,
                                                           uint32_t count, uint32_t total_rows,
                                                           int metric, uint32_t* __restrict__ fail_list,
                                                           uint32_t* __restrict__ fail_count) {
-    constexpr int KP = 128;
+    constexpr int KP = E * kWave;
     const int lane = threadIdx.x & 63;
     const uint32_t flat = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (flat >= total_rows) return;
@@ -567,10 +568,10 @@ __global__ __launch_bounds__(256) void rerank_wide_kernel(const DatasetDesc* __r
     const float* xq = s.coords + static_cast<size_t>(p) * d;
     const float nq = s.norms[p];
 
-    uint64_t key[2];
+    uint64_t key[E];
     bool full = true;
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < E; i++) {
         const uint64_t raw = s.cand[static_cast<size_t>(p) * KP + i * kWave + lane];
         const bool present = raw != kNoKey;
         full = full && __ballot(present) == ~0ull;
@@ -579,59 +580,87 @@ __global__ __launch_bounds__(256) void rerank_wide_kernel(const DatasetDesc* __r
         const float dsq = valid ? rerank_square(s, xq, nq, c, d) : kInf;
         key[i] = valid ? make_key(dsq, s.perm ? s.perm[c] : c) : kNoKey;
     }
-    wave_sort<2>(key, lane);
-    const int n_valid = __popcll(__ballot(key[0] != kNoKey)) + __popcll(__ballot(key[1] != kNoKey));
-    const float my_d[2] = {key_value(key[0]), key_value(key[1])};
-    const uint32_t my_id[2] = {key_id(key[0]), key_id(key[1])};
-    // Distance and id of sorted key e (e wave-uniform).
-    auto dist_at = [&](int e) { return e < kWave ? __shfl(my_d[0], e, kWave) : __shfl(my_d[1], e - kWave, kWave); };
+    wave_sort<E>(key, lane);
+    int n_valid = 0;
+    float my_d[E];
+    uint32_t my_id[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        n_valid += __popcll(__ballot(key[i] != kNoKey));
+        my_d[i] = key_value(key[i]);
+        my_id[i] = key_id(key[i]);
+    }
+    // Sorted key e's distance and id (e wave-uniform: word e / 64, lane e % 64).
+    auto dist_at = [&](int e) {
+        float v = 0.0f;
+#pragma unroll
+        for (int i = 0; i < E; i++)
+            if (e / kWave == i) v = __shfl(my_d[i], e % kWave, kWave);
+        return v;
+    };
     auto id_at = [&](int e) {
-        return static_cast<uint32_t>(e < kWave ? __shfl(static_cast<int>(my_id[0]), e, kWave)
-                                               : __shfl(static_cast<int>(my_id[1]), e - kWave, kWave));
+        uint32_t v = 0u;
+#pragma unroll
+        for (int i = 0; i < E; i++)
+            if (e / kWave == i) v = static_cast<uint32_t>(__shfl(static_cast<int>(my_id[i]), e % kWave, kWave));
+        return v;
     };
     const int k = static_cast<int>(s.k);
     const double t = static_cast<double>(dist_at(k - 1));
     const double tau = t * static_cast<double>(d + 2) * 0x1p-23;
     const bool tie = n_valid > k && (static_cast<double>(dist_at(k)) - t <= tau);
 
-    int pos[2] = {lane, kWave + lane};
-    if (tie) {
-        bool in_band[2];
-        uint64_t band[2];
+    int pos[E];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < E; i++) pos[i] = i * kWave + lane;
+    if (tie) {
+        bool in_band[E];
+        uint64_t band[E];
+        int len = 0;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
             in_band[i] = i * kWave + lane < n_valid && __builtin_fabs(static_cast<double>(my_d[i]) - t) <= tau;
             band[i] = __ballot(in_band[i]);
+            len += __popcll(band[i]);
         }
-        const int len = __popcll(band[0]) + __popcll(band[1]);
         if (len <= kBandCap) {
-            const int lo = band[0] ? __builtin_ctzll(band[0]) : kWave + __builtin_ctzll(band[1]);
-            const int hi = band[1] ? kWave + 63 - __builtin_clzll(band[1]) : 63 - __builtin_clzll(band[0]);
-            double d64[2];
+            // First and last sorted positions of the band (it is contiguous in the sort).
+            int lo = -1, hi = -1;
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
+            for (int i = 0; i < E; i++) {
+                if (band[i] && lo < 0) lo = i * kWave + __builtin_ctzll(band[i]);
+                if (band[i]) hi = i * kWave + 63 - __builtin_clzll(band[i]);
+            }
+            double d64[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) {
                 d64[i] = 0.0;
                 if (in_band[i] && my_d[i] != 0.0f) {
                     const uint32_t mp = s.inv_perm ? s.inv_perm[my_id[i]] : my_id[i];
                     d64[i] = direct_square_f64(xq, s.coords + static_cast<size_t>(mp) * d, d);
                 }
             }
-            int rank[2] = {0, 0};
+            int rank[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) rank[i] = 0;
             for (int e = lo; e <= hi; e++) {
-                const double od = e < kWave ? __shfl(d64[0], e, kWave) : __shfl(d64[1], e - kWave, kWave);
+                double od = 0.0;
+#pragma unroll
+                for (int i = 0; i < E; i++)
+                    if (e / kWave == i) od = __shfl(d64[i], e % kWave, kWave);
                 const uint32_t oid = id_at(e);
 #pragma unroll
-                for (int i = 0; i < 2; i++)
+                for (int i = 0; i < E; i++)
                     if (od < d64[i] || (od == d64[i] && oid < my_id[i])) rank[i]++;
             }
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < E; i++)
                 if (in_band[i]) pos[i] = lo + rank[i];
         }
     }
     if (row_certified(s, p, nq, d, t, tau, full, n_valid)) {
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
+        for (int i = 0; i < E; i++) {
             if (pos[i] < k && i * kWave + lane < n_valid) {
                 s.out_indices[static_cast<size_t>(q) * k + pos[i]] = my_id[i];
                 s.out_distances[static_cast<size_t>(q) * k + pos[i]] =
@@ -649,7 +678,12 @@ void launch_rerank(const DatasetDesc* sets, const BatchShape& shape, unsigned kp
     if (shape.total_rows == 0) return;
     if (kps & 128u) {
         const unsigned blocks = (shape.total_rows + 3) / 4;
-        hipLaunchKernelGGL(rerank_wide_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+        hipLaunchKernelGGL(rerank_wide_kernel<2>, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+                           shape.total_rows, shape.metric, fail_list, fail_count);
+    }
+    if (kps & 256u) {
+        const unsigned blocks = (shape.total_rows + 3) / 4;
+        hipLaunchKernelGGL(rerank_wide_kernel<4>, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
                            shape.total_rows, shape.metric, fail_list, fail_count);
     }
     if (kps & 32u) {

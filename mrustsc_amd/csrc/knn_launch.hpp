@@ -40,10 +40,15 @@ constexpr int kCapMax = 512;  // largest candidate buffer (slots per query row) 
 constexpr int kBandCap = 64;
 constexpr int kMaxK = 256;      // largest k any path serves
 // Candidates kept per row (KP) by the MFMA pass: 32 for k <= 15, 64 for k <= 47, 128 for
-// k <= 111 (k + 17 <= KP: the list holds the first k and a margin for the boundary band).
-constexpr int kMaxPassK = 111;  // largest k the MFMA candidate pass serves (KP = 128)
-constexpr int kMaxLocalK = 47;  // largest k of the local-origin fp16 pass (KP = 64)
-__host__ __device__ inline uint32_t pass_kp(uint32_t k) { return k + 17 <= 32 ? 32u : k + 17 <= 64 ? 64u : 128u; }
+// k <= 111, 256 for k <= 239 (k + 17 <= KP: the list holds the first k and a margin for the
+// boundary band). KP = 256 takes 1024-slot buffers and single builds (the batched kernel
+// serves k <= kMaxBatchK).
+constexpr int kMaxPassK = 239;   // largest k the MFMA candidate pass serves (KP = 256)
+constexpr int kMaxBatchK = 111;  // largest k of the batched pass (KP = 128)
+constexpr int kMaxLocalK = 47;   // largest k of the local-origin fp16 pass (KP = 64)
+__host__ __device__ inline uint32_t pass_kp(uint32_t k) {
+    return k + 17 <= 32 ? 32u : k + 17 <= 64 ? 64u : k + 17 <= 128 ? 128u : 256u;
+}
 constexpr int kMaxPassD = 128;  // largest d with a compiled MFMA variant
 constexpr int kAccWordsHost = 10;
 // Largest squared norm the MFMA candidate pass serves. Past it (coordinates beyond about

@@ -328,7 +328,7 @@ __device__ __forceinline__ void finish_rows(const DatasetDesc& s, const PassItem
 // references of tiles [tile_begin, tile_end), into buffer slot `split` (persistent pass:
 // into the workgroup's own buffer region). FIXED: the dataset's chunk count equals NCH at
 // compile time.
-template <int OP, int NCH, bool FIXED, bool DMA = false, bool PAIRS = false>
+template <int OP, int NCH, bool FIXED, bool DMA = false, bool PAIRS = false, bool BIG = false>
 __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem& item,
                                            PassSmem<OP, NCH>* sm, DmaTiles dma = {}) {
     typedef Operand<OP, NCH> Op;
@@ -420,7 +420,9 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     const uint32_t seg = cap / 4;
     // A tile appends at most 16 keys to a segment (4 column blocks x 4 references).
     // Pair steps (split tiles with PAIRS) scan two tiles between compactions: 32 keys.
-    const uint32_t trigger = seg - (PAIRS ? 32u : 16u);
+    // BIG (1024-slot buffers, KP = 256): a 256-slot segment could fill, and its count must
+    // stay below 256 for the byte-wide segment counts, so it compacts one key earlier.
+    const uint32_t trigger = seg - (PAIRS ? 32u : 16u) - (BIG ? 1u : 0u);
     SCRUST_GLOBAL uint64_t* const lane_seg =
         rbuf + static_cast<size_t>(w * kWave + lc) * cap + g * seg;
 
@@ -741,9 +743,13 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             full |= ((f | f >> 16 | f >> 32 | f >> 48) & 0xFFFFull) << (16 * QB);
         });
         if (!full) return false;
-        if (cap <= 128) compact_rows(std::integral_constant<int, 2>{}, full);
-        else if (cap <= 256) compact_rows(std::integral_constant<int, 4>{}, full);
-        else compact_rows(std::integral_constant<int, 8>{}, full);
+        if constexpr (BIG) {
+            compact_rows(std::integral_constant<int, 16>{}, full);
+        } else {
+            if (cap <= 128) compact_rows(std::integral_constant<int, 2>{}, full);
+            else if (cap <= 256) compact_rows(std::integral_constant<int, 4>{}, full);
+            else compact_rows(std::integral_constant<int, 8>{}, full);
+        }
         return true;
     };
 
@@ -1238,7 +1244,7 @@ __device__ __forceinline__ PassItem single_item(const DatasetDesc& s, uint32_t v
 
 // The persistent variant (PAIRS) is held to two waves per SIMD (256 registers a lane), which
 // its two workgroups per CU need.
-template <int OP, int NCH, bool PAIRS = false>
+template <int OP, int NCH, bool PAIRS = false, bool BIG = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PAIRS ? 2 : 1, 2)))
 void pass_kernel(const DatasetDesc* __restrict__ set) {
     const DatasetDesc& s = *set;
@@ -1295,10 +1301,10 @@ void pass_kernel(const DatasetDesc* __restrict__ set) {
             __shared__ u32x4 b0[kTileSlots<OP, NCH>];
             __shared__ u32x4 b1[kTileSlots<OP, NCH>];
             __shared__ u32x4 b2[kTileSlots<OP, NCH>];
-            pass_block<OP, NCH, true, true>(s, item, nullptr, DmaTiles{b0, b1, b2});
+            pass_block<OP, NCH, true, true, false, BIG>(s, item, nullptr, DmaTiles{b0, b1, b2});
         } else {
             __shared__ PassSmem<OP, NCH> sm;
-            pass_block<OP, NCH, true>(s, item, &sm);
+            pass_block<OP, NCH, true, false, false, BIG>(s, item, &sm);
         }
     }
 }
@@ -1589,6 +1595,12 @@ static void run_pass(const DatasetDesc* set, const DatasetDesc& host, hipStream_
             hipLaunchKernelGGL((pass_kernel<OP, NCH, true>), dim3(host.slots), dim3(256), 0, stream, set);
             return;
         }
+    }
+    // Plans keeping KP = 256 candidates (k > 111) have 1024-slot buffers: their own variant,
+    // whose 16-keys-per-lane compaction would raise every other variant's register peak.
+    if (host.cap > static_cast<uint32_t>(kCapMax)) {
+        hipLaunchKernelGGL((pass_kernel<OP, NCH, false, true>), dim3(groups), dim3(256), 0, stream, set);
+        return;
     }
     hipLaunchKernelGGL((pass_kernel<OP, NCH>), dim3(groups), dim3(256), 0, stream, set);
 }

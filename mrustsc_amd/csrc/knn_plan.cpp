@@ -136,7 +136,8 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         s.cap = cap;
         TuneEntry tuned{};
         bool have_tuned = false;
-        const bool pass = plan.operand != kOperandNone && sp.k <= static_cast<size_t>(kMaxPassK) &&
+        const bool pass = plan.operand != kOperandNone &&
+                          sp.k <= static_cast<size_t>(single ? kMaxPassK : kMaxBatchK) &&
                           sp.d <= static_cast<size_t>(kMaxPassD);
         s.on_pass = pass ? 1u : 0u;
         if (pass) {
@@ -280,7 +281,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // shared GPU (server requests): its workgroups hold every slot until the launch
             // ends, where the other requests' kernels would wait for them.
             const bool persist = single && !shared && !s.local && split_pass_nch(plan.operand, s.nch) &&
-                                 s.cap >= 256 && blocks > slots;
+                                 s.cap >= 256 && s.cap <= static_cast<uint32_t>(kCapMax) && blocks > slots;
             if (persist) {
                 s.persist = 1;
                 s.slots = static_cast<uint32_t>(slots);

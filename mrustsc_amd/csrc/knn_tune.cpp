@@ -119,6 +119,8 @@ scrust_hip_status tune(scrust_hip_context* ctx, const float* x, size_t n, size_t
     scrust_hip_status st = plan_layout(specs, &opt, base, ctx->compute_units, true, &defaults);
     if (st != SCRUST_HIP_OK) return st;
     if (!base.any_pass) return SCRUST_HIP_OK;  // exact path: no shape to choose
+    // KP = 256 (k > 111) has one shape: 1024-slot buffers in one range.
+    if (base.sets[0].kp > 128u) return SCRUST_HIP_OK;
     const DatasetDesc& s = base.sets[0];
     // (A launch's last-round blocks may scan in more ranges; the shape is the other blocks'.)
     const TuneEntry def{s.cap, s.tail_row0 ? 1u : s.splits, base.ordered ? 2u : 1u, 0.0f};

@@ -94,18 +94,16 @@ def test_c2_whole_graph_equals_the_oracle(context, precision):
 
 
 @pytest.mark.parametrize("k,precision", [(64, "bf16"), (64, "fp32"), (100, "bf16"), (100, "fp32"),
-                                         (111, "bf16"), (128, "bf16")])
+                                         (111, "bf16"), (128, "bf16"), (128, "fp32")])
 def test_c2_large_k_whole_graph_equals_the_oracle(context, k, precision):
     fix = _fixture("c2_large_k")["graphs"][str(k)]
     x = embedding(fix, 100_000)
     print(f"[case] C2 100000 x 50 k={k} {precision}", flush=True)
     ids, dist, st = device_build(x, k, context, precision, "euclidean")
     print(st, flush=True)
-    if k <= 111:
-        assert st["path"] == precision and st["candidates"] == 128
-        assert st["fallback_rows"] < 100
-    else:
-        assert st["path"] == "exact"
+    # KP = 128 up to k = 111, then KP = 256 (1024-slot buffers).
+    assert st["path"] == precision and st["candidates"] == (128 if k <= 111 else 256)
+    assert st["fallback_rows"] < 100
     check_against(fix, ids, dist, f"C2 k={k} {precision}")
 
 

@@ -254,11 +254,11 @@ def test_boundary_near_ties_are_resolved_in_fp64(context, precision):
 
 @pytest.mark.parametrize("precision", ("bf16", "fp32"))
 @pytest.mark.parametrize("tile", [128, 256, 512])
-@pytest.mark.parametrize("k", [5, 15, 30, 47, 48, 100, 111])
+@pytest.mark.parametrize("k", [5, 15, 30, 47, 48, 100, 111, 128])
 def test_every_candidate_buffer_size_matches_the_oracle(context, precision, tile, k):
     # The buffer size changes when rows are compacted, never the result: each size, with
-    # KP = 32 (k <= 15), KP = 64 (k <= 47, where 128 slots are raised to 256) and KP = 128
-    # (k <= 111, where every size is raised to 512).
+    # KP = 32 (k <= 15), KP = 64 (k <= 47, where 128 slots are raised to 256), KP = 128
+    # (k <= 111, where every size is raised to 512) and KP = 256 (every size raised to 1024).
     x = synthetic.clustered(2500, 24, seed=k + tile)
     print(f"[case] tile={tile} k={k} precision={precision}", flush=True)
     ids, dist = mrustsc_amd.knn_hip(x, k, precision=precision, tile=tile, context=context)
@@ -410,12 +410,13 @@ def test_random_shapes_options_and_data_match_the_oracle(context):
         np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)
 
 
-def test_short_lists_of_the_128_candidate_pass_match_the_oracle(context):
-    # n - 1 < 128: every other cell is a candidate and the list's tail is empty. A larger
-    # build first leaves the context's candidate slots full of its keys.
+def test_short_lists_of_the_wide_candidate_passes_match_the_oracle(context):
+    # n - 1 < KP (128 or 256): every other cell is a candidate and the list's tail is empty.
+    # A larger build first leaves the context's candidate slots full of its keys.
     big = synthetic.clustered(3000, 20, seed=9)
     mrustsc_amd.knn_hip(big, 100, context=context)
-    for n, k in [(50, 49), (65, 48), (100, 60), (111, 48), (112, 111), (128, 100), (129, 111)]:
+    for n, k in [(50, 49), (65, 48), (100, 60), (111, 48), (112, 111), (128, 100), (129, 111),
+                 (113, 112), (200, 150), (256, 239), (257, 239), (300, 128)]:
         for metric in ("euclidean", "cosine"):
             for precision in ("bf16", "fp32"):
                 x = np.random.default_rng(n + k).normal(size=(n, 5)).astype(np.float32)
@@ -428,14 +429,16 @@ def test_short_lists_of_the_128_candidate_pass_match_the_oracle(context):
 
 
 def test_random_large_k_cases_match_the_oracle(context):
-    # 40 seeded cases with k 48-128 (the 128-candidate pass up to k = 111, the exact kernel
-    # above), n 130-20,000, both metrics and pass precisions, every row order, on clustered,
-    # lattice (ties, wide tie bands) and scaled-normal data.
+    # 48 seeded cases with k 48-240 (the 128-candidate pass up to k = 111, the 256-candidate
+    # pass up to k = 239, the exact kernel above), n 130-20,000 (at least 2 k), both metrics
+    # and pass precisions, every row order, on clustered, lattice (ties, wide tie bands) and
+    # scaled-normal data.
     rng = np.random.default_rng(4096)
-    for case in range(40):
+    for case in range(48):
         n = int(np.exp(rng.uniform(np.log(130), np.log(20000))))
         d = int(rng.choice([2, 3, 8, 20, 31, 50, 64, 100, 128]))
-        k = int(rng.choice([48, 63, 64, 65, 80, 100, 110, 111, 112, 128]))
+        k = int(rng.choice([48, 63, 64, 65, 80, 100, 110, 111, 112, 128, 150, 200, 239, 240]))
+        n = max(n, 2 * k)
         metric = str(rng.choice(["euclidean", "cosine"]))
         precision = str(rng.choice(["bf16", "fp32"]))
         order = str(rng.choice(["auto", "input", "grouped"]))
@@ -450,7 +453,7 @@ def test_random_large_k_cases_match_the_oracle(context):
         print(f"[case] {tag}", flush=True)
         ids, dist, st = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, order=order,
                                             context=context, stats=True)
-        assert st["candidates"] == (128 if k <= 111 else 0), tag
+        assert st["candidates"] == (128 if k <= 111 else 256 if k <= 239 else 0), tag
         q = np.arange(n) if n <= 2000 else rng.choice(n, 300, replace=False)
         want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
         np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)

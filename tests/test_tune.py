@@ -153,7 +153,7 @@ def test_a_neighbouring_bucket_is_untouched_and_the_exact_path_has_nothing_to_tu
     _, _, st = mrustsc_amd.knn_hip(y, 15, context=context, stats=True)
     assert st["tuned"] == 0
     z = synthetic.clustered(600, 20, seed=5, n_clusters=5)
-    r = mrustsc_amd.tune(z, 120, context=context)  # k > 111: exact kernel only
+    r = mrustsc_amd.tune(z, 120, context=context)  # k > 111: one shape (1024-slot buffers)
     assert r["shapes"] == 0 and mrustsc_amd.tune_size() == 1
 
 
