@@ -80,17 +80,38 @@ __device__ __forceinline__ float direct_square(const float* __restrict__ a,
     return s < snap_threshold(d, na, nb) ? 0.0f : s;
 }
 
-// direct_square with the loads widened to VEC floats (rows 4 * VEC-byte aligned: VEC divides
-// d); the fma chain is the same, in dimension order.
+// direct_square over two rows in global memory, the loads widened to VEC floats (rows
+// 4 * VEC-byte aligned: VEC divides d) and issued 16 floats at a time before their
+// differences (one wait per batch instead of one per load); the fma chain is the same, in
+// dimension order.
 template <int VEC>
 __device__ __forceinline__ float direct_square_vec(const float* __restrict__ a,
                                                    const float* __restrict__ b, int d, float na,
                                                    float nb) {
     typedef float vec_t __attribute__((ext_vector_type(VEC)));
-    const vec_t* av = reinterpret_cast<const vec_t*>(a);
-    const vec_t* bv = reinterpret_cast<const vec_t*>(b);
+    constexpr int U = 16 / VEC;
+    const SCRUST_GLOBAL vec_t* av = reinterpret_cast<const SCRUST_GLOBAL vec_t*>(global_ptr(a));
+    const SCRUST_GLOBAL vec_t* bv = reinterpret_cast<const SCRUST_GLOBAL vec_t*>(global_ptr(b));
+    const int nv = d / VEC;
     float s = 0.0f;
-    for (int j = 0; j < d / VEC; j++) {
+    int j = 0;
+    for (; j + U <= nv; j += U) {
+        vec_t x[U], y[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            x[u] = av[j + u];
+            y[u] = bv[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float delta = x[u][e] - y[u][e];
+                s = __builtin_fmaf(delta, delta, s);
+            }
+        }
+    }
+    for (; j < nv; j++) {
         const vec_t x = av[j], y = bv[j];
 #pragma unroll
         for (int e = 0; e < VEC; e++) {

@@ -462,7 +462,7 @@ __device__ __forceinline__ float rerank_square(const DatasetDesc& s, const float
     // Rows are 16- or 8-byte aligned when 4 or 2 divides d (the arena is 256-byte aligned).
     if ((d & 3) == 0) return direct_square_vec<4>(xq, xr, d, nq, s.norms[c]);
     if ((d & 1) == 0) return direct_square_vec<2>(xq, xr, d, nq, s.norms[c]);
-    return direct_square(xq, xr, d, nq, s.norms[c]);
+    return direct_square_vec<1>(xq, xr, d, nq, s.norms[c]);
 }
 
 template <int KP>
