@@ -4,7 +4,8 @@
 // The server allocates its device-memory budget once and hands out regions of it with a
 // first-fit allocator. A scheduler thread admits queued requests in arrival order while
 // a slot (one HIP stream, pinned staging buffers, a completion event) is idle and the
-// request's region fits; the build (upload, the single-embedding pipeline, download of
+// request's region fits (past a head whose region does not, a later request that fits may go
+// first, a bounded number of times); the build (upload, the single-embedding pipeline, download of
 // the results into pinned memory) is enqueued on the slot's stream, so builds admitted
 // together run concurrently on the device. Completion is observed by polling the slot's
 // event; the results are copied into the caller's buffers and the region and slot are
@@ -27,6 +28,11 @@
 #include "scrust_hip_internal.hpp"
 
 namespace scrust_hip {
+
+// Admission: how far past a head that does not fit the scheduler looks, and how many
+// requests may pass a head before it waits for memory itself.
+constexpr size_t kBypassWindow = 16;
+constexpr uint32_t kBypassMax = 32;
 
 namespace {
 
@@ -98,6 +104,7 @@ struct Request {
     bool done = false;
     size_t offset = 0, bytes = 0;
     size_t arena_bytes = 0, meta_bytes = 0;
+    uint32_t bypassed = 0;  // times a later request was admitted ahead of this one at the head
     uint32_t slot = 0;
     uint64_t fallback_rows = 0;
     Plan plan;  // laid out at submission, placed into the region at admission
@@ -278,7 +285,10 @@ void scrust_hip_server::run() {
             progress = true;
             cv_done.notify_all();
         }
-        // Admit in arrival order while a slot is idle and the head request fits.
+        // Admit while a slot is idle: the head request when its region fits; when it does
+        // not, the first of the next kBypassWindow requests whose region does (a smaller build
+        // need not wait behind a large one for memory). A head bypassed kBypassMax times
+        // takes the next region that frees, so no request waits for ever.
         while (!queue.empty()) {
             Slot* idle = nullptr;
             for (Slot& slot : slots)
@@ -287,9 +297,23 @@ void scrust_hip_server::run() {
                     break;
                 }
             if (!idle) break;
-            Request& r = *queue.front();
-            if (!alloc.alloc(r.bytes, &r.offset)) break;
-            queue.pop_front();
+            size_t pick = queue.size();
+            size_t at = 0;
+            if (alloc.alloc(queue.front()->bytes, &at)) {
+                pick = 0;
+            } else if (queue.front()->bypassed < kBypassMax) {
+                const size_t window = std::min(queue.size(), kBypassWindow);
+                for (size_t i = 1; i < window; i++)
+                    if (alloc.alloc(queue[i]->bytes, &at)) {
+                        pick = i;
+                        queue.front()->bypassed++;
+                        break;
+                    }
+            }
+            if (pick == queue.size()) break;
+            Request& r = *queue[pick];
+            r.offset = at;
+            queue.erase(queue.begin() + static_cast<std::ptrdiff_t>(pick));
             idle->req = &r;
             r.slot = static_cast<uint32_t>(idle - slots.data());
             r.admit_ns = now_ns();

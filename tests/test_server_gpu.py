@@ -60,6 +60,32 @@ def test_admission_respects_the_memory_budget(context):
     assert stats["peak_memory_bytes"] <= budget
 
 
+def test_a_small_request_passes_a_head_that_does_not_fit(context):
+    # Budget for one large build and a small one, not two large ones: the second large
+    # request waits for memory at the head of the queue while the small one behind it is
+    # admitted; every result still equals its alone-run.
+    big = [synthetic.clustered(60000, 50, seed=70 + i) for i in range(2)]
+    small = synthetic.clustered(3000, 50, seed=72)
+    sizes = []
+    for x in (big[0], small):
+        with mrustsc_amd.Server(slots=2, memory_bytes=1 << 30) as probe:
+            probe.wait(probe.submit(x, 15))
+            sizes.append(probe.stats()["peak_memory_bytes"])
+    budget = sizes[0] + sizes[1] + (sizes[0] - sizes[1]) // 2
+    print(f"[case] server budget {budget} (large {sizes[0]}, small {sizes[1]})", flush=True)
+    with mrustsc_amd.Server(slots=4, memory_bytes=budget) as server:
+        tickets = [server.submit(big[0], 15), server.submit(big[1], 15), server.submit(small, 15)]
+        results = [server.wait(t) for t in tickets]
+        stats = server.stats()
+    assert stats["peak_memory_bytes"] <= budget
+    infos = [r[2] for r in results]
+    assert infos[2]["admit_ns"] < infos[1]["admit_ns"], infos
+    for (ids, dist, _), x in zip(results, (big[0], big[1], small)):
+        want_ids, want_dist = mrustsc_amd.knn_hip(x, 15, context=context)
+        np.testing.assert_array_equal(ids, want_ids)
+        np.testing.assert_array_equal(dist, want_dist)
+
+
 def test_a_request_larger_than_the_budget_is_refused():
     with mrustsc_amd.Server(slots=2, memory_bytes=16 << 20) as server:
         x = synthetic.clustered(60000, 50, seed=3)
