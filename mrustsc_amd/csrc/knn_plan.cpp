@@ -218,6 +218,13 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // most kSelectKeysMax keys, so splits x cap <= 1024.
             const uint32_t max_splits = kSelectKeysMax / s.cap;
             uint32_t splits;
+            // A build the persistent pass takes (bf16 split tiles, more blocks than workgroup
+            // slots, not on a shared GPU) scans every block in one range on the spatial order:
+            // its queues balance the blocks, its ball bound skips the far pairs, and the last
+            // round's blocks scan in ranges (below). Ranges from the start cost it the order
+            // and the bound (140,000 cells: 17.7 ms in 4 ranges of the input order).
+            const bool persist_shape = single && !shared && !s.local && split_pass_nch(plan.operand, s.nch) &&
+                                       s.cap >= 256 && s.cap <= static_cast<uint32_t>(kCapMax) && blocks > slots;
             // An item's fixed cost is about 64k cycles; a tile pair costs 2 x 4 x 4 x nch
             // MFMAs of 16 (bf16) or 32 (fp32) cycles.
             const double pair_cycles = 32.0 * s.nch * (plan.operand == kOperandBf16 ? 16.0 : 32.0);
@@ -226,7 +233,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                 splits = 1;  // every work item scans all references, from its group's tiles
                 s.chunk_pairs = pairs;
             } else if (single) {
-                splits = choose_splits(blocks, pairs, slots, max_splits, overhead);
+                splits = persist_shape ? 1u : choose_splits(blocks, pairs, slots, max_splits, overhead);
                 if (have_tuned && tuned.splits >= 1 && tuned.splits <= max_splits) {
                     // A tuned count is taken when it cuts the scan into exactly that many
                     // whole-pair ranges.
@@ -263,7 +270,10 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // spatial order), which spreads that round's work over every slot.
             const uint64_t rem = blocks % slots;
             const uint32_t tail_splits = std::min(kTailSplits, max_splits);
-            if (single && !s.local && splits == 1 && blocks > slots && rem > slots / 2 && tail_splits > 1) {
+            // The persistent pass splits any last round: its queues hand the ranges to the
+            // workgroups that free up first, where whole blocks would leave the others idle.
+            if (single && !s.local && splits == 1 && blocks > slots && rem > 0 && (rem > slots / 2 || persist_shape) &&
+                tail_splits > 1) {
                 const uint32_t len = (pairs + tail_splits - 1) / tail_splits;
                 if ((pairs + len - 1) / len == tail_splits) {
                     splits = tail_splits;

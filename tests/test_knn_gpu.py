@@ -484,6 +484,42 @@ def test_persistent_pass_equals_per_row_buffers_and_the_oracle(context):
         np.testing.assert_array_equal(dist[queries], want_dist)
 
 
+def test_ball_bound_keeps_every_graph_equal_to_a_scan_of_every_pair(context):
+    # The persistent pass on the spatial order skips tile pairs its ball bound proves out of
+    # reach. Seeded cases past 131,072 cells, clustered (the bound skips pairs) and without
+    # structure (it skips few or none), both metrics, d 10-52, k 1-47: every row equal to the
+    # fp32 pass (which keeps no balls and scans every pair), sampled rows equal to the oracle.
+    rng = np.random.default_rng(2024)
+    cases = [(140_000, 50, 15, "euclidean", "clustered"), (260_000, 17, 5, "cosine", "clustered"),
+             (200_000, 32, 30, "euclidean", "normal"), (150_000, 10, 1, "euclidean", "clustered"),
+             (520_000, 50, 47, "cosine", "clustered"), (180_000, 52, 15, "euclidean", "duplicates")]
+    for n, d, k, metric, kind in cases:
+        if kind == "clustered":
+            x = synthetic.clustered(n, d, seed=int(rng.integers(1 << 30)), n_clusters=int(rng.integers(5, 31)))
+        elif kind == "normal":
+            x = rng.normal(size=(n, d)).astype(np.float32)
+        else:
+            x = synthetic.clustered(n, d, seed=int(rng.integers(1 << 30)), n_clusters=8)
+            x[rng.choice(n, n // 10, replace=False)] = x[rng.choice(n, n // 10, replace=False)]
+        tag = f"n={n} d={d} k={k} {metric} {kind}"
+        print(f"[case] {tag}", flush=True)
+        ids, dist, st = mrustsc_amd.knn_hip(x, k, metric=metric, context=context, stats=True)
+        ids32, dist32, st32 = mrustsc_amd.knn_hip(x, k, metric=metric, precision="fp32", context=context,
+                                                  stats=True)
+        print(st, flush=True)
+        assert st["persistent"] == 1, tag
+        every = st32["pairs_scored"]
+        assert 0 < st["pairs_scored"] <= every, tag
+        if kind == "clustered":
+            assert st["pairs_scored"] < every, tag
+        np.testing.assert_array_equal(ids, ids32, err_msg=tag)
+        np.testing.assert_array_equal(dist, dist32, err_msg=tag)
+        queries = np.concatenate([rng.choice(n, 24, replace=False), [0, n - 1]])
+        want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=queries)
+        np.testing.assert_array_equal(ids[queries], want_ids, err_msg=tag)
+        np.testing.assert_array_equal(dist[queries], want_dist, err_msg=tag)
+
+
 def test_persistent_pass_skipped_for_huge_coordinates_lists_every_row(context):
     # A dataset past the pass's norm limit skips the scan; the persistent pass still gives
     # every row an empty candidate list, so every row takes the exact kernel.
