@@ -218,15 +218,14 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // most kSelectKeysMax keys, so splits x cap <= 1024.
             const uint32_t max_splits = kSelectKeysMax / s.cap;
             uint32_t splits;
-            // A bf16 two-stage build of more blocks than workgroup slots scans every block in
-            // one range on the spatial order, and the last round's blocks in ranges (below):
-            // the persistent pass's queues balance the blocks and its ball bound skips the far
-            // pairs; a server request (whole tiles, sharing the GPU) keeps the order's short
-            // append bounds. Ranges from the start cost both the order (140,000 cells: 17.7 ms
-            // in 4 ranges of the input order against 6.9 ms; serving at 0.8 of saturation p99
-            // 62-65 -> 26-27 ms, saturation 370 -> 426-449 requests/s).
-            const bool persist_shape = single && !s.local && split_pass_nch(plan.operand, s.nch) &&
-                                       s.cap >= 256 && s.cap <= static_cast<uint32_t>(kCapMax) && blocks > slots;
+            // A build of more blocks than workgroup slots scans every block in one range on the
+            // spatial order, and the last round's blocks in ranges (below): the order keeps the
+            // append bounds short, and on the persistent pass its ball bound skips the far pairs.
+            // Ranges from the start cost that order (the cost model gave such builds 3 or 4
+            // ranges of the input order): pass at 140,000 cells 17.7 -> 6.9 ms, at 160,000 x 100
+            // 34.3 -> 16.4 ms, at 300,000 x 50 fp32 88.6 -> 74.3 ms; serving at 0.8 of saturation
+            // p99 62-65 -> 26-27 ms, saturation 370 -> 426-449 requests/s.
+            const bool one_range = single && !s.local && blocks > slots;
             // An item's fixed cost is about 64k cycles; a tile pair costs 2 x 4 x 4 x nch
             // MFMAs of 16 (bf16) or 32 (fp32) cycles.
             const double pair_cycles = 32.0 * s.nch * (plan.operand == kOperandBf16 ? 16.0 : 32.0);
@@ -235,7 +234,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
                 splits = 1;  // every work item scans all references, from its group's tiles
                 s.chunk_pairs = pairs;
             } else if (single) {
-                splits = persist_shape ? 1u : choose_splits(blocks, pairs, slots, max_splits, overhead);
+                splits = one_range ? 1u : choose_splits(blocks, pairs, slots, max_splits, overhead);
                 if (have_tuned && tuned.splits >= 1 && tuned.splits <= max_splits) {
                     // A tuned count is taken when it cuts the scan into exactly that many
                     // whole-pair ranges.
@@ -274,7 +273,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             const uint32_t tail_splits = std::min(kTailSplits, max_splits);
             // Such builds split any last round: the ranges go to the workgroups that free up
             // first, where whole blocks would leave the others idle.
-            if (single && !s.local && splits == 1 && blocks > slots && rem > 0 && (rem > slots / 2 || persist_shape) &&
+            if (single && !s.local && splits == 1 && blocks > slots && rem > 0 && (rem > slots / 2 || one_range) &&
                 tail_splits > 1) {
                 const uint32_t len = (pairs + tail_splits - 1) / tail_splits;
                 if ((pairs + len - 1) / len == tail_splits) {
