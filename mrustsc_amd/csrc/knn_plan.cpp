@@ -218,14 +218,18 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // most kSelectKeysMax keys, so splits x cap <= 1024.
             const uint32_t max_splits = kSelectKeysMax / s.cap;
             uint32_t splits;
-            // A build of more blocks than workgroup slots scans every block in one range on the
-            // spatial order, and the last round's blocks in ranges (below): the order keeps the
-            // append bounds short, and on the persistent pass its ball bound skips the far pairs.
-            // Ranges from the start cost that order (the cost model gave such builds 3 or 4
-            // ranges of the input order): pass at 140,000 cells 17.7 -> 6.9 ms, at 160,000 x 100
-            // 34.3 -> 16.4 ms, at 300,000 x 50 fp32 88.6 -> 74.3 ms; serving at 0.8 of saturation
-            // p99 62-65 -> 26-27 ms, saturation 370 -> 426-449 requests/s.
-            const bool one_range = single && !s.local && blocks > slots;
+            // A build of more blocks than workgroup slots, or a bf16 build of at least
+            // kOrderMinCells cells, scans every block in one range on the spatial order, and the
+            // last round's blocks in ranges (below): the order keeps the append bounds short,
+            // and on the persistent pass its ball bound skips the far pairs. Ranges from the
+            // start cost that order (the cost model gave such builds 2 to 4 ranges of the input
+            // order): pass at 80,000 cells 5.14 -> 3.67 ms, at 140,000 17.7 -> 6.9 ms, at
+            // 160,000 x 100 34.3 -> 16.4 ms, at 300,000 x 50 fp32 88.6 -> 74.3 ms; serving at
+            // 0.8 of saturation p99 62-65 -> 26-27 ms, saturation 370 -> 426-449 requests/s. The
+            // fp32 pass of a single round keeps the model's ranges (80,000 cells: 9.04 against
+            // 9.84 ms in one range).
+            const bool one_range = single && !s.local &&
+                                   (blocks > slots || (plan.operand == kOperandBf16 && sp.n >= kOrderMinCells));
             // An item's fixed cost is about 64k cycles; a tile pair costs 2 x 4 x 4 x nch
             // MFMAs of 16 (bf16) or 32 (fp32) cycles.
             const double pair_cycles = 32.0 * s.nch * (plan.operand == kOperandBf16 ? 16.0 : 32.0);
