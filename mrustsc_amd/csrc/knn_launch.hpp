@@ -166,7 +166,8 @@ struct DatasetDesc {
     uint32_t slots;
     uint32_t* queue;
     uint64_t* tail_cand;
-    // Bounding balls of the persistent pass (pass order, coordinates): ball b < pairs is tile
+    // Bounding balls (pass order, coordinates) of the persistent pass and of the wide bf16
+    // rows' register-staged pass of more than one round: ball b < pairs is tile
     // pair b (reference rows 128 b .. 128 b + 127), ball pairs + j query block j (rows 256 j ..
     // 256 j + 255); centre b at balls + b * ball_stride (d floats, zero to ball_stride), then
     // (pairs + blocks) radii, each an upper bound of the distance of the ball's rows from its
@@ -181,7 +182,7 @@ inline bool split_pass_nch(int operand, uint32_t nch) { return operand == kOpera
 // items scanned (the others skipped by the ball bound).
 constexpr int kQueueWords = 10;
 constexpr int kQueueScanned = 9;
-constexpr uint32_t kBallMaxDims = 64;  // widest dataset the persistent pass keeps balls for
+constexpr uint32_t kBallMaxDims = 128;  // widest dataset a pass keeps balls for
 
 // Row offset of reference range j's buffers and segment counts: range 0 holds every row,
 // a range j >= 1 only the rows from tail_row0 on (every row when tail_row0 is 0), so row r

@@ -220,6 +220,7 @@ __device__ __forceinline__ f32x4 mfma_k32(f16x8 b, f16x8 a, f32x4 acc) {
 template <int OP, int NCH>
 struct PassSmem {
     u32x4 tile[2][kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max)];
+    float ball[16 + 128];  // the ball bound's LDS (kBallLds floats)
 };
 
 // Three tile buffers for LDS-direct loads, each its own __shared__ array: the compiler tells
@@ -269,6 +270,7 @@ constexpr int kCoarseSlots = kTileRefs * 4 * bf16_coarse_chunks(NCH);
 // barrier parity) at 0, the block's radius at 8, its centre (at most kBallMaxD floats) at 16.
 constexpr int kBallMaxD = static_cast<int>(kBallMaxDims);
 constexpr int kBallLds = 16 + kBallMaxD;
+static_assert(kBallLds <= 16 + 128, "the ball LDS fits PassSmem::ball");
 constexpr int kFineStage = 3 * 64;  // fine chunks of one 16-reference block (NCH - coarse <= 3)
 // Three ring slots of two tiles' coarse parts in half of a CU's LDS (two workgroups per CU).
 template <int OP, int NCH>
@@ -753,7 +755,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         return true;
     };
 
-    constexpr bool kBall = kTwoStage && DMA && PAIRS;
+    // The persistent pass's split loop and the register-staged loop of wide bf16 rows.
+    constexpr bool kBall = kTwoStage && ((DMA && PAIRS) || !DMA);
     // Ball bound (s.balls; the persistent pass): the steps of tile pair
     // P are skipped when no row q of the block can append a reference r of P. A row appends
     // only pairs whose computed score exceeds its bound phi_q, and the computed score lies
@@ -767,8 +770,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
     // barrier into the half of the ball LDS of that barrier's parity; after the barrier every
     // wave reads the same four values (theta_max), and so computes the same survivors and
     // scans the same steps.
-    const bool prune = kBall && dma.ball != nullptr && s.balls != nullptr;
-    float* const blds = dma.ball;
+    float* const blds = DMA ? dma.ball : (sm ? sm->ball : nullptr);
+    const bool prune = kBall && blds != nullptr && s.balls != nullptr;
     const SCRUST_GLOBAL float* const balls = global_ptr(s.balls);
     const uint32_t bstride = s.ball_stride;
     const uint32_t ball_pairs = (n + 2 * kTileRefs - 1) / (2 * kTileRefs);
@@ -1102,6 +1105,52 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         }
         if (tid == 0) atomicAdd(&s.queue[kQueueScanned], scanned);
     } else if constexpr (!DMA) {
+    if constexpr (kBall) {
+        if (prune) {
+            // Wide bf16 rows of a multi-round build on the spatial order: the same loop over
+            // the surviving pairs (q, the next survivor's tiles loaded while q's are consumed),
+            // the ball bound's theta exchange around each barrier.
+            auto sync = [&]() __attribute__((always_inline)) {
+                ball_publish();
+                __syncthreads();
+                ball_absorb();
+            };
+            auto after = [&]() __attribute__((always_inline)) {
+                if (compact()) refresh_theta();
+            };
+            uint32_t q = next();
+            if (q != kNone) {
+                uint32_t qn = next();
+                SCRUST_LOAD_TILE(pa, 2 * q)
+                SCRUST_LOAD_TILE(pb, 2 * q + 1)
+                SCRUST_STORE_TILE(pa, 0)
+                sync();
+                for (;;) {
+                    scanned++;
+                    // Past the last survivor the loads repeat its own tiles (never stored).
+                    const uint32_t tn = qn != kNone ? 2 * qn : 2 * q;
+                    SCRUST_LOAD_TILE(pa, tn)
+                    consume(phys(2 * q), 0);
+                    SCRUST_STORE_TILE(pb, 1)
+                    sync();
+                    after();
+                    SCRUST_LOAD_TILE(pb, tn + 1)
+                    consume(phys(2 * q + 1), 1);
+                    if (qn == kNone) break;
+                    SCRUST_STORE_TILE(pa, 0)
+                    sync();
+                    after();
+                    q = qn;
+                    qn = next();
+                }
+                sync();
+                after();
+            }
+            if (tid == 0) atomicAdd(&s.queue[kQueueScanned], scanned);
+            goto scan_done;
+        }
+    }
+    {
     // The loop has one exit, at its bottom (the tile count is even): a mid-loop exit would
     // merge into the loop head, where the compiler must then assume the prefetch just
     // issued is outstanding and waits for it. Loads past the end reload the last tile.
@@ -1122,6 +1171,8 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         __syncthreads();
         compact();
     }
+    }
+    scan_done:;
     } else {
         // LDS-direct loads (global_load_lds_dwordx4): one wave-wide load fills 64 consecutive
         // 16-byte slots of a tile buffer, lane l the l-th. Slot u of a tile is reference row
@@ -1383,10 +1434,12 @@ __global__ __launch_bounds__(256) void ball_kernel(const DatasetDesc* __restrict
     __shared__ float part[4][kBallMaxD];
     __shared__ float centre[kBallMaxD];
     __shared__ float wfar[4];
-    float sum = 0.0f;
-    if (lane < d)
-        for (uint32_t r = r0 + w; r < r1; r += 4) sum += coords[static_cast<size_t>(r) * d + lane];
-    part[w][lane] = sum;
+    for (uint32_t j = lane; j < static_cast<uint32_t>(kBallMaxD); j += kWave) {
+        float sum = 0.0f;
+        if (j < d)
+            for (uint32_t r = r0 + w; r < r1; r += 4) sum += coords[static_cast<size_t>(r) * d + j];
+        part[w][j] = sum;
+    }
     __syncthreads();
     if (tid < stride) {
         const float c = tid < d ? (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]) / static_cast<float>(r1 - r0) : 0.0f;

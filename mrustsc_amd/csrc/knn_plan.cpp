@@ -335,7 +335,10 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             // and a pair out of reach of every row of the block is skipped. Builds of up to 512
             // blocks keep none: at 100,000 cells the bound skipped 4% of the pairs and its
             // tests cost 10% of the pass.
-            if (s.persist && plan.ordered && sp.d <= kBallMaxDims) {
+            const bool wide_rounds = single && plan.operand == kOperandBf16 && !s.local && s.nch > 5 &&
+                                     bf16_coarse_chunks(static_cast<int>(s.nch)) < static_cast<int>(s.nch) &&
+                                     blocks > slots && s.cap <= static_cast<uint32_t>(kCapMax);
+            if ((s.persist || wide_rounds) && plan.ordered && sp.d <= kBallMaxDims) {
                 s.ball_stride = (static_cast<uint32_t>(sp.d) + 3u) / 4u * 4u;
                 s.balls = static_cast<float*>(take((pairs + blocks) * (s.ball_stride + 1ull) * sizeof(float)));
             }
