@@ -44,7 +44,9 @@ typedef enum scrust_hip_precision {
     SCRUST_HIP_BF16 = 0,
     /* The same pipeline with an fp32 MFMA candidate pass: the plain FP32 kernel. */
     SCRUST_HIP_FP32 = 1,
-    /* fp32 direct differences over every pair (no MFMA); the path for k > 239 or d > 128. */
+    /* fp32 direct differences over every pair (no MFMA); the path for k > 239 or d > 256
+     * (batches: k > 111 or d > 128). A bf16 or fp16 single call with 128 < d <= 256 takes
+     * the fp32 pass. */
     SCRUST_HIP_EXACT = 2,
     /* fp16 single products of locally centred coordinates on MFMA (one origin per 512 query
      * rows of the spatial row order, fp32 accumulate, a rigorous per-pair error term folded

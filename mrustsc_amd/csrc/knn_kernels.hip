@@ -186,8 +186,8 @@ constexpr int kPrepRows = 64;
 
 __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __restrict__ sets,
                                                            int metric, unsigned* __restrict__ flags) {
-    __shared__ float xs[kPrepRows * kMaxPassD];
-    __shared__ double mean_s[kMaxPassD];
+    __shared__ float xs[kPrepRows * kMaxPackedD];
+    __shared__ double mean_s[kMaxPackedD];
     __shared__ float nrm_s[kPrepRows];
     constexpr uint16_t kBf16Inf = 0x7F80u;
     const DatasetDesc& s = sets[blockIdx.z];
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void prepare_rows_kernel(const DatasetDesc* __
 // the reference side puts |b''|^2 2^-16 (hi, lo) and an upper bound of |b''| in the same
 // three places (knn_pass.hip). anorm holds a lower bound of |a''|^2.
 __global__ __launch_bounds__(256) void local_queries_kernel(const DatasetDesc* __restrict__ sets) {
-    __shared__ float o_s[kMaxPassD + 32];
+    __shared__ float o_s[kMaxPackedD + 32];
     const DatasetDesc& s = sets[blockIdx.y];
     if (!s.on_pass || !s.local) return;
     const uint32_t n = s.n, r0 = blockIdx.x * kLocalRows;

@@ -49,7 +49,8 @@ constexpr int kMaxLocalK = 47;   // largest k of the local-origin fp16 pass (KP 
 __host__ __device__ inline uint32_t pass_kp(uint32_t k) {
     return k + 17 <= 32 ? 32u : k + 17 <= 64 ? 64u : k + 17 <= 128 ? 128u : 256u;
 }
-constexpr int kMaxPassD = 128;  // largest d with a compiled MFMA variant
+constexpr int kMaxPackedD = 128;  // largest d of the packed 16-bit operands (bf16, fp16)
+constexpr int kMaxPassD = 256;    // largest d with a compiled MFMA variant (fp32 past kMaxPackedD)
 constexpr int kAccWordsHost = 10;
 // Largest squared norm the MFMA candidate pass serves. Past it (coordinates beyond about
 // 10^15, or a norm that overflowed f32) packed products, their sums or the bf16 rounding of

@@ -102,6 +102,13 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         plan.operand = local ? kOperandF16Local : kOperandBf16;
         plan.precision = local ? SCRUST_HIP_FP16 : SCRUST_HIP_BF16;
     }
+    // The packed 16-bit operands hold up to kMaxPackedD features: a wider single dataset
+    // takes the fp32 pass (up to kMaxPassD features), which gives the same graph.
+    if (single && plan.operand == kOperandBf16 && specs[0].d > static_cast<size_t>(kMaxPackedD) &&
+        specs[0].d <= static_cast<size_t>(kMaxPassD)) {
+        plan.operand = kOperandF32;
+        plan.precision = SCRUST_HIP_FP32;
+    }
     plan.shape.count = static_cast<uint32_t>(specs.size());
     plan.shape.metric = metric;
     plan.shape.operand = plan.operand;
@@ -138,7 +145,7 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         bool have_tuned = false;
         const bool pass = plan.operand != kOperandNone &&
                           sp.k <= static_cast<size_t>(single ? kMaxPassK : kMaxBatchK) &&
-                          sp.d <= static_cast<size_t>(kMaxPassD);
+                          sp.d <= static_cast<size_t>(single && plan.operand == kOperandF32 ? kMaxPassD : kMaxPackedD);
         s.on_pass = pass ? 1u : 0u;
         if (pass) {
             // The local-origin pass's bound is looser for rows far from their work item's

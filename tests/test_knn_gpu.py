@@ -428,6 +428,32 @@ def test_short_lists_of_the_wide_candidate_passes_match_the_oracle(context):
                 np.testing.assert_array_equal(dist, want_dist, err_msg=tag)
 
 
+def test_rows_of_129_to_256_features_take_the_fp32_pass_and_match_the_oracle(context):
+    # The packed 16-bit operands hold up to 128 features; single builds of 129-256 take the
+    # fp32 pass (49 or 65 chunks) whatever precision was asked, past 256 the exact kernel.
+    # Both sides of each chunk boundary, one-round and multi-round builds (one range on the
+    # spatial order), both metrics, k across the candidate tiers.
+    cases = [(3000, 129, 15, "euclidean", "bf16"), (5000, 150, 30, "cosine", "fp16"),
+             (20000, 192, 15, "euclidean", "fp32"), (9000, 193, 100, "euclidean", "bf16"),
+             (30000, 200, 5, "cosine", "bf16"), (7000, 256, 15, "euclidean", "bf16"),
+             (4000, 256, 200, "euclidean", "fp32"), (2500, 257, 15, "euclidean", "bf16"),
+             (150000, 200, 15, "euclidean", "bf16"), (140000, 256, 30, "cosine", "bf16")]
+    rng = np.random.default_rng(129)
+    for n, d, k, metric, precision in cases:
+        tag = f"n={n} d={d} k={k} {metric} {precision}"
+        print(f"[case] {tag}", flush=True)
+        x = synthetic.clustered(n, d, seed=d, n_clusters=12)
+        ids, dist, st = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, context=context,
+                                            stats=True)
+        assert st["path"] == ("fp32" if d <= 256 else "exact"), tag
+        if d <= 256:
+            assert st["fallback_rows"] < n // 100 + 8, tag
+        q = np.arange(n) if n <= 3000 else rng.choice(n, 200, replace=False)
+        want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
+        np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
+        np.testing.assert_array_equal(dist[q], want_dist, err_msg=tag)
+
+
 def test_random_large_k_cases_match_the_oracle(context):
     # 48 seeded cases with k 48-240 (the 128-candidate pass up to k = 111, the 256-candidate
     # pass up to k = 239, the exact kernel above), n 130-20,000 (at least 2 k), both metrics
