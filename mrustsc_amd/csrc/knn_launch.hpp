@@ -183,7 +183,7 @@ inline bool split_pass_nch(int operand, uint32_t nch) { return operand == kOpera
 // items scanned (the others skipped by the ball bound).
 constexpr int kQueueWords = 10;
 constexpr int kQueueScanned = 9;
-constexpr uint32_t kBallMaxDims = 128;  // widest dataset a pass keeps balls for
+constexpr uint32_t kBallMaxDims = 256;  // widest dataset a pass keeps balls for
 
 // Row offset of reference range j's buffers and segment counts: range 0 holds every row,
 // a range j >= 1 only the rows from tail_row0 on (every row when tail_row0 is 0), so row r

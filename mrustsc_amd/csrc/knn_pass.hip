@@ -220,7 +220,7 @@ __device__ __forceinline__ f32x4 mfma_k32(f16x8 b, f16x8 a, f32x4 acc) {
 template <int OP, int NCH>
 struct PassSmem {
     u32x4 tile[2][kTileRefs * lds_stride(Operand<OP, NCH>::kGroupU4Max)];
-    float ball[16 + 128];  // the ball bound's LDS (kBallLds floats)
+    float ball[16 + kBallMaxDims];  // the ball bound's LDS (kBallLds floats)
 };
 
 // Three tile buffers for LDS-direct loads, each its own __shared__ array: the compiler tells
@@ -270,7 +270,7 @@ constexpr int kCoarseSlots = kTileRefs * 4 * bf16_coarse_chunks(NCH);
 // barrier parity) at 0, the block's radius at 8, its centre (at most kBallMaxD floats) at 16.
 constexpr int kBallMaxD = static_cast<int>(kBallMaxDims);
 constexpr int kBallLds = 16 + kBallMaxD;
-static_assert(kBallLds <= 16 + 128, "the ball LDS fits PassSmem::ball");
+static_assert(kBallLds <= 16 + static_cast<int>(kBallMaxDims), "the ball LDS fits PassSmem::ball");
 constexpr int kFineStage = 3 * 64;  // fine chunks of one 16-reference block (NCH - coarse <= 3)
 // Three ring slots of two tiles' coarse parts in half of a CU's LDS (two workgroups per CU).
 template <int OP, int NCH>
@@ -599,6 +599,22 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
             skip_b[qb] = 0x1p-7f * 1.02f * qn * rb + 0x1p-13f * mx;
             gate[qb] = phi[qb] - skip_b[qb];
         }
+    } else if constexpr (OP == kOperandF32 && FIXED) {
+        // fp32 operands (the ball bound of the wide fp32 pass): the computed score lies within
+        // acc_rel (|q||r| + |r|^2 / 2) of q.r - |r|^2 / 2, plus half the f32 rounding of the
+        // |r|^2 feature ((d + 1) u |r|^2); skip_b_q bounds both, rounded up.
+        const float mx = __uint_as_float(*global_ptr(s.max_norm_bits));
+        const float rb = __builtin_sqrtf(mx * (1.0f + 0x1p-15f));
+        const float ar = static_cast<float>(s.acc_rel) * 1.05f;
+        const float nr = static_cast<float>(s.d + 2) * 0x1p-24f;
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+            uint32_t row = qbase + w * kWave + qb * 16 + lc;
+            if (row >= n) row = n - 1;
+            const float qn = __builtin_sqrtf(global_ptr(s.norms)[row] * (1.0f + 0x1p-15f));
+            skip_b[qb] = ar * (1.02f * qn * rb + 0.51f * mx) + nr * mx;
+            gate[qb] = 0.0f;
+        }
     }
     auto consume = [&](uint32_t t, int cur) __attribute__((always_inline)) {
         f32x4 acc0[4], acc1[4];
@@ -755,8 +771,9 @@ __device__ __forceinline__ void pass_block(const DatasetDesc& s, const PassItem&
         return true;
     };
 
-    // The persistent pass's split loop and the register-staged loop of wide bf16 rows.
-    constexpr bool kBall = kTwoStage && ((DMA && PAIRS) || !DMA);
+    // The persistent pass's split loop and the register-staged loop of wide bf16 rows and of
+    // the fp32 pass (whose plans keep balls only past the packed operands' 128 features).
+    constexpr bool kBall = (kTwoStage && ((DMA && PAIRS) || !DMA)) || (OP == kOperandF32 && FIXED && !DMA);
     // Ball bound (s.balls; the persistent pass): the steps of tile pair
     // P are skipped when no row q of the block can append a reference r of P. A row appends
     // only pairs whose computed score exceeds its bound phi_q, and the computed score lies

@@ -345,7 +345,11 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
             const bool wide_rounds = single && plan.operand == kOperandBf16 && !s.local && s.nch > 5 &&
                                      bf16_coarse_chunks(static_cast<int>(s.nch)) < static_cast<int>(s.nch) &&
                                      blocks > slots && s.cap <= static_cast<uint32_t>(kCapMax);
-            if ((s.persist || wide_rounds) && plan.ordered && sp.d <= kBallMaxDims) {
+            // The fp32 pass of 129-256 features (the path of every precision there) keeps
+            // them too; at up to 128 it stays the plain fp32 kernel.
+            const bool wide_f32 = single && plan.operand == kOperandF32 && sp.d > static_cast<size_t>(kMaxPackedD) &&
+                                  blocks > slots && s.cap <= static_cast<uint32_t>(kCapMax);
+            if ((s.persist || wide_rounds || wide_f32) && plan.ordered && sp.d <= kBallMaxDims) {
                 s.ball_stride = (static_cast<uint32_t>(sp.d) + 3u) / 4u * 4u;
                 s.balls = static_cast<float*>(take((pairs + blocks) * (s.ball_stride + 1ull) * sizeof(float)));
             }

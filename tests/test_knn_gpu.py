@@ -432,7 +432,7 @@ def test_rows_of_129_to_256_features_take_the_fp32_pass_and_match_the_oracle(con
     # The packed 16-bit operands hold up to 128 features; single builds of 129-256 take the
     # fp32 pass (49 or 65 chunks) whatever precision was asked, past 256 the exact kernel.
     # Both sides of each chunk boundary, one-round and multi-round builds (one range on the
-    # spatial order), both metrics, k across the candidate tiers.
+    # spatial order, with the ball bound), both metrics, k across the candidate tiers.
     cases = [(3000, 129, 15, "euclidean", "bf16"), (5000, 150, 30, "cosine", "fp16"),
              (20000, 192, 15, "euclidean", "fp32"), (9000, 193, 100, "euclidean", "bf16"),
              (30000, 200, 5, "cosine", "bf16"), (7000, 256, 15, "euclidean", "bf16"),
@@ -448,6 +448,14 @@ def test_rows_of_129_to_256_features_take_the_fp32_pass_and_match_the_oracle(con
         assert st["path"] == ("fp32" if d <= 256 else "exact"), tag
         if d <= 256:
             assert st["fallback_rows"] < n // 100 + 8, tag
+        if n > 131_072:
+            # Multi-round builds on the spatial order skip tile pairs by their ball bound: every
+            # row equal to the input-order build (no balls).
+            assert st["order"] == 2, tag
+            ids_in, dist_in = mrustsc_amd.knn_hip(x, k, metric=metric, precision=precision, order="input",
+                                                  context=context)
+            np.testing.assert_array_equal(ids, ids_in, err_msg=tag)
+            np.testing.assert_array_equal(dist, dist_in, err_msg=tag)
         q = np.arange(n) if n <= 3000 else rng.choice(n, 200, replace=False)
         want_ids, want_dist = oracle.knn(x, k, metric=metric, queries=q)
         np.testing.assert_array_equal(ids[q], want_ids, err_msg=tag)
