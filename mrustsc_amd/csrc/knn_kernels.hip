@@ -391,9 +391,11 @@ void launch_prepare_all(const DatasetDesc* sets, const BatchShape& shape, unsign
         hipLaunchKernelGGL(colmean_kernel, dim3(shape.max_col_groups, shape.count), dim3(kWave), 0,
                            stream, sets);
     }
+    // Datasets on a packed 16-bit pass are prepared by prepare_rows_kernel alone.
     const unsigned blocks = (shape.total_prep + 255) / 256;
-    hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
-                       shape.total_prep, shape.metric, shape.operand, flags);
+    if (shape.direct_sets > 0)
+        hipLaunchKernelGGL(prepare_kernel, dim3(blocks), dim3(256), 0, stream, sets, shape.count,
+                           shape.total_prep, shape.metric, shape.operand, flags);
     if ((shape.operand == kOperandBf16 || shape.operand == kOperandF16Local) && shape.max_prep_blocks > 0)
         hipLaunchKernelGGL(prepare_rows_kernel, dim3(shape.max_prep_blocks, 1, shape.count), dim3(256), 0,
                            stream, sets, shape.metric, flags);

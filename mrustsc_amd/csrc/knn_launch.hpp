@@ -217,6 +217,7 @@ struct BatchShape {
     uint32_t max_d;           // largest d
     uint32_t max_prep_blocks; // max ceil(ref_rows / 64) over datasets on the bf16 pass
     uint32_t max_local_blocks; // max npad / kLocalRows over datasets on the local-origin pass
+    uint32_t direct_sets;      // datasets prepare_kernel serves (not on a packed 16-bit pass)
     int metric;
     int operand;
 };

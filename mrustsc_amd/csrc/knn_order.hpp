@@ -12,14 +12,14 @@ constexpr size_t kOrderMinCells = 65536;
 
 // Byte offsets of the ordering workspace, relative to its base.
 struct OrderLayout {
-    size_t cells, cells_sorted, rows, temp, temp_bytes, total;
-    uint32_t pivots, bits, dims;
+    size_t cells, counts, totals, total;
+    uint32_t pivots, dims, chunks;
 };
 
 OrderLayout order_layout(size_t n, size_t d);
 
-// Enqueues the cell assignment, a stable radix sort of the rows by cell, and the inverse
-// permutation: perm[p] = input row at pass position p, inv_perm[row] = its position.
+// Enqueues the cell assignment and a stable counting sort of the rows by cell, which writes
+// perm[p] = input row at pass position p and inv_perm[row] = its position.
 hipError_t order_enqueue(const float* x, size_t n, size_t d, void* workspace,
                          const OrderLayout& layout, uint32_t* perm, uint32_t* inv_perm,
                          hipStream_t stream);

@@ -363,6 +363,8 @@ scrust_hip_status plan_layout(const std::vector<SetSpec>& specs,
         if (pass && (plan.operand == kOperandBf16 || plan.operand == kOperandF16Local))
             plan.shape.max_prep_blocks = std::max<uint32_t>(plan.shape.max_prep_blocks,
                                                             static_cast<uint32_t>((ref_rows + 63) / 64));
+        else
+            plan.shape.direct_sets++;
         if (pass && s.local)
             plan.shape.max_local_blocks = std::max<uint32_t>(plan.shape.max_local_blocks, s.npad / kLocalRows);
         row += s.n;
