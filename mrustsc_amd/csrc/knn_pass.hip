@@ -1620,7 +1620,7 @@ void launch_select(const DatasetDesc* sets, const BatchShape& shape, unsigned wi
 // ---------------------------------------------------------------------------
 
 static const int kBf16Chunks[] = {1, 2, 3, 4, 5, 6, 8, 10, 13};
-static const int kF32Chunks[] = {1, 2, 3, 4, 6, 8, 10, 13, 16, 20, 26, 33, 49, 65};
+static const int kF32Chunks[] = {1, 2, 3, 4, 6, 8, 10, 13, 16, 20, 26, 33, 49, 57, 65};
 
 int bf16_chunks(int d) {
     const int need = (3 * d + 3 + 31) / 32;
@@ -1713,7 +1713,8 @@ bool launch_pass(const DatasetDesc* set, const DatasetDesc& host, int operand,
         SCRUST_PASS_CASE(kOperandF32, 10) SCRUST_PASS_CASE(kOperandF32, 13)
         SCRUST_PASS_CASE(kOperandF32, 16) SCRUST_PASS_CASE(kOperandF32, 20)
         SCRUST_PASS_CASE(kOperandF32, 26) SCRUST_PASS_CASE(kOperandF32, 33)
-        SCRUST_PASS_CASE(kOperandF32, 49) SCRUST_PASS_CASE(kOperandF32, 65)
+        SCRUST_PASS_CASE(kOperandF32, 49) SCRUST_PASS_CASE(kOperandF32, 57)
+        SCRUST_PASS_CASE(kOperandF32, 65)
         default: return false;
     }
 }

@@ -430,13 +430,14 @@ def test_short_lists_of_the_wide_candidate_passes_match_the_oracle(context):
 
 def test_rows_of_129_to_256_features_take_the_fp32_pass_and_match_the_oracle(context):
     # The packed 16-bit operands hold up to 128 features; single builds of 129-256 take the
-    # fp32 pass (49 or 65 chunks) whatever precision was asked, past 256 the exact kernel.
+    # fp32 pass (49, 57 or 65 chunks) whatever precision was asked, past 256 the exact kernel.
     # Both sides of each chunk boundary, one-round and multi-round builds (one range on the
     # spatial order, with the ball bound), both metrics, k across the candidate tiers.
     cases = [(3000, 129, 15, "euclidean", "bf16"), (5000, 150, 30, "cosine", "fp16"),
              (20000, 192, 15, "euclidean", "fp32"), (9000, 193, 100, "euclidean", "bf16"),
              (30000, 200, 5, "cosine", "bf16"), (7000, 256, 15, "euclidean", "bf16"),
              (4000, 256, 200, "euclidean", "fp32"), (2500, 257, 15, "euclidean", "bf16"),
+             (6000, 224, 15, "euclidean", "bf16"), (6000, 225, 30, "cosine", "bf16"),
              (150000, 200, 15, "euclidean", "bf16"), (140000, 256, 30, "cosine", "bf16")]
     rng = np.random.default_rng(129)
     for n, d, k, metric, precision in cases:
